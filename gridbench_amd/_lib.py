@@ -1,0 +1,136 @@
+"""ctypes binding of libgbx.so (the C-ABI declared in include/gbx.h).
+
+The library is loaded from this package directory, where
+`gridbench_amd/csrc/Makefile` builds it.  There is no fallback: if the
+library is missing, importing this module raises.
+"""
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgbx.so")
+
+if not os.path.exists(LIB_PATH):
+    raise ImportError(
+        f"{LIB_PATH} is missing: build it with `make -C gridbench_amd/csrc` "
+        "or __graft_entry__.build()")
+
+lib = C.CDLL(LIB_PATH)
+
+OK, EINVAL, ENULL, EALIAS, ETABLE, ENOMEM, EHIP, ENODEV, EUNSUPPORTED, EMISMATCH, ENOCONV = range(11)
+F64, F32, F16, BF16 = 0, 1, 2, 3
+PREC_NAMES = {F64: "f64", F32: "f32", F16: "f16", BF16: "bf16"}
+PREC_BY_NAME = {v: k for k, v in PREC_NAMES.items()}
+
+
+class GbxError(RuntimeError):
+    """A libgbx call returned a non-zero status."""
+
+    def __init__(self, status, where):
+        self.status = status
+        msg = lib.gbx_last_error().decode()
+        super().__init__(f"{where}: {lib.gbx_strerror(status).decode()} ({status}): {msg}")
+
+
+def check(status, where="libgbx"):
+    if status != OK:
+        raise GbxError(status, where)
+
+
+class KCfg(C.Structure):
+    _fields_ = [("spw", C.c_int), ("threads", C.c_int), ("xcd", C.c_int), ("nt", C.c_int), ("kind", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class CGParams(C.Structure):
+    _fields_ = [("kappa", C.c_double), ("tol", C.c_double), ("inner_tol", C.c_double),
+                ("max_outer", C.c_int), ("max_inner", C.c_int)]
+
+
+class CGStats(C.Structure):
+    _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("true_resid", C.c_double),
+                ("seconds", C.c_double), ("hi_applies", C.c_int), ("lo_applies", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+vp = C.c_void_p
+i32, u64, dbl = C.c_int, C.c_uint64, C.c_double
+pi32, pu64, pdbl, pflt, pu8 = (C.POINTER(C.c_int), C.POINTER(C.c_uint64), C.POINTER(C.c_double),
+                               C.POINTER(C.c_float), C.POINTER(C.c_uint8))
+
+# name: (restype, argtypes) -- every function declared in include/gbx.h
+PROTOTYPES = {
+    "gbx_strerror": (C.c_char_p, [i32]),
+    "gbx_last_error": (C.c_char_p, []),
+    "gbx_version": (i32, []),
+    "gbx_device_count": (i32, []),
+    "gbx_dslash_kernel_f64": (i32, [i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_kernel_f32": (i32, [i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_last_path": (i32, []),
+    "gbx_table_generate": (i32, [pi32, i32, u64, pu64, pu8]),
+    "gbx_table_recognise": (i32, [i32, u64, u64, pu64, pu8, pi32]),
+    "gbx_gen_gaussian": (i32, [u64, u64, u64, u64, pdbl]),
+    "gbx_gen_su3_links": (i32, [pi32, u64, pdbl]),
+    "gbx_double_links": (i32, [pi32, pdbl, pdbl]),
+    "gbx_gen_spinor": (i32, [pi32, i32, i32, u64, pdbl]),
+    "gbx_geom_create": (i32, [pi32, pi32, C.POINTER(vp)]),
+    "gbx_geom_destroy": (i32, [vp]),
+    "gbx_geom_volume": (i32, [vp, pu64]),
+    "gbx_geom_stream": (i32, [vp, C.POINTER(vp)]),
+    "gbx_sync": (i32, [vp]),
+    "gbx_gauge_create": (i32, [vp, i32, i32, C.POINTER(vp)]),
+    "gbx_gauge_destroy": (i32, [vp]),
+    "gbx_gauge_upload": (i32, [vp, pdbl]),
+    "gbx_gauge_upload_ref_f64": (i32, [vp, pdbl, i32]),
+    "gbx_gauge_upload_ref_f32": (i32, [vp, pflt, i32]),
+    "gbx_spinor_create": (i32, [vp, i32, i32, i32, C.POINTER(vp)]),
+    "gbx_spinor_destroy": (i32, [vp]),
+    "gbx_spinor_bytes": (i32, [vp, pu64]),
+    "gbx_spinor_upload": (i32, [vp, pdbl]),
+    "gbx_spinor_download": (i32, [vp, pdbl]),
+    "gbx_spinor_upload_rhs": (i32, [vp, i32, pdbl]),
+    "gbx_spinor_download_rhs": (i32, [vp, i32, pdbl]),
+    "gbx_spinor_upload_ref_f64": (i32, [vp, pdbl, i32]),
+    "gbx_spinor_download_ref_f64": (i32, [vp, pdbl, i32]),
+    "gbx_spinor_upload_ref_f32": (i32, [vp, pflt, i32]),
+    "gbx_spinor_download_ref_f32": (i32, [vp, pflt, i32]),
+    "gbx_spinor_copy": (i32, [vp, vp]),
+    "gbx_dhop": (i32, [vp, vp, vp, i32]),
+    "gbx_dhop_axpby": (i32, [vp, vp, vp, i32, dbl, dbl]),
+    "gbx_dhop_cfg": (i32, [vp, vp, vp, i32, C.POINTER(KCfg)]),
+    "gbx_dhop_get_cfg": (i32, [vp, vp, C.POINTER(KCfg)]),
+    "gbx_dhop_time": (i32, [vp, vp, vp, i32, C.POINTER(KCfg), i32, pdbl]),
+    "gbx_tune": (i32, [vp, vp, vp, i32, C.c_char_p, C.POINTER(KCfg), pdbl]),
+    "gbx_spinor_dot": (i32, [vp, vp, pdbl, pdbl]),
+    "gbx_spinor_norm2": (i32, [vp, pdbl]),
+    "gbx_spinor_axpy": (i32, [dbl, vp, vp]),
+    "gbx_spinor_scale": (i32, [vp, dbl]),
+    "gbx_spinor_zero": (i32, [vp]),
+    "gbx_cg_solve": (i32, [vp, vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]),
+    "gbx_cg_solve_f64": (i32, [vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]),
+    "gbx_batch_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32, C.POINTER(vp)]),
+    "gbx_batch_destroy": (i32, [vp]),
+    "gbx_batch_run": (i32, [vp]),
+    "gbx_batch_time": (i32, [vp, i32, pdbl]),
+    "gbx_batch_info": (i32, [vp, pu64, pi32]),
+    "gbx_stream_copy_bw": (i32, [u64, i32, pdbl]),
+}
+
+for _name, (_res, _args) in PROTOTYPES.items():
+    _f = getattr(lib, _name)  # AttributeError here = the library lacks a declared symbol
+    _f.restype = _res
+    _f.argtypes = _args
+
+
+def ptr(a, ctype):
+    """ctypes pointer to a contiguous numpy array's data."""
+    assert a.flags["C_CONTIGUOUS"], "array must be C-contiguous"
+    return a.ctypes.data_as(C.POINTER(ctype))
+
+
+def ivec(values):
+    return (C.c_int * len(values))(*[int(v) for v in values])

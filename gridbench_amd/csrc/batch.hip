@@ -1,0 +1,237 @@
+// Batched hopping term over many independent lattices in one persistent,
+// HIP-graph-captured launch.
+//
+// Work list: every lattice's slot range is cut into items of `spw` sites
+// (spw*M close to one workgroup of work); items are ordered by decreasing
+// cost so the tail of the queue holds the small ones.  A persistent grid of
+// workgroups (a few per CU) pulls items from an atomic counter until the list
+// is exhausted -- every workgroup reaches that exit.  Per item the workgroup
+// stages the item's links into LDS and computes all (site, m) points, exactly
+// like the single-lattice kernel.  The counter is cleared by a one-thread
+// kernel node (no memset nodes in the graph).
+#include <algorithm>
+#include <vector>
+
+#include "dhop_dev.h"
+#include "gbx_internal.h"
+
+namespace {
+
+struct LatDesc {
+  const void *gauge;
+  const uint32_t *nbr;
+  const void *in;
+  void *out;
+  uint32_t M, nslot, c12, pad;
+};
+
+constexpr int kThreads = 256;
+
+__global__ void k_reset(uint32_t *c) { c[0] = 0u; }
+
+template <int P, bool DAG>
+__global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
+                                                         uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw,
+                                                         int nt)
+{
+  typedef typename dev::Traits<P>::acc A;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_idx;
+  A *sU = (A *)smem;
+  uint32_t *sN = (uint32_t *)(sU + (size_t)max_spw * 8 * 18);
+  for (;;) {
+    if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1u);
+    __syncthreads();
+    const uint32_t idx = s_idx;
+    if (idx >= nitems) break;  // same value in every thread of the workgroup
+    const uint4 it = items[idx];
+    const LatDesc L = lat[it.x];
+    const uint32_t q0 = it.y, nsl = it.z;
+    if (L.c12) dev::stage_links<P, true>(L.gauge, L.nbr, q0, nsl, sU, sN);
+    else dev::stage_links<P, false>(L.gauge, L.nbr, q0, nsl, sU, sN);
+    const uint32_t M = L.M, n = nsl * M;
+    for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
+      const uint32_t sl = w / M, m = w - sl * M;
+      A rr[12], ri[12];
+      dev::dhop_site<P, DAG>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+      dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
+    }
+    __syncthreads();  // LDS and s_idx are reused by the next item
+  }
+}
+
+}  // namespace
+
+struct gbx_batch_s {
+  int prec = 0, dag = 0, n = 0;
+  uint32_t nitems = 0, max_spw = 1;
+  int grid = 0;
+  hipStream_t stream = nullptr;
+  LatDesc *d_lat = nullptr;
+  uint4 *d_items = nullptr;
+  uint32_t *d_counter = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::vector<gbx_geom_s *> geoms;
+  ~gbx_batch_s()
+  {
+    if (stream) (void)hipStreamSynchronize(stream);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipFree(d_lat);
+    (void)hipFree(d_items);
+    (void)hipFree(d_counter);
+    if (stream) (void)hipStreamDestroy(stream);
+    for (gbx_geom_s *g : geoms) geom_release(g);
+  }
+};
+
+namespace {
+
+size_t lds_for(int prec, uint32_t spw) { return (size_t)spw * 8 * 18 * (prec == GBX_F64 ? 8 : 4) + (size_t)spw * 8 * 4; }
+
+template <int P>
+void launch(gbx_batch_s *b, hipStream_t st, int nt)
+{
+  const size_t lds = lds_for(P, b->max_spw);
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, b->d_counter);
+  if (b->dag)
+    hipLaunchKernelGGL((k_dhop_batch<P, true>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
+                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+  else
+    hipLaunchKernelGGL((k_dhop_batch<P, false>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
+                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+}
+
+void launch_any(gbx_batch_s *b, hipStream_t st)
+{
+  switch (b->prec) {
+  case GBX_F64: launch<GBX_F64>(b, st, 1); break;
+  case GBX_F32: launch<GBX_F32>(b, st, 1); break;
+  case GBX_F16: launch<GBX_F16>(b, st, 1); break;
+  default: launch<GBX_BF16>(b, st, 1); break;
+  }
+}
+
+}  // namespace
+
+extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag, gbx_batch *bp)
+{
+  GBX_CHECK(bp && u && out && in, GBX_ENULL, "gbx_batch_create: NULL argument");
+  *bp = nullptr;
+  GBX_CHECK(n >= 1, GBX_EINVAL, "gbx_batch_create: n=%d", n);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_batch_create: dag=%d", dag);
+  const int prec = u[0] ? u[0]->prec : -1;
+  std::vector<LatDesc> lat(n);
+  struct Item {
+    uint32_t lat, q0, nsl;
+    uint64_t cost;
+  };
+  std::vector<Item> items;
+  uint32_t max_spw = 1;
+  for (int i = 0; i < n; i++) {
+    GBX_CHECK(u[i] && out[i] && in[i], GBX_ENULL, "gbx_batch_create: NULL handle at %d", i);
+    GBX_CHECK(u[i]->geom == in[i]->geom && u[i]->geom == out[i]->geom, GBX_EMISMATCH, "gbx_batch_create: lattice %d mixes geometries", i);
+    GBX_CHECK(u[i]->prec == prec && in[i]->prec == prec && out[i]->prec == prec, GBX_EMISMATCH,
+              "gbx_batch_create: lattice %d precision differs from lattice 0", i);
+    GBX_CHECK(in[i]->M == out[i]->M && in[i]->d != out[i]->d, GBX_EMISMATCH, "gbx_batch_create: lattice %d in/out mismatch", i);
+    for (int j = 0; j < n; j++)
+      GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
+                "gbx_batch_create: output %d aliases another lattice's field", i);
+    const uint32_t M = in[i]->M, nslot = (uint32_t)u[i]->geom->vol;
+    uint32_t spw = (kThreads + M - 1) / M;
+    if (spw < 1) spw = 1;
+    if (spw > 64) spw = 64;
+    max_spw = std::max(max_spw, spw);
+    lat[i] = LatDesc{u[i]->d, u[i]->geom->nbr, in[i]->d, out[i]->d, M, nslot, (uint32_t)u[i]->comp12, 0};
+    for (uint32_t q0 = 0; q0 < nslot; q0 += spw) {
+      const uint32_t nsl = std::min(spw, nslot - q0);
+      items.push_back(Item{(uint32_t)i, q0, nsl, (uint64_t)nsl * M});
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
+  GBX_TRY(gbx_require_device());
+  gbx_batch_s *b = new gbx_batch_s();
+  b->prec = prec;
+  b->dag = dag;
+  b->n = n;
+  b->nitems = (uint32_t)items.size();
+  b->max_spw = max_spw;
+  for (int i = 0; i < n; i++) {
+    geom_retain(u[i]->geom);
+    b->geoms.push_back(u[i]->geom);
+  }
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  b->grid = ncu * (prec == GBX_F64 ? 4 : 6);
+  std::vector<uint4> hitems(items.size());
+  for (size_t k = 0; k < items.size(); k++) hitems[k] = make_uint4(items[k].lat, items[k].q0, items[k].nsl, 0);
+  auto fail = [&](int s) {
+    delete b;
+    return s;
+  };
+  if (hipStreamCreate(&b->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: stream"));
+  if (hipMalloc(&b->d_lat, sizeof(LatDesc) * n) != hipSuccess || hipMalloc(&b->d_items, sizeof(uint4) * hitems.size()) != hipSuccess ||
+      hipMalloc(&b->d_counter, 256) != hipSuccess)
+    return fail(gbx_fail(GBX_ENOMEM, "gbx_batch_create: device allocation"));
+  if (hipMemcpy(b->d_lat, lat.data(), sizeof(LatDesc) * n, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(b->d_items, hitems.data(), sizeof(uint4) * hitems.size(), hipMemcpyHostToDevice) != hipSuccess)
+    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: upload"));
+  // Inputs may still be in flight on their geometries' streams.
+  for (gbx_geom_s *g : b->geoms) (void)hipStreamSynchronize(g->stream);
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: begin capture"));
+  launch_any(b, b->stream);
+  if (hipStreamEndCapture(b->stream, &b->graph) != hipSuccess || !b->graph)
+    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: end capture"));
+  if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
+    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: instantiate"));
+  *bp = b;
+  return GBX_OK;
+}
+
+extern "C" int gbx_batch_destroy(gbx_batch b)
+{
+  delete b;
+  return GBX_OK;
+}
+
+extern "C" int gbx_batch_run(gbx_batch b)
+{
+  GBX_CHECK(b, GBX_ENULL, "gbx_batch_run: NULL batch");
+  for (gbx_geom_s *g : b->geoms) GBX_HIP(hipStreamSynchronize(g->stream));
+  GBX_HIP(hipGraphLaunch(b->exec, b->stream));
+  GBX_HIP(hipStreamSynchronize(b->stream));
+  return GBX_OK;
+}
+
+extern "C" int gbx_batch_time(gbx_batch b, int nrep, double *usec)
+{
+  GBX_CHECK(b && usec, GBX_ENULL, "gbx_batch_time: NULL argument");
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_batch_time: nrep=%d", nrep);
+  GBX_TRY(gbx_batch_run(b));  // warm-up
+  hipEvent_t e0, e1;
+  GBX_HIP(hipEventCreate(&e0));
+  GBX_HIP(hipEventCreate(&e1));
+  int rc = GBX_OK;
+  (void)hipEventRecord(e0, b->stream);
+  for (int r = 0; r < nrep && rc == GBX_OK; r++)
+    if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: graph launch");
+  (void)hipEventRecord(e1, b->stream);
+  if (hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: kernel failed");
+  float ms = 0.f;
+  (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK) *usec = 1000.0 * ms / nrep;
+  return rc;
+}
+
+extern "C" int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups)
+{
+  GBX_CHECK(b && items && workgroups, GBX_ENULL, "gbx_batch_info: NULL argument");
+  *items = b->nitems;
+  *workgroups = b->grid;
+  return GBX_OK;
+}

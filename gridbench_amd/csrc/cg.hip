@@ -1,0 +1,368 @@
+// Defect-correction conjugate gradient for  M^dagger M x = b,
+// M = 1 - kappa Dhop  (M^dagger = 1 - kappa Dhop^dagger).
+//
+// Outer loop, FP64 on the GPU:
+//   r = b - M^dag M x  (FP64 hopping kernels; the "true" residual)
+//   stop when |r| / |b| <= tol
+//   r_in = s * r with s = sqrt(N) / |r|  (unit-RMS entries: safe for FP16)
+//   e = inner_solve(M_lo^dag M_lo e = r_in) to |r_in - ...| <= inner_tol |r_in|
+//   x += e / s          (reliable update of x in FP64)
+// Inner CG: vectors e, r in FP32; p and A p in the low storage precision of
+// the inner gauge field (FP32, FP16 or BF16), which is what the low-precision
+// hopping kernel reads and writes; every arithmetic operation in FP32; dots
+// and norms in FP64 with deterministic reductions.  One iteration is two
+// fused M applications (out = in - kappa Dhop in), one dot, one fused
+// update-and-norm pass and one fused direction update.
+#include <chrono>
+#include <cmath>
+
+#include "dhop_dev.h"
+#include "gbx_internal.h"
+
+namespace {
+
+constexpr int kRB = 1024, kRT = 256;
+
+template <int P> struct Pt {
+  typedef typename dev::Traits<P>::acc A;
+  static constexpr int R = dev::Traits<P>::R, NCH = dev::Traits<P>::NCH;
+};
+
+template <int P, typename A>
+__device__ __forceinline__ void load_pt(const void *b, uint64_t q, uint32_t M, uint32_t m, A *v)
+{
+#pragma unroll
+  for (int k = 0; k < Pt<P>::NCH; k++) dev::load_chunk<P, A>(b, (q * Pt<P>::NCH + k) * M + m, v + k * Pt<P>::R);
+}
+
+template <int P, typename A>
+__device__ __forceinline__ void store_pt(void *b, uint64_t q, uint32_t M, uint32_t m, const A *v)
+{
+#pragma unroll
+  for (int k = 0; k < Pt<P>::NCH; k++) dev::store_chunk<P, A>(b, (q * Pt<P>::NCH + k) * M + m, v + k * Pt<P>::R, false);
+}
+
+__device__ __forceinline__ void bred(double &a, double *sh)
+{
+  sh[threadIdx.x] = a;
+  __syncthreads();
+  for (int s = kRT / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+    __syncthreads();
+  }
+  a = sh[0];
+}
+
+// e += a p; r -= a Ap; partial |r|^2.   (e, r: precision PE; p, Ap: PP)
+template <int PE, int PP>
+__global__ void __launch_bounds__(kRT) k_cg_update(double a, const void *__restrict__ p, const void *__restrict__ Ap,
+                                                   void *__restrict__ e, void *__restrict__ r, uint64_t npts, uint32_t M,
+                                                   double *__restrict__ partial)
+{
+  __shared__ double sh[kRT];
+  typedef typename Pt<PE>::A A;
+  double acc = 0.0;
+  for (uint64_t t = blockIdx.x * (uint64_t)kRT + threadIdx.x; t < npts; t += (uint64_t)gridDim.x * kRT) {
+    const uint64_t q = t / M;
+    const uint32_t m = (uint32_t)(t % M);
+    A pv[24], apv[24], ev[24], rv[24];
+    load_pt<PP, A>(p, q, M, m, pv);
+    load_pt<PP, A>(Ap, q, M, m, apv);
+    load_pt<PE, A>(e, q, M, m, ev);
+    load_pt<PE, A>(r, q, M, m, rv);
+#pragma unroll
+    for (int j = 0; j < 24; j++) {
+      ev[j] += (A)a * pv[j];
+      rv[j] -= (A)a * apv[j];
+    }
+    store_pt<PE, A>(e, q, M, m, ev);
+    store_pt<PE, A>(r, q, M, m, rv);
+    // norm of the stored (rounded) residual
+    load_pt<PE, A>(r, q, M, m, rv);
+#pragma unroll
+    for (int j = 0; j < 24; j++) acc += (double)rv[j] * (double)rv[j];
+  }
+  bred(acc, sh);
+  if (threadIdx.x == 0) partial[blockIdx.x] = acc;
+}
+
+// p = r + b p
+template <int PE, int PP>
+__global__ void __launch_bounds__(kRT) k_cg_xpay(double b, const void *__restrict__ r, void *__restrict__ p, uint64_t npts,
+                                                 uint32_t M)
+{
+  typedef typename Pt<PE>::A A;
+  for (uint64_t t = blockIdx.x * (uint64_t)kRT + threadIdx.x; t < npts; t += (uint64_t)gridDim.x * kRT) {
+    const uint64_t q = t / M;
+    const uint32_t m = (uint32_t)(t % M);
+    A rv[24], pv[24];
+    load_pt<PE, A>(r, q, M, m, rv);
+    load_pt<PP, A>(p, q, M, m, pv);
+#pragma unroll
+    for (int j = 0; j < 24; j++) pv[j] = rv[j] + (A)b * pv[j];
+    store_pt<PP, A>(p, q, M, m, pv);
+  }
+}
+
+// z = a x + b y, all FP64 or same precision P (used for r = b - t)
+template <int P>
+__global__ void __launch_bounds__(kRT) k_axpby(double a, const void *__restrict__ x, double b, const void *__restrict__ y,
+                                               void *__restrict__ z, uint64_t nchunk)
+{
+  typedef typename Pt<P>::A A;
+  constexpr int R = Pt<P>::R;
+  for (uint64_t t = blockIdx.x * (uint64_t)kRT + threadIdx.x; t < nchunk; t += (uint64_t)gridDim.x * kRT) {
+    A xv[R], yv[R];
+    dev::load_chunk<P, A>(x, t, xv);
+    dev::load_chunk<P, A>(y, t, yv);
+#pragma unroll
+    for (int j = 0; j < R; j++) xv[j] = (A)a * xv[j] + (A)b * yv[j];
+    dev::store_chunk<P, A>(z, t, xv, false);
+  }
+}
+
+// x (FP64) += a e (precision PS)
+template <int PS>
+__global__ void __launch_bounds__(kRT) k_axpy_into_f64(double a, const void *__restrict__ e, void *__restrict__ x, uint64_t npts,
+                                                       uint32_t M)
+{
+  for (uint64_t t = blockIdx.x * (uint64_t)kRT + threadIdx.x; t < npts; t += (uint64_t)gridDim.x * kRT) {
+    const uint64_t q = t / M;
+    const uint32_t m = (uint32_t)(t % M);
+    double ev[24], xv[24];
+    load_pt<PS, double>(e, q, M, m, ev);
+    load_pt<GBX_F64, double>(x, q, M, m, xv);
+#pragma unroll
+    for (int j = 0; j < 24; j++) xv[j] += a * ev[j];
+    store_pt<GBX_F64, double>(x, q, M, m, xv);
+  }
+}
+
+__global__ void __launch_bounds__(kRT) k_final(const double *__restrict__ partial, int n, double *__restrict__ out)
+{
+  __shared__ double sh[kRT];
+  double a = 0.0;
+  for (int i = threadIdx.x; i < n; i += kRT) a += partial[i];
+  bred(a, sh);
+  if (threadIdx.x == 0) out[0] = a;
+}
+
+struct Scratch {
+  double *partial = nullptr, *res = nullptr, *host = nullptr;
+  ~Scratch()
+  {
+    (void)hipFree(partial);
+    (void)hipFree(res);
+    (void)hipHostFree(host);
+  }
+  int init()
+  {
+    GBX_HIP(hipMalloc(&partial, sizeof(double) * kRB));
+    GBX_HIP(hipMalloc(&res, sizeof(double)));
+    GBX_HIP(hipHostMalloc(&host, sizeof(double), hipHostMallocDefault));
+    return GBX_OK;
+  }
+  int fetch(hipStream_t st, double *v)
+  {
+    hipLaunchKernelGGL(k_final, dim3(1), dim3(kRT), 0, st, (const double *)partial, kRB, res);
+    GBX_HIP(hipMemcpyAsync(host, res, sizeof(double), hipMemcpyDeviceToHost, st));
+    GBX_HIP(hipStreamSynchronize(st));
+    *v = *host;
+    return GBX_OK;
+  }
+};
+
+// Owning handle for temporaries.
+struct Tmp {
+  gbx_spinor_s *s = nullptr;
+  ~Tmp() { gbx_spinor_destroy(s); }
+};
+
+int make(gbx_geom_s *g, int Ls, int prec, Tmp &t) { return gbx_spinor_create(g, Ls, 1, prec, &t.s); }
+
+template <int PE, int PP>
+void launch_update(double a, const gbx_spinor_s *p, const gbx_spinor_s *Ap, gbx_spinor_s *e, gbx_spinor_s *r, Scratch &sc,
+                   hipStream_t st)
+{
+  const uint64_t npts = r->geom->vol * (uint64_t)r->M;
+  hipLaunchKernelGGL((k_cg_update<PE, PP>), dim3(kRB), dim3(kRT), 0, st, a, p->d, Ap->d, e->d, r->d, npts, r->M, sc.partial);
+}
+
+template <int PE, int PP>
+void launch_xpay(double b, const gbx_spinor_s *r, gbx_spinor_s *p, hipStream_t st)
+{
+  const uint64_t npts = r->geom->vol * (uint64_t)r->M;
+  hipLaunchKernelGGL((k_cg_xpay<PE, PP>), dim3(kRB), dim3(kRT), 0, st, b, r->d, p->d, npts, r->M);
+}
+
+// y = M^dag M x through t (all of u's precision):
+// t = x - kappa Dhop x;  y = t - kappa Dhop^dag t.
+int apply_mdagm(const gbx_gauge_s *u, gbx_spinor_s *y, const gbx_spinor_s *x, gbx_spinor_s *t, double kappa, hipStream_t st)
+{
+  gbx_kcfg k;
+  gbx_dhop_get_cfg((gbx_gauge)u, (gbx_spinor)x, &k);
+  if (k.kind != 0) k = gbx_kcfg{dhop_default_cfg(u, x).spw, dhop_default_cfg(u, x).threads, 1, 0, 0};
+  dhop_cfg c{k.spw, k.threads, k.xcd, 0};
+  GBX_TRY(dhop_launch(u, t, x, 0, c, 1.0, -kappa, x, st));
+  GBX_TRY(dhop_launch(u, y, t, 1, c, 1.0, -kappa, t, st));
+  return GBX_OK;
+}
+
+// Inner CG on M^dag M e = r (e zero on entry, r overwritten).  Vectors e, r
+// precision PE; p, Ap, t precision PP (the gauge field's).
+template <int PE, int PP>
+int inner_cg(const gbx_gauge_s *u, gbx_spinor_s *e, gbx_spinor_s *r, gbx_spinor_s *p, gbx_spinor_s *Ap, gbx_spinor_s *t,
+             double kappa, double rel_tol, int max_iter, int *iters, int *applies, Scratch &sc, hipStream_t st)
+{
+  double rr;
+  GBX_TRY(spinor_norm2_on(r, &rr, st));
+  const double stop = rel_tol * rel_tol * rr;
+  GBX_TRY(spinor_zero_on(e, st));
+  GBX_TRY(spinor_convert_scaled(p, r, 1.0, st));
+  int it = 0;
+  while (it < max_iter && rr > stop) {
+    GBX_TRY(apply_mdagm(u, Ap, p, t, kappa, st));
+    (*applies)++;
+    double pap, dummy;
+    GBX_TRY(spinor_dot_on(p, Ap, &pap, &dummy, st));
+    if (!(pap > 0.0)) break;  // breakdown (rounding at the accuracy floor)
+    const double a = rr / pap;
+    launch_update<PE, PP>(a, p, Ap, e, r, sc, st);
+    GBX_HIP(hipGetLastError());
+    double rr_new;
+    GBX_TRY(sc.fetch(st, &rr_new));
+    const double b = rr_new / rr;
+    rr = rr_new;
+    launch_xpay<PE, PP>(b, r, p, st);
+    GBX_HIP(hipGetLastError());
+    it++;
+  }
+  *iters += it;
+  return GBX_OK;
+}
+
+template <int PE, int PP>
+int inner_dispatch_pp(const gbx_gauge_s *u, gbx_spinor_s *e, gbx_spinor_s *r, gbx_spinor_s *p, gbx_spinor_s *Ap,
+                      gbx_spinor_s *t, double kappa, double tol, int maxit, int *iters, int *applies, Scratch &sc,
+                      hipStream_t st)
+{
+  return inner_cg<PE, PP>(u, e, r, p, Ap, t, kappa, tol, maxit, iters, applies, sc, st);
+}
+
+int inner_dispatch(const gbx_gauge_s *u, gbx_spinor_s *e, gbx_spinor_s *r, gbx_spinor_s *p, gbx_spinor_s *Ap, gbx_spinor_s *t,
+                   double kappa, double tol, int maxit, int *iters, int *applies, Scratch &sc, hipStream_t st)
+{
+  if (e->prec == GBX_F64) return inner_dispatch_pp<GBX_F64, GBX_F64>(u, e, r, p, Ap, t, kappa, tol, maxit, iters, applies, sc, st);
+  switch (p->prec) {
+  case GBX_F32: return inner_dispatch_pp<GBX_F32, GBX_F32>(u, e, r, p, Ap, t, kappa, tol, maxit, iters, applies, sc, st);
+  case GBX_F16: return inner_dispatch_pp<GBX_F32, GBX_F16>(u, e, r, p, Ap, t, kappa, tol, maxit, iters, applies, sc, st);
+  default: return inner_dispatch_pp<GBX_F32, GBX_BF16>(u, e, r, p, Ap, t, kappa, tol, maxit, iters, applies, sc, st);
+  }
+}
+
+int axpby_f64(double a, const gbx_spinor_s *x, double b, const gbx_spinor_s *y, gbx_spinor_s *z, hipStream_t st)
+{
+  const uint64_t n = z->geom->vol * (uint64_t)z->M * 12;
+  hipLaunchKernelGGL(k_axpby<GBX_F64>, dim3(kRB), dim3(kRT), 0, st, a, x->d, b, y->d, z->d, n);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+int axpy_into_f64(double a, const gbx_spinor_s *e, gbx_spinor_s *x, hipStream_t st)
+{
+  const uint64_t npts = x->geom->vol * (uint64_t)x->M;
+  switch (e->prec) {
+  case GBX_F64: hipLaunchKernelGGL(k_axpy_into_f64<GBX_F64>, dim3(kRB), dim3(kRT), 0, st, a, e->d, x->d, npts, x->M); break;
+  case GBX_F32: hipLaunchKernelGGL(k_axpy_into_f64<GBX_F32>, dim3(kRB), dim3(kRT), 0, st, a, e->d, x->d, npts, x->M); break;
+  case GBX_F16: hipLaunchKernelGGL(k_axpy_into_f64<GBX_F16>, dim3(kRB), dim3(kRT), 0, st, a, e->d, x->d, npts, x->M); break;
+  default: hipLaunchKernelGGL(k_axpy_into_f64<GBX_BF16>, dim3(kRB), dim3(kRT), 0, st, a, e->d, x->d, npts, x->M); break;
+  }
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+int check_args(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, const gbx_spinor_s *x, const gbx_spinor_s *b,
+               const gbx_cg_params *p)
+{
+  GBX_CHECK(u_hi && x && b && p, GBX_ENULL, "cg: NULL argument");
+  GBX_CHECK(u_hi->prec == GBX_F64, GBX_EINVAL, "cg: the outer gauge field must be FP64");
+  GBX_CHECK(x->prec == GBX_F64 && b->prec == GBX_F64, GBX_EINVAL, "cg: x and b must be FP64");
+  GBX_CHECK(x->nrhs == 1 && b->nrhs == 1, GBX_EINVAL, "cg: one right-hand side");
+  GBX_CHECK(x->geom == u_hi->geom && b->geom == u_hi->geom && x->Ls == b->Ls, GBX_EMISMATCH, "cg: shapes differ");
+  GBX_CHECK(x->d != b->d, GBX_EALIAS, "cg: x aliases b");
+  if (u_lo) {
+    GBX_CHECK(u_lo->geom == u_hi->geom, GBX_EMISMATCH, "cg: inner gauge field on another geometry");
+    GBX_CHECK(u_lo->prec != GBX_F64, GBX_EINVAL, "cg: the inner gauge field must be FP32, FP16 or BF16");
+  }
+  GBX_CHECK(std::fabs(p->kappa) < 0.125, GBX_EINVAL, "cg: |kappa| = %g must be < 1/8", p->kappa);
+  GBX_CHECK(p->tol > 0 && p->inner_tol > 0 && p->inner_tol < 1 && p->max_outer >= 1 && p->max_inner >= 1, GBX_EINVAL,
+            "cg: bad tolerances or iteration limits");
+  return GBX_OK;
+}
+
+int solve(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, gbx_spinor_s *x, const gbx_spinor_s *b, const gbx_cg_params *prm,
+          gbx_cg_stats *stats)
+{
+  GBX_TRY(check_args(u_hi, u_lo, x, b, prm));
+  const auto t0 = std::chrono::steady_clock::now();
+  gbx_cg_stats st{};
+  gbx_geom_s *g = u_hi->geom;
+  hipStream_t s = g->stream;
+  const int Ls = x->Ls;
+  const int plo = u_lo ? u_lo->prec : GBX_F64;
+  const int pe = u_lo ? GBX_F32 : GBX_F64;
+  Tmp r, t, y, e, ri, p, Ap, tl;
+  Scratch sc;
+  GBX_TRY(sc.init());
+  GBX_TRY(make(g, Ls, GBX_F64, r));
+  GBX_TRY(make(g, Ls, GBX_F64, t));
+  GBX_TRY(make(g, Ls, GBX_F64, y));
+  GBX_TRY(make(g, Ls, pe, e));
+  GBX_TRY(make(g, Ls, pe, ri));
+  GBX_TRY(make(g, Ls, plo, p));
+  GBX_TRY(make(g, Ls, plo, Ap));
+  GBX_TRY(make(g, Ls, plo, tl));
+  const gbx_gauge_s *uin = u_lo ? u_lo : u_hi;
+  double bb;
+  GBX_TRY(spinor_norm2_on(b, &bb, s));
+  GBX_CHECK(bb > 0.0, GBX_EINVAL, "cg: b is zero");
+  const double bnorm = std::sqrt(bb);
+  const double N = (double)g->vol * Ls * 24;
+  double rel = 1.0;
+  for (;;) {
+    // true residual r = b - M^dag M x in FP64
+    GBX_TRY(apply_mdagm(u_hi, y.s, x, t.s, prm->kappa, s));
+    st.hi_applies++;
+    GBX_TRY(axpby_f64(1.0, b, -1.0, y.s, r.s, s));
+    double rr;
+    GBX_TRY(spinor_norm2_on(r.s, &rr, s));
+    rel = std::sqrt(rr) / bnorm;
+    if (rel <= prm->tol || st.outer_iters >= prm->max_outer || !(rr > 0.0)) break;
+    const double scale = std::sqrt(N) / std::sqrt(rr);
+    GBX_TRY(spinor_convert_scaled(ri.s, r.s, scale, s));
+    GBX_TRY(inner_dispatch(uin, e.s, ri.s, p.s, Ap.s, tl.s, prm->kappa, prm->inner_tol, prm->max_inner, &st.inner_iters,
+                           &st.lo_applies, sc, s));
+    GBX_TRY(axpy_into_f64(1.0 / scale, e.s, x, s));
+    st.outer_iters++;
+  }
+  GBX_HIP(hipStreamSynchronize(s));
+  st.true_resid = rel;
+  st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) *stats = st;
+  if (rel > prm->tol) return gbx_fail(GBX_ENOCONV, "cg: relative residual %.3e after %d outer iterations", rel, st.outer_iters);
+  return GBX_OK;
+}
+
+}  // namespace
+
+extern "C" int gbx_cg_solve(gbx_gauge u_hi, gbx_gauge u_lo, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p,
+                            gbx_cg_stats *st)
+{
+  GBX_CHECK(u_lo, GBX_ENULL, "gbx_cg_solve: NULL inner gauge field (use gbx_cg_solve_f64)");
+  return solve(u_hi, u_lo, x, b, p, st);
+}
+
+extern "C" int gbx_cg_solve_f64(gbx_gauge u, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p, gbx_cg_stats *st)
+{
+  return solve(u, nullptr, x, b, p, st);
+}

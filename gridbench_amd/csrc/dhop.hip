@@ -1,0 +1,294 @@
+// The VALU hopping-term kernel over the SoA layout and its C-ABI entries.
+//
+// Workgroup = `spw` consecutive slots x all M = Ls*nrhs minor indices.  The
+// 8 links of each of its sites are read from HBM once, converted to the
+// arithmetic precision (and rebuilt from 12 reals when compressed) into LDS,
+// and applied from LDS to every s-slice and right-hand side of the site.  A
+// thread computes one (site, m) 5D point: 8 neighbour gathers of 16-byte
+// chunks (a wavefront of consecutive m reads contiguous 1 KiB runs), 8 x
+// (project, SU(3) x 2 colour vectors, reconstruct) in registers, one store.
+// With xcd = 1 workgroup b runs logical block (b % 8) * ceil(nwg/8) + b / 8:
+// the hardware deals workgroups round-robin over the 8 XCDs, so each XCD
+// then walks one contiguous eighth of the slot range and its L2 sees the
+// spatially compact neighbourhoods the slot order provides.
+#include <mutex>
+#include <unordered_map>
+
+#include "dhop_dev.h"
+#include "gbx_internal.h"
+
+namespace {
+
+template <int P, bool DAG, bool C12, bool AXPBY>
+__global__ void __launch_bounds__(1024) k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr,
+                                               const void *__restrict__ in, void *__restrict__ out,
+                                               const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw,
+                                               uint32_t nwg, uint32_t per_xcd, int xcd, int nt,
+                                               typename dev::Traits<P>::acc alpha, typename dev::Traits<P>::acc beta)
+{
+  typedef typename dev::Traits<P>::acc A;
+  extern __shared__ __align__(16) unsigned char smem[];
+  A *sU = (A *)smem;
+  uint32_t *sN = (uint32_t *)(sU + (size_t)spw * 8 * 18);
+
+  const uint32_t L = xcd ? (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (L >= nwg) return;  // uniform over the workgroup, before any barrier
+  const uint32_t q0 = L * spw;
+  const uint32_t nsl = min(spw, nslot - q0);
+  dev::stage_links<P, C12>(gauge, nbr, q0, nsl, sU, sN);
+
+  const uint32_t items = nsl * M;
+  for (uint32_t w = threadIdx.x; w < items; w += blockDim.x) {
+    const uint32_t sl = w / M, m = w - sl * M;
+    A rr[12], ri[12];
+    dev::dhop_site<P, DAG>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    dev::store_site<P, AXPBY>(out, center, q0 + sl, M, m, rr, ri, alpha, beta, nt != 0);
+  }
+}
+
+template <int P, bool DAG, bool C12, bool AXPBY>
+void launch_t(dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge, const uint32_t *nbr, const void *in,
+              void *out, const void *center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t nwg, uint32_t per_xcd,
+              int xcd, int nt, double alpha, double beta)
+{
+  typedef typename dev::Traits<P>::acc A;
+  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY>), grid, block, lds, st, gauge, nbr, in, out, center, nslot, M, spw, nwg,
+                     per_xcd, xcd, nt, (A)alpha, (A)beta);
+}
+
+template <int P>
+void launch_p(bool dag, bool c12, bool axpby, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge,
+              const uint32_t *nbr, const void *in, void *out, const void *center, uint32_t nslot, uint32_t M, uint32_t spw,
+              uint32_t nwg, uint32_t per_xcd, int xcd, int nt, double alpha, double beta)
+{
+#define GBX_L(D, C, X) \
+  launch_t<P, D, C, X>(grid, block, lds, st, gauge, nbr, in, out, center, nslot, M, spw, nwg, per_xcd, xcd, nt, alpha, beta)
+  if (!dag && !c12 && !axpby) GBX_L(false, false, false);
+  else if (!dag && !c12 && axpby) GBX_L(false, false, true);
+  else if (!dag && c12 && !axpby) GBX_L(false, true, false);
+  else if (!dag && c12 && axpby) GBX_L(false, true, true);
+  else if (dag && !c12 && !axpby) GBX_L(true, false, false);
+  else if (dag && !c12 && axpby) GBX_L(true, false, true);
+  else if (dag && c12 && !axpby) GBX_L(true, true, false);
+  else GBX_L(true, true, true);
+#undef GBX_L
+}
+
+size_t lds_bytes(int prec, int spw) { return (size_t)spw * 8 * 18 * (prec == GBX_F64 ? 8 : 4) + (size_t)spw * 8 * 4; }
+
+// Tuned configurations keyed by problem signature.
+std::mutex g_cfg_mu;
+std::unordered_map<std::string, gbx_kcfg> g_cfg;
+
+}  // namespace
+
+std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
+{
+  char buf[160];
+  const gbx_geom_s *g = u->geom;
+  snprintf(buf, sizeof buf, "p%d_c%d_Ls%d_n%d_%dx%dx%dx%d_b%dx%dx%dx%d", in->prec, u->comp12, in->Ls, in->nrhs, g->dims[0],
+           g->dims[1], g->dims[2], g->dims[3], g->block[0], g->block[1], g->block[2], g->block[3]);
+  return buf;
+}
+
+void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c)
+{
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  g_cfg[sig] = c;
+}
+
+bool dhop_lookup_cfg(const std::string &sig, gbx_kcfg *c)
+{
+  std::lock_guard<std::mutex> lk(g_cfg_mu);
+  auto it = g_cfg.find(sig);
+  if (it == g_cfg.end()) return false;
+  *c = it->second;
+  return true;
+}
+
+dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in)
+{
+  (void)u;
+  dhop_cfg c;
+  const uint32_t M = in->M;
+  // LDS per workgroup stays <= 40 KB so several workgroups share a CU.
+  const int spw_max = in->prec == GBX_F64 ? 32 : 64;
+  // Prefer a workgroup of exactly spw*M threads, a multiple of 64, near 256.
+  int spw = 0;
+  for (int s = 1; s <= spw_max; s++) {
+    const uint32_t t = (uint32_t)s * M;
+    if (t > 1024) break;
+    if (t % 64 == 0 && t >= 256) {
+      spw = s;
+      break;
+    }
+  }
+  if (spw) {
+    c.spw = spw;
+    c.threads = (int)(spw * M);
+  } else if (M >= 256) {
+    c.spw = 1;
+    c.threads = 256;
+  } else {
+    c.spw = (int)((256 + M - 1) / M);
+    if (c.spw > spw_max) c.spw = spw_max;
+    c.threads = (int)(((uint32_t)c.spw * M + 63) / 64 * 64);
+    if (c.threads > 1024) c.threads = 1024;
+  }
+  c.xcd = 1;
+  c.nt = 1;
+  return c;
+}
+
+int dhop_cfg_check(const dhop_cfg &c, const gbx_gauge_s *u, const gbx_spinor_s *in)
+{
+  (void)in;
+  GBX_CHECK(c.spw >= 1 && c.spw <= 4096, GBX_EINVAL, "kernel config: spw=%d", c.spw);
+  GBX_CHECK(c.threads >= 64 && c.threads <= 1024 && c.threads % 64 == 0, GBX_EINVAL, "kernel config: threads=%d", c.threads);
+  GBX_CHECK(lds_bytes(u->prec, c.spw) <= 160 * 1024, GBX_EINVAL, "kernel config: spw=%d needs %zu B of LDS", c.spw,
+            lds_bytes(u->prec, c.spw));
+  return GBX_OK;
+}
+
+static int check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const gbx_spinor_s *in)
+{
+  GBX_CHECK(u && out && in, GBX_ENULL, "dhop: NULL handle");
+  GBX_CHECK(u->d && out->d && in->d, GBX_ENULL, "dhop: unallocated field");
+  GBX_CHECK(out != in && out->d != in->d, GBX_EALIAS, "dhop: out aliases in");
+  GBX_CHECK(u->geom == out->geom && u->geom == in->geom, GBX_EMISMATCH, "dhop: fields on different geometries");
+  GBX_CHECK(out->Ls == in->Ls && out->nrhs == in->nrhs, GBX_EMISMATCH, "dhop: in/out shapes differ");
+  GBX_CHECK(out->prec == in->prec && u->prec == in->prec, GBX_EMISMATCH,
+            "dhop: precisions differ (gauge %d, in %d, out %d)", u->prec, in->prec, out->prec);
+  return GBX_OK;
+}
+
+int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const dhop_cfg &c, double alpha,
+                double beta, const gbx_spinor_s *center, hipStream_t st)
+{
+  GBX_TRY(check_operands(u, out, in));
+  GBX_TRY(dhop_cfg_check(c, u, in));
+  const bool axpby = !(alpha == 0.0 && beta == 1.0);
+  if (axpby) {
+    GBX_CHECK(center && center->d, GBX_ENULL, "dhop_axpby: NULL center field");
+    GBX_CHECK(center->d != out->d, GBX_EALIAS, "dhop_axpby: out aliases the center field");
+    GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == in->geom, GBX_EMISMATCH,
+              "dhop_axpby: center field shape differs");
+  }
+  const uint32_t nslot = (uint32_t)u->geom->vol;
+  const uint32_t nwg = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
+  const uint32_t per_xcd = (nwg + 7) / 8;
+  const dim3 grid(c.xcd ? per_xcd * 8 : nwg), block((unsigned)c.threads);
+  const size_t lds = lds_bytes(u->prec, c.spw);
+  const void *cen = axpby ? center->d : nullptr;
+  switch (u->prec) {
+  case GBX_F64:
+    launch_p<GBX_F64>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
+                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
+    break;
+  case GBX_F32:
+    launch_p<GBX_F32>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
+                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
+    break;
+  case GBX_F16:
+    launch_p<GBX_F16>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
+                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
+    break;
+  default:
+    launch_p<GBX_BF16>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
+                       (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
+    break;
+  }
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+// ------------------------------------------------------------------- C-ABI
+int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in);
+int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st);
+
+static gbx_kcfg to_kcfg(const dhop_cfg &c)
+{
+  gbx_kcfg k;
+  k.spw = c.spw;
+  k.threads = c.threads;
+  k.xcd = c.xcd;
+  k.nt = c.nt;
+  k.kind = 0;
+  return k;
+}
+
+static dhop_cfg from_kcfg(const gbx_kcfg &k)
+{
+  dhop_cfg c;
+  c.spw = k.spw;
+  c.threads = k.threads;
+  c.xcd = k.xcd;
+  c.nt = k.nt;
+  return c;
+}
+
+int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st)
+{
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dhop: dag must be 0 or 1");
+  if (k.kind == 1) return mfma_launch(u, out, in, dag, k, st);
+  GBX_CHECK(k.kind == 0, GBX_EINVAL, "dhop: kernel kind %d", k.kind);
+  return dhop_launch(u, out, in, dag, from_kcfg(k), 0.0, 1.0, nullptr, st);
+}
+
+extern "C" int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg)
+{
+  GBX_CHECK(u && in && cfg, GBX_ENULL, "gbx_dhop_get_cfg: NULL argument");
+  if (!dhop_lookup_cfg(dhop_signature(u, in), cfg)) *cfg = to_kcfg(dhop_default_cfg(u, in));
+  return GBX_OK;
+}
+
+extern "C" int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg)
+{
+  GBX_CHECK(cfg, GBX_ENULL, "gbx_dhop_cfg: NULL config");
+  GBX_TRY(check_operands(u, out, in));
+  return dhop_run(u, out, in, dag, *cfg, u->geom->stream);
+}
+
+extern "C" int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag)
+{
+  GBX_TRY(check_operands(u, out, in));
+  gbx_kcfg k;
+  gbx_dhop_get_cfg(u, in, &k);
+  return dhop_run(u, out, in, dag, k, u->geom->stream);
+}
+
+extern "C" int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta)
+{
+  GBX_TRY(check_operands(u, out, in));
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_dhop_axpby: dag must be 0 or 1");
+  gbx_kcfg k;
+  gbx_dhop_get_cfg(u, in, &k);
+  if (k.kind != 0) k = to_kcfg(dhop_default_cfg(u, in));
+  return dhop_launch(u, out, in, dag, from_kcfg(k), alpha, beta, in, u->geom->stream);
+}
+
+extern "C" int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg, int nrep, double *usec)
+{
+  GBX_CHECK(usec, GBX_ENULL, "gbx_dhop_time: NULL usec");
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_dhop_time: nrep=%d", nrep);
+  GBX_TRY(check_operands(u, out, in));
+  gbx_kcfg k;
+  if (cfg) k = *cfg;
+  else gbx_dhop_get_cfg(u, in, &k);
+  hipStream_t st = u->geom->stream;
+  GBX_TRY(dhop_run(u, out, in, dag, k, st));  // warm-up
+  hipEvent_t e0, e1;
+  GBX_HIP(hipEventCreate(&e0));
+  GBX_HIP(hipEventCreate(&e1));
+  int rc = GBX_OK;
+  if (hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+  for (int r = 0; r < nrep && rc == GBX_OK; r++) rc = dhop_run(u, out, in, dag, k, st);
+  if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+  if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventSynchronize failed");
+  float ms = 0.f;
+  if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK) *usec = 1000.0 * (double)ms / nrep;
+  return rc;
+}

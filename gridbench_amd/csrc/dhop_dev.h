@@ -1,0 +1,230 @@
+// Device building blocks of the hopping term: spin projection, SU(3) x
+// half-spinor, reconstruction.  Operator and leg order as the reference Hand
+// kernel (WilsonKernelsHandCpu.h:426-446): leg d reads the neighbour in
+// direction d (Xp Yp Zp Tp Xm Ym Zm Tm) and link U[x][d]; non-dagger uses the
+// projector *M_PROJ on forward legs and *P_PROJ on backward legs
+// (WilsonKernelsHandCpu.h:94-159, 165-303); the dagger swaps them.
+#pragma once
+
+#include "layout_dev.h"
+
+namespace dev {
+
+enum { PXP = 0, PYP, PZP, PTP, PXM, PYM, PZM, PTM };
+
+// Projector kind used on leg `dir`.
+template <bool DAG, int DIR> struct LegProj {
+  static constexpr int value = DAG ? DIR : (DIR ^ 4);
+};
+
+// v[2*(sp*3+c) + {0,1}] = re/im of spin sp colour c of the neighbour spinor.
+// h[0..2] = upper half-spinor row 0, h[3..5] = row 1 (re in hr, im in hi).
+template <int K, typename A>
+__device__ __forceinline__ void project(const A *v, A *hr, A *hi)
+{
+#define PR(sp, c) v[2 * ((sp) * 3 + (c))]
+#define PI(sp, c) v[2 * ((sp) * 3 + (c)) + 1]
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (K == PXP) {  // h0 = p0 + i p3, h1 = p1 + i p2
+      hr[c] = PR(0, c) - PI(3, c); hi[c] = PI(0, c) + PR(3, c);
+      hr[3 + c] = PR(1, c) - PI(2, c); hi[3 + c] = PI(1, c) + PR(2, c);
+    } else if (K == PYP) {  // h0 = p0 - p3, h1 = p1 + p2
+      hr[c] = PR(0, c) - PR(3, c); hi[c] = PI(0, c) - PI(3, c);
+      hr[3 + c] = PR(1, c) + PR(2, c); hi[3 + c] = PI(1, c) + PI(2, c);
+    } else if (K == PZP) {  // h0 = p0 + i p2, h1 = p1 - i p3
+      hr[c] = PR(0, c) - PI(2, c); hi[c] = PI(0, c) + PR(2, c);
+      hr[3 + c] = PR(1, c) + PI(3, c); hi[3 + c] = PI(1, c) - PR(3, c);
+    } else if (K == PTP) {  // h0 = p0 + p2, h1 = p1 + p3
+      hr[c] = PR(0, c) + PR(2, c); hi[c] = PI(0, c) + PI(2, c);
+      hr[3 + c] = PR(1, c) + PR(3, c); hi[3 + c] = PI(1, c) + PI(3, c);
+    } else if (K == PXM) {  // h0 = p0 - i p3, h1 = p1 - i p2
+      hr[c] = PR(0, c) + PI(3, c); hi[c] = PI(0, c) - PR(3, c);
+      hr[3 + c] = PR(1, c) + PI(2, c); hi[3 + c] = PI(1, c) - PR(2, c);
+    } else if (K == PYM) {  // h0 = p0 + p3, h1 = p1 - p2
+      hr[c] = PR(0, c) + PR(3, c); hi[c] = PI(0, c) + PI(3, c);
+      hr[3 + c] = PR(1, c) - PR(2, c); hi[3 + c] = PI(1, c) - PI(2, c);
+    } else if (K == PZM) {  // h0 = p0 - i p2, h1 = p1 + i p3
+      hr[c] = PR(0, c) + PI(2, c); hi[c] = PI(0, c) - PR(2, c);
+      hr[3 + c] = PR(1, c) - PI(3, c); hi[3 + c] = PI(1, c) + PR(3, c);
+    } else {  // PTM: h0 = p0 - p2, h1 = p1 - p3
+      hr[c] = PR(0, c) - PR(2, c); hi[c] = PI(0, c) - PI(2, c);
+      hr[3 + c] = PR(1, c) - PR(3, c); hi[3 + c] = PI(1, c) - PI(3, c);
+    }
+  }
+#undef PR
+#undef PI
+}
+
+// u[sp*3+i] = sum_j U[i][j] h[sp*3+j]; U row-major complex, 18 reals.
+template <typename A>
+__device__ __forceinline__ void su3_mul(const A *U, const A *hr, const A *hi, A *ur, A *ui)
+{
+#pragma unroll
+  for (int sp = 0; sp < 2; sp++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      A re = U[(i * 3) * 2] * hr[sp * 3] - U[(i * 3) * 2 + 1] * hi[sp * 3];
+      A im = U[(i * 3) * 2] * hi[sp * 3] + U[(i * 3) * 2 + 1] * hr[sp * 3];
+#pragma unroll
+      for (int j = 1; j < 3; j++) {
+        re += U[(i * 3 + j) * 2] * hr[sp * 3 + j] - U[(i * 3 + j) * 2 + 1] * hi[sp * 3 + j];
+        im += U[(i * 3 + j) * 2] * hi[sp * 3 + j] + U[(i * 3 + j) * 2 + 1] * hr[sp * 3 + j];
+      }
+      ur[sp * 3 + i] = re;
+      ui[sp * 3 + i] = im;
+    }
+}
+
+// Accumulate (FIRST: assign) the reconstructed 4-spinor of kind K.
+// u0 = (ur[0..2], ui[0..2]), u1 = (ur[3..5], ui[3..5]); r index sp*3+c.
+template <int K, bool FIRST, typename A>
+__device__ __forceinline__ void recon(const A *ur, const A *ui, A *rr, A *ri)
+{
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    const A u0r = ur[c], u0i = ui[c], u1r = ur[3 + c], u1i = ui[3 + c];
+    if (FIRST) {
+      rr[c] = u0r; ri[c] = u0i;
+      rr[3 + c] = u1r; ri[3 + c] = u1i;
+      if (K == PXP) {  // r2 = -i u1, r3 = -i u0
+        rr[6 + c] = u1i; ri[6 + c] = -u1r;
+        rr[9 + c] = u0i; ri[9 + c] = -u0r;
+      } else {  // PXM: r2 = i u1, r3 = i u0
+        rr[6 + c] = -u1i; ri[6 + c] = u1r;
+        rr[9 + c] = -u0i; ri[9 + c] = u0r;
+      }
+      continue;
+    }
+    rr[c] += u0r; ri[c] += u0i;
+    rr[3 + c] += u1r; ri[3 + c] += u1i;
+    if (K == PXP) {  // r2 -= i u1, r3 -= i u0
+      rr[6 + c] += u1i; ri[6 + c] -= u1r;
+      rr[9 + c] += u0i; ri[9 + c] -= u0r;
+    } else if (K == PXM) {  // r2 += i u1, r3 += i u0
+      rr[6 + c] -= u1i; ri[6 + c] += u1r;
+      rr[9 + c] -= u0i; ri[9 + c] += u0r;
+    } else if (K == PYP) {  // r2 += u1, r3 -= u0
+      rr[6 + c] += u1r; ri[6 + c] += u1i;
+      rr[9 + c] -= u0r; ri[9 + c] -= u0i;
+    } else if (K == PYM) {  // r2 -= u1, r3 += u0
+      rr[6 + c] -= u1r; ri[6 + c] -= u1i;
+      rr[9 + c] += u0r; ri[9 + c] += u0i;
+    } else if (K == PZP) {  // r2 -= i u0, r3 += i u1
+      rr[6 + c] += u0i; ri[6 + c] -= u0r;
+      rr[9 + c] -= u1i; ri[9 + c] += u1r;
+    } else if (K == PZM) {  // r2 += i u0, r3 -= i u1
+      rr[6 + c] -= u0i; ri[6 + c] += u0r;
+      rr[9 + c] += u1i; ri[9 + c] -= u1r;
+    } else if (K == PTP) {  // r2 += u0, r3 += u1
+      rr[6 + c] += u0r; ri[6 + c] += u0i;
+      rr[9 + c] += u1r; ri[9 + c] += u1i;
+    } else {  // PTM: r2 -= u0, r3 -= u1
+      rr[6 + c] -= u0r; ri[6 + c] -= u0i;
+      rr[9 + c] -= u1r; ri[9 + c] -= u1i;
+    }
+  }
+}
+
+// Third row of an SU(3) matrix from the first two: row2 = conj(row0 x row1).
+template <typename A>
+__device__ __forceinline__ void su3_row2(A *U)
+{
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    const int a = (j + 1) % 3, b = (j + 2) % 3;
+    const A ar = U[a * 2], ai = U[a * 2 + 1], br = U[6 + b * 2], bi = U[6 + b * 2 + 1];
+    const A cr = U[b * 2], ci = U[b * 2 + 1], dr = U[6 + a * 2], di = U[6 + a * 2 + 1];
+    const A xr = (ar * br - ai * bi) - (cr * dr - ci * di);
+    const A xi = (ar * bi + ai * br) - (cr * di + ci * dr);
+    U[12 + j * 2] = xr;
+    U[12 + j * 2 + 1] = -xi;
+  }
+}
+
+// Stage the links (and neighbour slots) of workgroup sites q0 .. q0+nsl-1
+// into LDS as accumulation-precision 3x3 matrices [site][dir][18].
+template <int P, bool C12>
+__device__ __forceinline__ void stage_links(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, uint32_t q0,
+                                            uint32_t nsl, typename Traits<P>::acc *sU, uint32_t *sN)
+{
+  typedef typename Traits<P>::acc A;
+  typedef typename Traits<P>::st S;
+  constexpr int RL = C12 ? 12 : 18;
+  const S *g = (const S *)gauge + (uint64_t)q0 * 8 * RL;
+  const uint32_t n = nsl * 8 * RL;
+  for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
+    const uint32_t link = e / RL, k = e - link * RL;
+    sU[link * 18 + k] = (A)g[e];
+  }
+  for (uint32_t e = threadIdx.x; e < nsl * 8; e += blockDim.x) sN[e] = nbr[(uint64_t)q0 * 8 + e];
+  if (C12) {
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < nsl * 8; l += blockDim.x) su3_row2(sU + l * 18);
+  }
+  __syncthreads();
+}
+
+// One stencil leg for minor index m: gather neighbour `nb`, project,
+// multiply by U (18 reals in LDS), reconstruct into r.
+template <int P, int K, bool FIRST>
+__device__ __forceinline__ void leg(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
+                                    const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
+                                    typename Traits<P>::acc *ri)
+{
+  typedef typename Traits<P>::acc A;
+  constexpr int NCH = Traits<P>::NCH, R = Traits<P>::R;
+  A v[24];
+  const uint64_t base = (uint64_t)nb * NCH * M + m;
+#pragma unroll
+  for (int k = 0; k < NCH; k++) load_chunk<P, A>(in, base + (uint64_t)k * M, v + k * R);
+  A hr[6], hi[6], ur[6], ui[6];
+  project<K>(v, hr, hi);
+  su3_mul(U, hr, hi, ur, ui);
+  recon<K, FIRST>(ur, ui, rr, ri);
+}
+
+// Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
+template <int P, bool DAG>
+__device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
+                                          const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
+                                          typename Traits<P>::acc *ri)
+{
+  leg<P, LegProj<DAG, 0>::value, true>(in, nb[0], M, m, U + 0 * 18, rr, ri);
+  leg<P, LegProj<DAG, 1>::value, false>(in, nb[1], M, m, U + 1 * 18, rr, ri);
+  leg<P, LegProj<DAG, 2>::value, false>(in, nb[2], M, m, U + 2 * 18, rr, ri);
+  leg<P, LegProj<DAG, 3>::value, false>(in, nb[3], M, m, U + 3 * 18, rr, ri);
+  leg<P, LegProj<DAG, 4>::value, false>(in, nb[4], M, m, U + 4 * 18, rr, ri);
+  leg<P, LegProj<DAG, 5>::value, false>(in, nb[5], M, m, U + 5 * 18, rr, ri);
+  leg<P, LegProj<DAG, 6>::value, false>(in, nb[6], M, m, U + 6 * 18, rr, ri);
+  leg<P, LegProj<DAG, 7>::value, false>(in, nb[7], M, m, U + 7 * 18, rr, ri);
+}
+
+// Write the 4-spinor result (optionally out = alpha*center + beta*result).
+template <int P, bool AXPBY>
+__device__ __forceinline__ void store_site(void *__restrict__ out, const void *__restrict__ center, uint32_t q, uint32_t M,
+                                           uint32_t m, const typename Traits<P>::acc *rr, const typename Traits<P>::acc *ri,
+                                           typename Traits<P>::acc alpha, typename Traits<P>::acc beta, bool nt)
+{
+  typedef typename Traits<P>::acc A;
+  constexpr int NCH = Traits<P>::NCH, R = Traits<P>::R;
+  const uint64_t base = (uint64_t)q * NCH * M + m;
+#pragma unroll
+  for (int k = 0; k < NCH; k++) {
+    A v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const int r = k * R + j;
+      v[j] = (r & 1) ? ri[r >> 1] : rr[r >> 1];
+    }
+    if (AXPBY) {
+      A c[R];
+      load_chunk<P, A>(center, base + (uint64_t)k * M, c);
+#pragma unroll
+      for (int j = 0; j < R; j++) v[j] = alpha * c[j] + beta * v[j];
+    }
+    store_chunk<P, A>(out, base + (uint64_t)k * M, v, nt);
+  }
+}
+
+}  // namespace dev

@@ -1,0 +1,255 @@
+// Drop-in replacement for the reference entry point
+//   template<class Simd> double dslash_kernel(int nrep, Simd *U, Simd *out,
+//       Simd *in, uint64_t *nbr, uint64_t nsite, uint64_t Ls, uint8_t *prm)
+// (WilsonKernelsHand.h:17-29), on host arrays in the reference layout.
+//
+// Two device paths:
+//  * canonical tables (recognised by gbx_table_recognise): the fields are
+//    repacked into the SoA layout and the fast kernel (dhop.hip) runs;
+//  * any other valid tables: a table-driven kernel on the reference layout
+//    honours nbr/prm exactly: thread = (5D outer site, lane), neighbour lane
+//    = lane ^ (Nsimd >> (ptype+1)) when prm is set (the reference GPU
+//    kernel's gather form, WilsonKernelsHandGpu.h:33-41).
+// Timing follows the reference: nrep applications, the clock covering
+// applications 1..nrep-1 (WilsonKernelsHandCpu.h:407-450), measured here
+// with HIP events on the stream the kernels run on.
+#include <cstring>
+#include <mutex>
+
+#include "dhop_dev.h"
+#include "gbx_internal.h"
+
+int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
+int64_t table_first_invalid(const uint64_t *nbr, const uint8_t *prm, uint64_t n, uint64_t n5);
+
+namespace {
+
+thread_local int g_last_path = 0;
+
+template <typename T, bool DAG>
+__global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *__restrict__ out, const T *__restrict__ in,
+                                                    const uint64_t *__restrict__ nbr, const uint8_t *__restrict__ prm,
+                                                    uint64_t nsite, uint64_t Ls, int nsimd)
+{
+  const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  if (t >= nsite * Ls * (uint64_t)nsimd) return;
+  const int lane = (int)(t % (uint64_t)nsimd);
+  const uint64_t ss = t / (uint64_t)nsimd;
+  const uint64_t sU = ss / Ls;
+  T rr[12], ri[12];
+  // leg order and projector pairs as dhop_site (WilsonKernelsHandCpu.h:435-442)
+#define GBX_TLEG(DIR, PTYPE, FIRST)                                                                                      \
+  {                                                                                                                      \
+    const uint64_t off = nbr[ss * 8 + DIR];                                                                              \
+    const int plane = prm[ss * 8 + DIR] ? (lane ^ (nsimd >> (PTYPE + 1))) : lane;                                        \
+    T v[24];                                                                                                             \
+    _Pragma("unroll") for (int sc = 0; sc < 12; sc++)                                                                    \
+    {                                                                                                                    \
+      const T *p = in + ((off * 12 + sc) * (uint64_t)nsimd + plane) * 2;                                                 \
+      v[2 * sc] = p[0];                                                                                                  \
+      v[2 * sc + 1] = p[1];                                                                                              \
+    }                                                                                                                    \
+    T Ul[18];                                                                                                            \
+    _Pragma("unroll") for (int e = 0; e < 9; e++)                                                                        \
+    {                                                                                                                    \
+      const T *p = U + (((sU * 8 + DIR) * 9 + e) * (uint64_t)nsimd + lane) * 2;                                          \
+      Ul[2 * e] = p[0];                                                                                                  \
+      Ul[2 * e + 1] = p[1];                                                                                              \
+    }                                                                                                                    \
+    T hr[6], hi[6], ur[6], ui[6];                                                                                        \
+    dev::project<dev::LegProj<DAG, DIR>::value>(v, hr, hi);                                                              \
+    dev::su3_mul(Ul, hr, hi, ur, ui);                                                                                    \
+    dev::recon<dev::LegProj<DAG, DIR>::value, FIRST>(ur, ui, rr, ri);                                                    \
+  }
+  GBX_TLEG(0, 3, true)
+  GBX_TLEG(1, 2, false)
+  GBX_TLEG(2, 1, false)
+  GBX_TLEG(3, 0, false)
+  GBX_TLEG(4, 3, false)
+  GBX_TLEG(5, 2, false)
+  GBX_TLEG(6, 1, false)
+  GBX_TLEG(7, 0, false)
+#undef GBX_TLEG
+#pragma unroll
+  for (int sc = 0; sc < 12; sc++) {
+    T *p = out + ((ss * 12 + sc) * (uint64_t)nsimd + lane) * 2;
+    p[0] = rr[sc];
+    p[1] = ri[sc];
+  }
+}
+
+bool overlaps(const void *a, size_t na, const void *b, size_t nb)
+{
+  const uintptr_t a0 = (uintptr_t)a, b0 = (uintptr_t)b;
+  return a0 < b0 + nb && b0 < a0 + na;
+}
+
+// Device buffers of the table path and SoA objects of the fast path, cached
+// across calls with the same shape (the reference benchmark calls once with
+// nrep = 1000; tests call many times with small shapes).
+struct DropinCache {
+  std::mutex mu;
+  void *dU = nullptr, *dIn = nullptr, *dOut = nullptr, *dNbr = nullptr, *dPrm = nullptr;
+  size_t nU = 0, nIn = 0, nOut = 0, nNbr = 0, nPrm = 0;
+  gbx_geom geom = nullptr;
+  gbx_gauge gauge = nullptr;
+  gbx_spinor sin = nullptr, sout = nullptr;
+  int dims[4] = {0, 0, 0, 0}, Ls = 0, prec = -1;
+};
+DropinCache g_cache;
+
+int ensure(void **p, size_t *have, size_t need)
+{
+  if (*have >= need && *p) return GBX_OK;
+  if (*p) (void)hipFree(*p);
+  *p = nullptr;
+  *have = 0;
+  GBX_HIP(hipMalloc(p, need));
+  *have = need;
+  return GBX_OK;
+}
+
+int fast_objects(const int dims[4], int Ls, int prec)
+{
+  DropinCache &c = g_cache;
+  if (c.geom && c.prec == prec && c.Ls == Ls && !memcmp(c.dims, dims, sizeof c.dims)) return GBX_OK;
+  gbx_spinor_destroy(c.sin);
+  gbx_spinor_destroy(c.sout);
+  gbx_gauge_destroy(c.gauge);
+  gbx_geom_destroy(c.geom);
+  c.sin = c.sout = nullptr;
+  c.gauge = nullptr;
+  c.geom = nullptr;
+  c.prec = -1;
+  GBX_TRY(gbx_geom_create(dims, nullptr, &c.geom));
+  GBX_TRY(gbx_gauge_create(c.geom, prec, 0, &c.gauge));
+  GBX_TRY(gbx_spinor_create(c.geom, Ls, 1, prec, &c.sin));
+  GBX_TRY(gbx_spinor_create(c.geom, Ls, 1, prec, &c.sout));
+  memcpy(c.dims, dims, sizeof c.dims);
+  c.Ls = Ls;
+  c.prec = prec;
+  return GBX_OK;
+}
+
+template <typename T> struct RefIO;
+template <> struct RefIO<double> {
+  static constexpr int prec = GBX_F64;
+  static int up_g(gbx_gauge u, const double *p, int n) { return gbx_gauge_upload_ref_f64(u, p, n); }
+  static int up_s(gbx_spinor x, const double *p, int n) { return gbx_spinor_upload_ref_f64(x, p, n); }
+  static int down_s(gbx_spinor x, double *p, int n) { return gbx_spinor_download_ref_f64(x, p, n); }
+};
+template <> struct RefIO<float> {
+  static constexpr int prec = GBX_F32;
+  static int up_g(gbx_gauge u, const float *p, int n) { return gbx_gauge_upload_ref_f32(u, p, n); }
+  static int up_s(gbx_spinor x, const float *p, int n) { return gbx_spinor_upload_ref_f32(x, p, n); }
+  static int down_s(gbx_spinor x, float *p, int n) { return gbx_spinor_download_ref_f32(x, p, n); }
+};
+
+// Time nrep launches of `launch` on stream st, reference convention.
+template <typename F> int timed_reps(int nrep, hipStream_t st, F launch, double *usec)
+{
+  hipEvent_t e0, e1;
+  GBX_HIP(hipEventCreate(&e0));
+  GBX_HIP(hipEventCreate(&e1));
+  int rc = GBX_OK;
+  for (int rep = 0; rep < nrep && rc == GBX_OK; rep++) {
+    if (rep == (nrep > 1 ? 1 : 0) && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+    if (rc == GBX_OK) rc = launch();
+  }
+  if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+  if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "kernel failed: %s",
+                                                                         hipGetErrorString(hipGetLastError()));
+  float ms = 0.f;
+  if (rc == GBX_OK) (void)hipEventElapsedTime(&ms, e0, e1);
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK && usec) *usec = 1000.0 * (double)ms;
+  return rc;
+}
+
+template <typename T>
+int dslash_kernel_impl(int nrep, int nsimd, int dag, const T *U, T *out, const T *in, const uint64_t *nbr, uint64_t nsite,
+                       uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  g_last_path = 0;
+  GBX_CHECK(U && out && in && nbr && prm, GBX_ENULL, "dslash_kernel: NULL array");
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "dslash_kernel: nrep=%d", nrep);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dslash_kernel: dag=%d", dag);
+  GBX_CHECK(nsimd == 1 || nsimd == 2 || nsimd == 4 || nsimd == 8 || nsimd == 16, GBX_EINVAL,
+            "dslash_kernel: nsimd=%d not in {1,2,4,8,16}", nsimd);
+  GBX_CHECK(nsite >= 1 && Ls >= 1 && Ls <= 4096 && nsite * Ls <= (1ull << 34), GBX_EINVAL,
+            "dslash_kernel: nsite=%llu Ls=%llu", (unsigned long long)nsite, (unsigned long long)Ls);
+  const uint64_t n5 = nsite * Ls;
+  const size_t nf = (size_t)n5 * 24 * nsimd * sizeof(T), nu = (size_t)nsite * 144 * nsimd * sizeof(T);
+  const size_t nt = (size_t)n5 * 8;
+  GBX_CHECK(!overlaps(out, nf, in, nf) && !overlaps(out, nf, U, nu) && !overlaps(out, nf, nbr, nt * 8) &&
+                !overlaps(out, nf, prm, nt),
+            GBX_EALIAS, "dslash_kernel: out overlaps an input array");
+  // Validate the tables (the reference trusts them blindly).
+  const int64_t bad = table_first_invalid(nbr, prm, nt, n5);
+  GBX_CHECK(bad < 0, GBX_ETABLE, "dslash_kernel: table entry %lld invalid (nbr=%llu, prm=%u, nsite*Ls=%llu)",
+            (long long)bad, (unsigned long long)nbr[bad < 0 ? 0 : bad], (unsigned)prm[bad < 0 ? 0 : bad],
+            (unsigned long long)n5);
+  GBX_TRY(gbx_require_device());
+  std::lock_guard<std::mutex> lk(g_cache.mu);
+  int dims[4];
+  const int canon = gbx_table_recognise(nsimd, nsite, Ls, nbr, prm, dims);
+  gbx_clear_error();
+  if (canon == GBX_OK) {
+    GBX_TRY(fast_objects(dims, (int)Ls, RefIO<T>::prec));
+    DropinCache &c = g_cache;
+    GBX_TRY(RefIO<T>::up_g(c.gauge, U, nsimd));
+    GBX_TRY(RefIO<T>::up_s(c.sin, in, nsimd));
+    gbx_kcfg k;
+    gbx_dhop_get_cfg(c.gauge, c.sin, &k);
+    hipStream_t st = c.geom->stream;
+    GBX_TRY(timed_reps(nrep, st, [&]() { return dhop_run(c.gauge, c.sout, c.sin, dag, k, st); }, usec));
+    GBX_TRY(RefIO<T>::down_s(c.sout, out, nsimd));
+    g_last_path = 1;
+    return GBX_OK;
+  }
+  // Generic table-driven path.
+  DropinCache &c = g_cache;
+  GBX_TRY(ensure(&c.dU, &c.nU, nu));
+  GBX_TRY(ensure(&c.dIn, &c.nIn, nf));
+  GBX_TRY(ensure(&c.dOut, &c.nOut, nf));
+  GBX_TRY(ensure(&c.dNbr, &c.nNbr, nt * 8));
+  GBX_TRY(ensure(&c.dPrm, &c.nPrm, nt));
+  hipStream_t st = nullptr;
+  GBX_HIP(hipMemcpy(c.dU, U, nu, hipMemcpyHostToDevice));
+  GBX_HIP(hipMemcpy(c.dIn, in, nf, hipMemcpyHostToDevice));
+  GBX_HIP(hipMemcpy(c.dNbr, nbr, nt * 8, hipMemcpyHostToDevice));
+  GBX_HIP(hipMemcpy(c.dPrm, prm, nt, hipMemcpyHostToDevice));
+  const uint64_t nthr = n5 * (uint64_t)nsimd;
+  const dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
+  auto launch = [&]() -> int {
+    if (dag)
+      hipLaunchKernelGGL((k_dhop_table<T, true>), grid, block, 0, st, (const T *)c.dU, (T *)c.dOut, (const T *)c.dIn,
+                         (const uint64_t *)c.dNbr, (const uint8_t *)c.dPrm, nsite, Ls, nsimd);
+    else
+      hipLaunchKernelGGL((k_dhop_table<T, false>), grid, block, 0, st, (const T *)c.dU, (T *)c.dOut, (const T *)c.dIn,
+                         (const uint64_t *)c.dNbr, (const uint8_t *)c.dPrm, nsite, Ls, nsimd);
+    GBX_HIP(hipGetLastError());
+    return GBX_OK;
+  };
+  GBX_TRY(timed_reps(nrep, st, launch, usec));
+  GBX_HIP(hipMemcpy(out, c.dOut, nf, hipMemcpyDeviceToHost));
+  g_last_path = 2;
+  return GBX_OK;
+}
+
+}  // namespace
+
+extern "C" int gbx_dslash_kernel_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
+                                     const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<double>(nrep, nsimd, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_kernel_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
+                                     const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<float>(nrep, nsimd, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_last_path(void) { return g_last_path; }

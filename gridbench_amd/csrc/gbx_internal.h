@@ -1,0 +1,126 @@
+// Internal declarations shared by the libgbx translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "gbx.h"
+
+// ---------------------------------------------------------------------------
+// Error reporting: every C-ABI entry point returns a status code and leaves a
+// thread-local message describing the failure (gbx_last_error).
+// ---------------------------------------------------------------------------
+int gbx_fail(int status, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+void gbx_clear_error();
+
+#define GBX_HIP(call)                                                                          \
+  do {                                                                                         \
+    hipError_t e_ = (call);                                                                    \
+    if (e_ != hipSuccess)                                                                      \
+      return gbx_fail(e_ == hipErrorOutOfMemory ? GBX_ENOMEM : GBX_EHIP, "%s failed: %s (%s:%d)", \
+                      #call, hipGetErrorString(e_), __FILE__, __LINE__);                       \
+  } while (0)
+
+#define GBX_CHECK(cond, status, ...)                                                           \
+  do {                                                                                         \
+    if (!(cond)) return gbx_fail(status, __VA_ARGS__);                                          \
+  } while (0)
+
+#define GBX_TRY(expr)                                                                          \
+  do {                                                                                         \
+    int st_ = (expr);                                                                          \
+    if (st_ != GBX_OK) return st_;                                                             \
+  } while (0)
+
+int gbx_require_device();  // GBX_ENODEV unless a HIP device is usable
+
+// ---------------------------------------------------------------------------
+// Storage precisions.  Spinors and links are stored as FP64, FP32, FP16 or
+// BF16; arithmetic is FP64 for FP64 storage and FP32 for everything else.
+// ---------------------------------------------------------------------------
+inline int prec_bytes(int p) { return p == GBX_F64 ? 8 : p == GBX_F32 ? 4 : 2; }
+inline bool prec_valid(int p) { return p >= GBX_F64 && p <= GBX_BF16; }
+
+// Device spinor layout ("site-major, Ls/RHS-minor"):
+//   chunk k of slot q, minor index m  ->  byte offset ((q*nch + k)*M + m)*16
+// where a 16-byte chunk holds 16/b consecutive reals of the 24-real spinor
+// (real r = (spin*3+colour)*2 + re/im), nch = 24*b/16 chunks per spinor,
+// M = Ls*nrhs and m = r_hs*Ls + s.  One lane loading one chunk therefore moves
+// 16 B and a wavefront of consecutive m moves 1 KiB of contiguous memory.
+inline int spinor_nchunk(int p) { return 24 * prec_bytes(p) / 16; }
+
+// Device gauge layout: slot-major, [slot][dir 0..7][R] with R = 18 reals
+// (3x3 complex, row-major) or 12 reals (rows 0 and 1 only; row 2 rebuilt as
+// the conjugated cross product), each real in the gauge storage precision.
+
+// ---------------------------------------------------------------------------
+// Objects behind the opaque C handles.
+// ---------------------------------------------------------------------------
+struct gbx_geom_s {
+  int dims[4];
+  uint64_t vol;           // 4D sites
+  int block[4];           // traversal block shape
+  int device;
+  hipStream_t stream;
+  // slot <-> lexicographic site permutation (host and device copies)
+  std::vector<uint32_t> site_of_slot_h, slot_of_site_h;
+  uint32_t *site_of_slot = nullptr, *slot_of_site = nullptr;
+  uint32_t *nbr = nullptr;  // [slot][8] neighbour slots, direction order Xp Yp Zp Tp Xm Ym Zm Tm
+  int refcount = 1;
+};
+
+struct gbx_gauge_s {
+  gbx_geom_s *geom;
+  int prec;
+  int comp12;             // 1: 12 reals per link stored
+  void *d = nullptr;      // device storage
+  size_t bytes = 0;
+};
+
+struct gbx_spinor_s {
+  gbx_geom_s *geom;
+  int Ls, nrhs, prec;
+  uint32_t M;             // Ls * nrhs
+  void *d = nullptr;
+  size_t bytes = 0;
+};
+
+void geom_retain(gbx_geom_s *g);
+void geom_release(gbx_geom_s *g);
+
+// Scratch buffer reused by uploads/downloads (grows on demand, per geometry
+// stream; freed with the library's cached allocations at exit).
+int staging_buffer(size_t bytes, void **ptr);
+
+// Kernel launch configuration of the VALU hopping kernel.
+struct dhop_cfg {
+  int spw;       // 4D sites per workgroup (links of these sites staged in LDS)
+  int threads;   // workgroup size
+  int xcd;       // 1: XCD-aware workgroup order (contiguous slot ranges per XCD)
+  int nt;        // 1: non-temporal output stores
+};
+
+// out = alpha*in_c + beta*Dhop(in) where in_c is `center` (may be null when alpha == 0).
+int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag,
+                const dhop_cfg &cfg, double alpha, double beta, const gbx_spinor_s *center,
+                hipStream_t stream);
+dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in);
+int dhop_cfg_check(const dhop_cfg &cfg, const gbx_gauge_s *u, const gbx_spinor_s *in);
+// Any kernel kind (VALU or MFMA) with an explicit configuration.
+int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
+std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in);
+void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c);
+bool dhop_lookup_cfg(const std::string &sig, gbx_kcfg *c);
+int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in);
+
+// Vector algebra on a given stream (blas.hip).
+int spinor_convert_scaled(gbx_spinor_s *dst, const gbx_spinor_s *src, double scale, hipStream_t st);
+int spinor_norm2_on(const gbx_spinor_s *x, double *n2, hipStream_t st);
+int spinor_dot_on(const gbx_spinor_s *x, const gbx_spinor_s *y, double *re, double *im, hipStream_t st);
+int spinor_axpy_on(double a, const gbx_spinor_s *x, gbx_spinor_s *y, hipStream_t st);
+int spinor_scale_on(gbx_spinor_s *x, double a, hipStream_t st);
+int spinor_zero_on(gbx_spinor_s *x, hipStream_t st);

@@ -1,0 +1,620 @@
+// Geometry (device site order + neighbour table) and the layout kernels that
+// move fields between host layouts and the device SoA layout.
+//
+// Device site order: the lattice is cut into blocks (default Lx x 4 x 4 x 4,
+// each extent the largest divisor not above the target); slots run block by
+// block (blocks lexicographic, x fastest) and lexicographically inside a
+// block.  Consecutive slots are therefore spatially compact, so the 8 stencil
+// neighbours of a workgroup's sites were touched recently and are still in
+// L2 / the memory-side cache.  nbr[slot][mu] holds neighbour slots in the
+// reference's direction order Xp Yp Zp Tp Xm Ym Zm Tm (WilsonKernelsHand.h:5-12).
+#include <mutex>
+
+#include "gbx_internal.h"
+#include "layout_dev.h"
+
+namespace {
+
+std::mutex g_staging_mu;
+void *g_staging = nullptr;
+size_t g_staging_bytes = 0;
+
+int largest_divisor_le(int n, int cap)
+{
+  for (int b = cap < n ? cap : n; b > 1; b--)
+    if (n % b == 0) return b;
+  return 1;
+}
+
+uint32_t nblocks(uint64_t n, int threads, uint32_t cap = 1u << 20)
+{
+  uint64_t b = (n + (uint64_t)threads - 1) / (uint64_t)threads;
+  if (b > cap) b = cap;
+  if (b == 0) b = 1;
+  return (uint32_t)b;
+}
+
+// ------------------------------------------------------------------ kernels
+// Lexicographic FP64 [nr][V][Ls][24] <-> minor indices m0 .. m0+Mr-1 of the
+// device field (Mr = nr*Ls: right-hand sides m0/Ls .. m0/Ls+nr-1).
+template <int P>
+__global__ void k_pack_lex(const double *__restrict__ lex, void *__restrict__ dst, const uint32_t *__restrict__ site_of_slot,
+                           uint64_t total, uint32_t V, uint32_t Ls, uint32_t M, uint32_t m0, uint32_t Mr)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t mm = (uint32_t)(t % Mr);
+    const uint64_t qk = t / Mr;
+    const uint32_t k = (uint32_t)(qk % NCH);
+    const uint32_t q = (uint32_t)(qk / NCH);
+    const uint32_t g = site_of_slot[q];
+    const uint32_t r = mm / Ls, s = mm % Ls;
+    const double *src = lex + (((uint64_t)r * V + g) * Ls + s) * 24 + (uint64_t)k * R;
+    double v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) v[j] = src[j];
+    dev::store_chunk<P>(dst, ((uint64_t)q * NCH + k) * M + m0 + mm, v, false);
+  }
+}
+
+template <int P>
+__global__ void k_unpack_lex(const void *__restrict__ srcd, double *__restrict__ lex, const uint32_t *__restrict__ site_of_slot,
+                             uint64_t total, uint32_t V, uint32_t Ls, uint32_t M, uint32_t m0, uint32_t Mr)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t mm = (uint32_t)(t % Mr);
+    const uint64_t qk = t / Mr;
+    const uint32_t k = (uint32_t)(qk % NCH);
+    const uint32_t q = (uint32_t)(qk / NCH);
+    const uint32_t g = site_of_slot[q];
+    const uint32_t r = mm / Ls, s = mm % Ls;
+    double v[R];
+    dev::load_chunk_f64<P>(srcd, ((uint64_t)q * NCH + k) * M + m0 + mm, v);
+    double *o = lex + (((uint64_t)r * V + g) * Ls + s) * 24 + (uint64_t)k * R;
+#pragma unroll
+    for (int j = 0; j < R; j++) o[j] = v[j];
+  }
+}
+
+// Reference virtual-node host layout [nsite*Ls][12][nsimd][2] <-> device.
+struct RefGeom {
+  int dims[4], simd[4], rd[4];
+  int nsimd;
+};
+
+__device__ inline void ref_coords(const RefGeom &rg, uint32_t g, uint64_t &o, uint32_t &lane)
+{
+  uint32_t x[4];
+  uint32_t r = g;
+#pragma unroll
+  for (int d = 0; d < 4; d++) {
+    x[d] = r % (uint32_t)rg.dims[d];
+    r /= (uint32_t)rg.dims[d];
+  }
+  o = 0;
+  lane = 0;
+#pragma unroll
+  for (int d = 3; d >= 0; d--) {
+    o = o * (uint64_t)rg.rd[d] + x[d] % (uint32_t)rg.rd[d];
+    lane = lane * (uint32_t)rg.simd[d] + x[d] / (uint32_t)rg.rd[d];
+  }
+}
+
+template <int P, typename T>
+__global__ void k_pack_ref(const T *__restrict__ ref, void *__restrict__ dst, const uint32_t *__restrict__ site_of_slot,
+                           uint64_t total, RefGeom rg, uint32_t Ls)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)(t % Ls);
+    const uint64_t qk = t / Ls;
+    const uint32_t k = (uint32_t)(qk % NCH);
+    const uint32_t q = (uint32_t)(qk / NCH);
+    uint64_t o;
+    uint32_t lane;
+    ref_coords(rg, site_of_slot[q], o, lane);
+    double v[R];
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const uint32_t ridx = k * R + j;
+      v[j] = (double)ref[(((o * Ls + s) * 12 + (ridx >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (ridx & 1)];
+    }
+    dev::store_chunk<P>(dst, t, v, false);
+  }
+}
+
+template <int P, typename T>
+__global__ void k_unpack_ref(const void *__restrict__ srcd, T *__restrict__ ref, const uint32_t *__restrict__ site_of_slot,
+                             uint64_t total, RefGeom rg, uint32_t Ls)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t s = (uint32_t)(t % Ls);
+    const uint64_t qk = t / Ls;
+    const uint32_t k = (uint32_t)(qk % NCH);
+    const uint32_t q = (uint32_t)(qk / NCH);
+    uint64_t o;
+    uint32_t lane;
+    ref_coords(rg, site_of_slot[q], o, lane);
+    double v[R];
+    dev::load_chunk_f64<P>(srcd, t, v);
+#pragma unroll
+    for (int j = 0; j < R; j++) {
+      const uint32_t ridx = k * R + j;
+      ref[(((o * Ls + s) * 12 + (ridx >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (ridx & 1)] = (T)v[j];
+    }
+  }
+}
+
+// Gauge: lexicographic doubled FP64 [V][8][18] or reference layout
+// [nsite][8][9][nsimd][2] -> device [slot][8][Rl] (Rl = 18 or 12 reals).
+template <int P>
+__global__ void k_pack_gauge_lex(const double *__restrict__ lex, void *__restrict__ dst, const uint32_t *__restrict__ site_of_slot,
+                                 uint64_t total, int Rl)
+{
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t e = (uint32_t)(t % Rl);
+    const uint64_t qd = t / Rl;
+    const uint32_t dir = (uint32_t)(qd % 8);
+    const uint32_t q = (uint32_t)(qd / 8);
+    const uint32_t g = site_of_slot[q];
+    dev::store_real<P>(dst, t, lex[((uint64_t)g * 8 + dir) * 18 + e]);
+  }
+}
+
+template <int P, typename T>
+__global__ void k_pack_gauge_ref(const T *__restrict__ ref, void *__restrict__ dst, const uint32_t *__restrict__ site_of_slot,
+                                 uint64_t total, int Rl, RefGeom rg)
+{
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < total; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint32_t e = (uint32_t)(t % Rl);
+    const uint64_t qd = t / Rl;
+    const uint32_t dir = (uint32_t)(qd % 8);
+    const uint32_t q = (uint32_t)(qd / 8);
+    uint64_t o;
+    uint32_t lane;
+    ref_coords(rg, site_of_slot[q], o, lane);
+    dev::store_real<P>(dst, t, (double)ref[(((o * 8 + dir) * 9 + (e >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (e & 1)]);
+  }
+}
+
+// ------------------------------------------------------------- host helpers
+int ref_geom(const gbx_geom_s *g, int nsimd, RefGeom &rg)
+{
+  rg.nsimd = nsimd;
+  for (int d = 0; d < 4; d++) rg.simd[d] = 1;
+  GBX_CHECK(nsimd == 1 || nsimd == 2 || nsimd == 4 || nsimd == 8 || nsimd == 16, GBX_EINVAL,
+            "nsimd=%d not in {1,2,4,8,16}", nsimd);
+  int d = 3;
+  for (int n = nsimd; n > 1; n >>= 1) {
+    rg.simd[d] *= 2;
+    d = (d + 3) % 4;
+  }
+  for (int e = 0; e < 4; e++) {
+    rg.dims[e] = g->dims[e];
+    GBX_CHECK(g->dims[e] % rg.simd[e] == 0, GBX_EINVAL, "dim %d = %d not divisible by %d lanes", e, g->dims[e], rg.simd[e]);
+    rg.rd[e] = g->dims[e] / rg.simd[e];
+  }
+  return GBX_OK;
+}
+
+int spinor_ok(const gbx_spinor_s *x)
+{
+  GBX_CHECK(x && x->d, GBX_ENULL, "spinor handle is NULL");
+  return GBX_OK;
+}
+
+template <template <int> class F, typename... Args>
+void dispatch_prec(int p, Args... a)
+{
+  switch (p) {
+  case GBX_F64: F<GBX_F64>::run(a...); break;
+  case GBX_F32: F<GBX_F32>::run(a...); break;
+  case GBX_F16: F<GBX_F16>::run(a...); break;
+  default: F<GBX_BF16>::run(a...); break;
+  }
+}
+
+template <int P> struct PackLex {
+  static void run(const double *lex, void *dst, const uint32_t *sos, uint64_t total, uint32_t V, uint32_t Ls, uint32_t M,
+                  uint32_t m0, uint32_t Mr, hipStream_t st)
+  {
+    hipLaunchKernelGGL(k_pack_lex<P>, dim3(nblocks(total, 256)), dim3(256), 0, st, lex, dst, sos, total, V, Ls, M, m0, Mr);
+  }
+};
+template <int P> struct UnpackLex {
+  static void run(const void *src, double *lex, const uint32_t *sos, uint64_t total, uint32_t V, uint32_t Ls, uint32_t M,
+                  uint32_t m0, uint32_t Mr, hipStream_t st)
+  {
+    hipLaunchKernelGGL(k_unpack_lex<P>, dim3(nblocks(total, 256)), dim3(256), 0, st, src, lex, sos, total, V, Ls, M, m0, Mr);
+  }
+};
+template <int P> struct PackGaugeLex {
+  static void run(const double *lex, void *dst, const uint32_t *sos, uint64_t total, int Rl, hipStream_t st)
+  {
+    hipLaunchKernelGGL(k_pack_gauge_lex<P>, dim3(nblocks(total, 256)), dim3(256), 0, st, lex, dst, sos, total, Rl);
+  }
+};
+
+template <typename T> struct PackRefT {
+  template <int P> struct F {
+    static void run(const T *ref, void *dst, const uint32_t *sos, uint64_t total, RefGeom rg, uint32_t Ls, hipStream_t st)
+    {
+      hipLaunchKernelGGL((k_pack_ref<P, T>), dim3(nblocks(total, 256)), dim3(256), 0, st, ref, dst, sos, total, rg, Ls);
+    }
+  };
+};
+template <typename T> struct UnpackRefT {
+  template <int P> struct F {
+    static void run(const void *src, T *ref, const uint32_t *sos, uint64_t total, RefGeom rg, uint32_t Ls, hipStream_t st)
+    {
+      hipLaunchKernelGGL((k_unpack_ref<P, T>), dim3(nblocks(total, 256)), dim3(256), 0, st, src, ref, sos, total, rg, Ls);
+    }
+  };
+};
+template <typename T> struct PackGaugeRefT {
+  template <int P> struct F {
+    static void run(const T *ref, void *dst, const uint32_t *sos, uint64_t total, int Rl, RefGeom rg, hipStream_t st)
+    {
+      hipLaunchKernelGGL((k_pack_gauge_ref<P, T>), dim3(nblocks(total, 256)), dim3(256), 0, st, ref, dst, sos, total, Rl, rg);
+    }
+  };
+};
+
+}  // namespace
+
+// ------------------------------------------------------------ staging buffer
+int staging_buffer(size_t bytes, void **ptr)
+{
+  // caller holds g_staging_mu via staging_lock()
+  if (bytes > g_staging_bytes) {
+    if (g_staging) GBX_HIP(hipFree(g_staging));
+    g_staging = nullptr;
+    g_staging_bytes = 0;
+    GBX_HIP(hipMalloc(&g_staging, bytes));
+    g_staging_bytes = bytes;
+  }
+  *ptr = g_staging;
+  return GBX_OK;
+}
+
+// ------------------------------------------------------------------- geometry
+void geom_retain(gbx_geom_s *g) { g->refcount++; }
+
+void geom_release(gbx_geom_s *g)
+{
+  if (--g->refcount > 0) return;
+  (void)hipSetDevice(g->device);
+  if (g->stream) (void)hipStreamSynchronize(g->stream);
+  (void)hipFree(g->site_of_slot);
+  (void)hipFree(g->slot_of_site);
+  (void)hipFree(g->nbr);
+  if (g->stream) (void)hipStreamDestroy(g->stream);
+  delete g;
+}
+
+extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out)
+{
+  GBX_CHECK(dims && out, GBX_ENULL, "gbx_geom_create: NULL argument");
+  *out = nullptr;
+  uint64_t vol = 1;
+  for (int d = 0; d < 4; d++) {
+    GBX_CHECK(dims[d] >= 1 && dims[d] <= 1024, GBX_EINVAL, "gbx_geom_create: dims[%d]=%d", d, dims[d]);
+    vol *= (uint64_t)dims[d];
+  }
+  GBX_CHECK(vol <= (1ull << 30), GBX_EINVAL, "gbx_geom_create: volume %llu too large", (unsigned long long)vol);
+  int blk[4];
+  for (int d = 0; d < 4; d++) {
+    if (block) {
+      GBX_CHECK(block[d] >= 1 && dims[d] % block[d] == 0, GBX_EINVAL, "gbx_geom_create: block[%d]=%d does not divide %d", d,
+                block[d], dims[d]);
+      blk[d] = block[d];
+    } else {
+      blk[d] = d == 0 ? dims[0] : largest_divisor_le(dims[d], 4);
+    }
+  }
+  GBX_TRY(gbx_require_device());
+  gbx_geom_s *g = new gbx_geom_s();
+  for (int d = 0; d < 4; d++) {
+    g->dims[d] = dims[d];
+    g->block[d] = blk[d];
+  }
+  g->vol = vol;
+  GBX_HIP(hipGetDevice(&g->device));
+  g->site_of_slot_h.resize(vol);
+  g->slot_of_site_h.resize(vol);
+  const int nb[4] = {dims[0] / blk[0], dims[1] / blk[1], dims[2] / blk[2], dims[3] / blk[3]};
+  const uint64_t bvol = (uint64_t)blk[0] * blk[1] * blk[2] * blk[3];
+  for (uint64_t site = 0; site < vol; site++) {
+    int x[4];
+    uint64_t r = site;
+    for (int d = 0; d < 4; d++) {
+      x[d] = (int)(r % (uint64_t)dims[d]);
+      r /= (uint64_t)dims[d];
+    }
+    uint64_t bi = 0, wi = 0;
+    for (int d = 3; d >= 0; d--) {
+      bi = bi * (uint64_t)nb[d] + (uint64_t)(x[d] / blk[d]);
+      wi = wi * (uint64_t)blk[d] + (uint64_t)(x[d] % blk[d]);
+    }
+    const uint64_t slot = bi * bvol + wi;
+    g->slot_of_site_h[site] = (uint32_t)slot;
+    g->site_of_slot_h[slot] = (uint32_t)site;
+  }
+  std::vector<uint32_t> nbr(vol * 8);
+  uint64_t stride[4] = {1, (uint64_t)dims[0], (uint64_t)dims[0] * dims[1], (uint64_t)dims[0] * dims[1] * dims[2]};
+  for (uint64_t q = 0; q < vol; q++) {
+    const uint64_t site = g->site_of_slot_h[q];
+    for (int mu = 0; mu < 8; mu++) {
+      const int d = mu & 3;
+      const int x = (int)((site / stride[d]) % (uint64_t)dims[d]);
+      uint64_t ns;
+      if (mu < 4) ns = (x + 1 == dims[d]) ? site - (uint64_t)x * stride[d] : site + stride[d];
+      else ns = (x == 0) ? site + (uint64_t)(dims[d] - 1) * stride[d] : site - stride[d];
+      nbr[q * 8 + mu] = g->slot_of_site_h[ns];
+    }
+  }
+  int st = GBX_OK;
+  auto fail = [&](int s) {
+    geom_release(g);
+    return s;
+  };
+  if (hipStreamCreate(&g->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "hipStreamCreate failed"));
+  if (hipMalloc(&g->site_of_slot, vol * 4) != hipSuccess || hipMalloc(&g->slot_of_site, vol * 4) != hipSuccess ||
+      hipMalloc(&g->nbr, vol * 32) != hipSuccess)
+    return fail(gbx_fail(GBX_ENOMEM, "gbx_geom_create: device allocation failed"));
+  if (hipMemcpy(g->site_of_slot, g->site_of_slot_h.data(), vol * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->slot_of_site, g->slot_of_site_h.data(), vol * 4, hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(g->nbr, nbr.data(), vol * 32, hipMemcpyHostToDevice) != hipSuccess)
+    return fail(gbx_fail(GBX_EHIP, "gbx_geom_create: upload failed"));
+  (void)st;
+  *out = g;
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_destroy(gbx_geom g)
+{
+  if (!g) return GBX_OK;
+  geom_release(g);
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_volume(gbx_geom g, uint64_t *vol)
+{
+  GBX_CHECK(g && vol, GBX_ENULL, "gbx_geom_volume: NULL argument");
+  *vol = g->vol;
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_stream(gbx_geom g, void **stream)
+{
+  GBX_CHECK(g && stream, GBX_ENULL, "gbx_geom_stream: NULL argument");
+  *stream = (void *)g->stream;
+  return GBX_OK;
+}
+
+extern "C" int gbx_sync(gbx_geom g)
+{
+  GBX_CHECK(g, GBX_ENULL, "gbx_sync: NULL geometry");
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+// --------------------------------------------------------------------- gauge
+extern "C" int gbx_gauge_create(gbx_geom g, int prec, int comp12, gbx_gauge *out)
+{
+  GBX_CHECK(g && out, GBX_ENULL, "gbx_gauge_create: NULL argument");
+  GBX_CHECK(prec_valid(prec), GBX_EINVAL, "gbx_gauge_create: precision %d", prec);
+  GBX_CHECK(comp12 == 0 || comp12 == 1, GBX_EINVAL, "gbx_gauge_create: comp12 must be 0 or 1");
+  *out = nullptr;
+  gbx_gauge_s *u = new gbx_gauge_s();
+  u->geom = g;
+  u->prec = prec;
+  u->comp12 = comp12;
+  u->bytes = g->vol * 8 * (comp12 ? 12 : 18) * (size_t)prec_bytes(prec);
+  if (hipMalloc(&u->d, u->bytes) != hipSuccess) {
+    delete u;
+    return gbx_fail(GBX_ENOMEM, "gbx_gauge_create: cannot allocate %zu bytes", (size_t)g->vol * 8 * 18 * prec_bytes(prec));
+  }
+  geom_retain(g);
+  *out = u;
+  return GBX_OK;
+}
+
+extern "C" int gbx_gauge_destroy(gbx_gauge u)
+{
+  if (!u) return GBX_OK;
+  (void)hipStreamSynchronize(u->geom->stream);
+  (void)hipFree(u->d);
+  geom_release(u->geom);
+  delete u;
+  return GBX_OK;
+}
+
+extern "C" int gbx_gauge_upload(gbx_gauge u, const double *doubled)
+{
+  GBX_CHECK(u && doubled, GBX_ENULL, "gbx_gauge_upload: NULL argument");
+  gbx_geom_s *g = u->geom;
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  void *stage;
+  const size_t in_bytes = g->vol * 8 * 18 * sizeof(double);
+  GBX_TRY(staging_buffer(in_bytes, &stage));
+  GBX_HIP(hipMemcpyAsync(stage, doubled, in_bytes, hipMemcpyHostToDevice, g->stream));
+  const int Rl = u->comp12 ? 12 : 18;
+  const uint64_t total = g->vol * 8 * (uint64_t)Rl;
+  dispatch_prec<PackGaugeLex>(u->prec, (const double *)stage, u->d, (const uint32_t *)g->site_of_slot, total, Rl, g->stream);
+  GBX_HIP(hipGetLastError());
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+template <typename T> static int gauge_upload_ref(gbx_gauge u, const T *ref, int nsimd)
+{
+  GBX_CHECK(u && ref, GBX_ENULL, "gbx_gauge_upload_ref: NULL argument");
+  gbx_geom_s *g = u->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg));
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  void *stage;
+  const size_t in_bytes = g->vol * 8 * 18 * sizeof(T);
+  GBX_TRY(staging_buffer(in_bytes, &stage));
+  GBX_HIP(hipMemcpyAsync(stage, ref, in_bytes, hipMemcpyHostToDevice, g->stream));
+  const int Rl = u->comp12 ? 12 : 18;
+  const uint64_t total = g->vol * 8 * (uint64_t)Rl;
+  dispatch_prec<PackGaugeRefT<T>::template F>(u->prec, (const T *)stage, u->d, (const uint32_t *)g->site_of_slot, total, Rl, rg,
+                                              g->stream);
+  GBX_HIP(hipGetLastError());
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+extern "C" int gbx_gauge_upload_ref_f64(gbx_gauge u, const double *ref, int nsimd) { return gauge_upload_ref(u, ref, nsimd); }
+extern "C" int gbx_gauge_upload_ref_f32(gbx_gauge u, const float *ref, int nsimd) { return gauge_upload_ref(u, ref, nsimd); }
+
+// -------------------------------------------------------------------- spinor
+extern "C" int gbx_spinor_create(gbx_geom g, int Ls, int nrhs, int prec, gbx_spinor *out)
+{
+  GBX_CHECK(g && out, GBX_ENULL, "gbx_spinor_create: NULL argument");
+  *out = nullptr;
+  GBX_CHECK(prec_valid(prec), GBX_EINVAL, "gbx_spinor_create: precision %d", prec);
+  GBX_CHECK(Ls >= 1 && nrhs >= 1 && (uint64_t)Ls * (uint64_t)nrhs <= 4096, GBX_EINVAL,
+            "gbx_spinor_create: Ls=%d nrhs=%d (Ls*nrhs must be in 1..4096)", Ls, nrhs);
+  gbx_spinor_s *x = new gbx_spinor_s();
+  x->geom = g;
+  x->Ls = Ls;
+  x->nrhs = nrhs;
+  x->prec = prec;
+  x->M = (uint32_t)(Ls * nrhs);
+  x->bytes = g->vol * (size_t)x->M * 24 * (size_t)prec_bytes(prec);
+  if (hipMalloc(&x->d, x->bytes) != hipSuccess) {
+    const size_t b = x->bytes;
+    delete x;
+    (void)hipGetLastError();
+    return gbx_fail(GBX_ENOMEM, "gbx_spinor_create: cannot allocate %zu bytes", b);
+  }
+  geom_retain(g);
+  *out = x;
+  return GBX_OK;
+}
+
+extern "C" int gbx_spinor_destroy(gbx_spinor x)
+{
+  if (!x) return GBX_OK;
+  (void)hipStreamSynchronize(x->geom->stream);
+  (void)hipFree(x->d);
+  geom_release(x->geom);
+  delete x;
+  return GBX_OK;
+}
+
+extern "C" int gbx_spinor_bytes(gbx_spinor x, uint64_t *bytes)
+{
+  GBX_CHECK(x && bytes, GBX_ENULL, "gbx_spinor_bytes: NULL argument");
+  *bytes = x->bytes;
+  return GBX_OK;
+}
+
+// Right-hand sides r0 .. r0+nr-1 from / to lexicographic FP64 [nr][V][Ls][24].
+static int spinor_xfer_lex(gbx_spinor x, int r0, int nr, double *lex, bool up)
+{
+  GBX_TRY(spinor_ok(x));
+  GBX_CHECK(lex, GBX_ENULL, "gbx_spinor_%s: NULL host array", up ? "upload" : "download");
+  GBX_CHECK(r0 >= 0 && nr >= 1 && r0 + nr <= x->nrhs, GBX_EINVAL, "gbx_spinor_%s: rhs range %d+%d of %d", up ? "upload" : "download",
+            r0, nr, x->nrhs);
+  gbx_geom_s *g = x->geom;
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  void *stage;
+  const uint32_t Mr = (uint32_t)(nr * x->Ls), m0 = (uint32_t)(r0 * x->Ls);
+  const size_t n = g->vol * (size_t)Mr * 24;
+  GBX_TRY(staging_buffer(n * sizeof(double), &stage));
+  const uint64_t total = g->vol * (uint64_t)spinor_nchunk(x->prec) * Mr;
+  if (up) {
+    GBX_HIP(hipMemcpyAsync(stage, lex, n * sizeof(double), hipMemcpyHostToDevice, g->stream));
+    dispatch_prec<PackLex>(x->prec, (const double *)stage, x->d, (const uint32_t *)g->site_of_slot, total, (uint32_t)g->vol,
+                           (uint32_t)x->Ls, x->M, m0, Mr, g->stream);
+    GBX_HIP(hipGetLastError());
+  } else {
+    dispatch_prec<UnpackLex>(x->prec, (const void *)x->d, (double *)stage, (const uint32_t *)g->site_of_slot, total,
+                             (uint32_t)g->vol, (uint32_t)x->Ls, x->M, m0, Mr, g->stream);
+    GBX_HIP(hipGetLastError());
+    GBX_HIP(hipMemcpyAsync(lex, stage, n * sizeof(double), hipMemcpyDeviceToHost, g->stream));
+  }
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+extern "C" int gbx_spinor_upload(gbx_spinor x, const double *lex)
+{
+  return spinor_xfer_lex(x, 0, x ? x->nrhs : 1, (double *)lex, true);
+}
+
+extern "C" int gbx_spinor_download(gbx_spinor x, double *lex) { return spinor_xfer_lex(x, 0, x ? x->nrhs : 1, lex, false); }
+
+extern "C" int gbx_spinor_upload_rhs(gbx_spinor x, int r, const double *lex) { return spinor_xfer_lex(x, r, 1, (double *)lex, true); }
+
+extern "C" int gbx_spinor_download_rhs(gbx_spinor x, int r, double *lex) { return spinor_xfer_lex(x, r, 1, lex, false); }
+
+template <typename T> static int spinor_upload_ref(gbx_spinor x, const T *ref, int nsimd)
+{
+  GBX_TRY(spinor_ok(x));
+  GBX_CHECK(ref, GBX_ENULL, "gbx_spinor_upload_ref: NULL host array");
+  GBX_CHECK(x->nrhs == 1, GBX_EINVAL, "gbx_spinor_upload_ref: reference layout holds one right-hand side");
+  gbx_geom_s *g = x->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg));
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  void *stage;
+  const size_t n = g->vol * (size_t)x->Ls * 24;
+  GBX_TRY(staging_buffer(n * sizeof(T), &stage));
+  GBX_HIP(hipMemcpyAsync(stage, ref, n * sizeof(T), hipMemcpyHostToDevice, g->stream));
+  const uint64_t total = g->vol * (uint64_t)spinor_nchunk(x->prec) * x->M;
+  dispatch_prec<PackRefT<T>::template F>(x->prec, (const T *)stage, x->d, (const uint32_t *)g->site_of_slot, total, rg,
+                                         (uint32_t)x->Ls, g->stream);
+  GBX_HIP(hipGetLastError());
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+template <typename T> static int spinor_download_ref(gbx_spinor x, T *ref, int nsimd)
+{
+  GBX_TRY(spinor_ok(x));
+  GBX_CHECK(ref, GBX_ENULL, "gbx_spinor_download_ref: NULL host array");
+  GBX_CHECK(x->nrhs == 1, GBX_EINVAL, "gbx_spinor_download_ref: reference layout holds one right-hand side");
+  gbx_geom_s *g = x->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg));
+  std::lock_guard<std::mutex> lk(g_staging_mu);
+  void *stage;
+  const size_t n = g->vol * (size_t)x->Ls * 24;
+  GBX_TRY(staging_buffer(n * sizeof(T), &stage));
+  const uint64_t total = g->vol * (uint64_t)spinor_nchunk(x->prec) * x->M;
+  dispatch_prec<UnpackRefT<T>::template F>(x->prec, (const void *)x->d, (T *)stage, (const uint32_t *)g->site_of_slot, total,
+                                           rg, (uint32_t)x->Ls, g->stream);
+  GBX_HIP(hipGetLastError());
+  GBX_HIP(hipMemcpyAsync(ref, stage, n * sizeof(T), hipMemcpyDeviceToHost, g->stream));
+  GBX_HIP(hipStreamSynchronize(g->stream));
+  return GBX_OK;
+}
+
+extern "C" int gbx_spinor_upload_ref_f64(gbx_spinor x, const double *ref, int nsimd) { return spinor_upload_ref(x, ref, nsimd); }
+extern "C" int gbx_spinor_download_ref_f64(gbx_spinor x, double *ref, int nsimd) { return spinor_download_ref(x, ref, nsimd); }
+extern "C" int gbx_spinor_upload_ref_f32(gbx_spinor x, const float *ref, int nsimd) { return spinor_upload_ref(x, ref, nsimd); }
+extern "C" int gbx_spinor_download_ref_f32(gbx_spinor x, float *ref, int nsimd) { return spinor_download_ref(x, ref, nsimd); }
+
+extern "C" int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src)
+{
+  GBX_TRY(spinor_ok(dst));
+  GBX_TRY(spinor_ok(src));
+  GBX_CHECK(dst->geom == src->geom && dst->Ls == src->Ls && dst->nrhs == src->nrhs, GBX_EMISMATCH,
+            "gbx_spinor_copy: different geometry or shape");
+  gbx_geom_s *g = dst->geom;
+  if (dst == src) return GBX_OK;
+  if (dst->prec == src->prec) {
+    GBX_HIP(hipMemcpyAsync(dst->d, src->d, dst->bytes, hipMemcpyDeviceToDevice, g->stream));
+    return GBX_OK;
+  }
+  // The chunked layouts of different precisions place reals differently:
+  // convert point by point (all 24 reals of one (slot, m) per thread).
+  return spinor_convert_scaled(dst, src, 1.0, g->stream);
+}

@@ -1,0 +1,60 @@
+// Achievable-HBM-bandwidth probe: a grid-stride 16-byte copy kernel, the
+// "STREAM copy" against which the hopping kernels' roofline fraction is
+// read, and a plain copy of known size for calibrating byte counters.
+#include "gbx_internal.h"
+
+namespace {
+
+typedef unsigned int u4 __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(256) k_copy16(const u4 *__restrict__ src, u4 *__restrict__ dst, uint64_t n)
+{
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += stride)
+    __builtin_nontemporal_store(src[i], dst + i);
+}
+
+}  // namespace
+
+extern "C" int gbx_stream_copy_bw(uint64_t bytes, int nrep, double *gbps)
+{
+  GBX_CHECK(gbps, GBX_ENULL, "gbx_stream_copy_bw: NULL result");
+  GBX_CHECK(bytes >= 16 * 1024 && bytes % 16 == 0, GBX_EINVAL, "gbx_stream_copy_bw: bytes=%llu",
+            (unsigned long long)bytes);
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_stream_copy_bw: nrep=%d", nrep);
+  GBX_TRY(gbx_require_device());
+  void *a = nullptr, *b = nullptr;
+  GBX_HIP(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return gbx_fail(GBX_ENOMEM, "gbx_stream_copy_bw: cannot allocate %llu bytes", (unsigned long long)bytes);
+  }
+  int rc = GBX_OK;
+  hipStream_t st;
+  hipEvent_t e0, e1;
+  if (hipStreamCreate(&st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipStreamCreate failed");
+  if (rc == GBX_OK) {
+    (void)hipEventCreate(&e0);
+    (void)hipEventCreate(&e1);
+    (void)hipMemsetAsync(a, 0x3c, bytes, st);
+    const uint64_t n = bytes / 16;
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const dim3 grid((unsigned)(ncu * 16)), block(256);
+    hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);  // warm-up
+    (void)hipEventRecord(e0, st);
+    for (int r = 0; r < nrep; r++) hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);
+    (void)hipEventRecord(e1, st);
+    if (hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "copy kernel failed");
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rc == GBX_OK) *gbps = 2.0 * (double)bytes * nrep / (ms * 1e-3) / 1e9;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+  }
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}

@@ -1,0 +1,383 @@
+"""Python mirror of the GridBench hot-path interface on top of libgbx.
+
+* `dslash_kernel(...)` keeps the reference entry point's argument list and
+  meaning (WilsonKernelsHand.h:17-29): host arrays in the reference layout,
+  nbr/prm tables, nrep applications, returns microseconds -- with the SIMD
+  lane count and the dagger explicit.
+* `Geometry`, `GaugeField`, `SpinorField`, `dhop` work on device-resident
+  fields in the library's SoA layout (the fast path).
+* Seeded input generation and stencil tables in the reference's form.
+"""
+import ctypes as C
+
+import numpy as np
+
+from ._lib import (lib, check, ptr, ivec, KCfg, CGParams, CGStats, GbxError, PREC_BY_NAME, PREC_NAMES,
+                   F64, F32, F16, BF16)
+
+FLOPS_PER_SITE = 1320  # bench.cc:145, per 5D site and application
+
+
+def _prec(p):
+    if isinstance(p, str):
+        return PREC_BY_NAME[p]
+    return int(p)
+
+
+def device_count():
+    return lib.gbx_device_count()
+
+
+# ---------------------------------------------------------------- generators
+def gen_gaussian(seed, stream, index0, n):
+    out = np.empty(n, dtype=np.float64)
+    check(lib.gbx_gen_gaussian(seed, stream, index0, n, ptr(out, C.c_double)), "gbx_gen_gaussian")
+    return out
+
+
+def gen_su3_links(dims, seed):
+    """Forward links U_mu(x), lexicographic [V][4][3][3] complex128."""
+    V = int(np.prod(dims))
+    out = np.empty((V, 4, 3, 3, 2), dtype=np.float64)
+    check(lib.gbx_gen_su3_links(ivec(dims), seed, ptr(out, C.c_double)), "gbx_gen_su3_links")
+    return out
+
+
+def double_links(dims, links):
+    """Doubled field [V][8][3][3][2]: U[x][mu] = U_mu(x), U[x][mu+4] = U_mu(x-mu)^dagger."""
+    links = np.ascontiguousarray(links, dtype=np.float64)
+    V = int(np.prod(dims))
+    out = np.empty((V, 8, 3, 3, 2), dtype=np.float64)
+    check(lib.gbx_double_links(ivec(dims), ptr(links, C.c_double), ptr(out, C.c_double)), "gbx_double_links")
+    return out
+
+
+def gen_doubled_gauge(dims, seed):
+    return double_links(dims, gen_su3_links(dims, seed))
+
+
+def gen_spinor(dims, Ls, nrhs, seed):
+    """Gaussian spinor, lexicographic [nrhs][V][Ls][4][3][2] float64."""
+    V = int(np.prod(dims))
+    out = np.empty((nrhs, V, Ls, 4, 3, 2), dtype=np.float64)
+    check(lib.gbx_gen_spinor(ivec(dims), Ls, nrhs, seed, ptr(out, C.c_double)), "gbx_gen_spinor")
+    return out
+
+
+# -------------------------------------------------------------------- tables
+def table_generate(dims, nsimd, Ls):
+    """(nbr, prm) in the reference's form for a periodic lattice."""
+    nsite = int(np.prod(dims)) // nsimd
+    nbr = np.empty(nsite * Ls * 8, dtype=np.uint64)
+    prm = np.empty(nsite * Ls * 8, dtype=np.uint8)
+    check(lib.gbx_table_generate(ivec(dims), nsimd, Ls, ptr(nbr, C.c_uint64), ptr(prm, C.c_uint8)),
+          "gbx_table_generate")
+    return nbr, prm
+
+
+def table_recognise(nsimd, nsite, Ls, nbr, prm):
+    """Lattice dims if (nbr, prm) are canonical periodic tables, else None."""
+    dims = (C.c_int * 4)()
+    rc = lib.gbx_table_recognise(nsimd, nsite, Ls, ptr(nbr, C.c_uint64), ptr(prm, C.c_uint8), dims)
+    return tuple(dims) if rc == 0 else None
+
+
+# ------------------------------------------------------------------- drop-in
+def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False):
+    """The reference entry dslash_kernel<Simd>(nrep, U, out, in, nbr, nsite, Ls, prm)
+    (WilsonKernelsHand.h:17-29) on numpy arrays in the reference layout.
+    float64 arrays run the double kernel, float32 the single.  Returns the
+    microseconds of applications 1..nrep-1, like the reference."""
+    usec = C.c_double(0.0)
+    if U.dtype == np.float64:
+        fn, ct = lib.gbx_dslash_kernel_f64, C.c_double
+    elif U.dtype == np.float32:
+        fn, ct = lib.gbx_dslash_kernel_f32, C.c_float
+    else:
+        raise TypeError("U must be float64 or float32")
+    for a in (out, in_):
+        if a.dtype != U.dtype:
+            raise TypeError("U, out, in must share a dtype")
+    if nbr.dtype != np.uint64 or prm.dtype != np.uint8:
+        raise TypeError("nbr must be uint64 and prm uint8")
+    check(fn(int(nrep), int(nsimd), int(bool(dag)), ptr(U, ct), ptr(out, ct), ptr(in_, ct), ptr(nbr, C.c_uint64),
+             int(nsite), int(Ls), ptr(prm, C.c_uint8), C.byref(usec)), "dslash_kernel")
+    return usec.value
+
+
+def dslash_last_path():
+    """1 = the last dslash_kernel call ran the SoA kernel, 2 = the table-driven kernel."""
+    return lib.gbx_dslash_last_path()
+
+
+# ----------------------------------------------------------- device objects
+class Geometry:
+    def __init__(self, dims, block=None):
+        self.dims = tuple(int(d) for d in dims)
+        h = C.c_void_p()
+        check(lib.gbx_geom_create(ivec(self.dims), ivec(block) if block is not None else None, C.byref(h)),
+              "gbx_geom_create")
+        self.h = h
+        self.volume = int(np.prod(self.dims))
+
+    def sync(self):
+        check(lib.gbx_sync(self.h), "gbx_sync")
+
+    def close(self):
+        if self.h:
+            lib.gbx_geom_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class GaugeField:
+    def __init__(self, geom, prec="f64", comp12=False):
+        self.geom = geom
+        self.prec = _prec(prec)
+        self.comp12 = bool(comp12)
+        h = C.c_void_p()
+        check(lib.gbx_gauge_create(geom.h, self.prec, int(self.comp12), C.byref(h)), "gbx_gauge_create")
+        self.h = h
+
+    def upload(self, doubled):
+        """Doubled field, lexicographic float64 [V][8][3][3][2]."""
+        a = np.ascontiguousarray(doubled, dtype=np.float64)
+        assert a.size == self.geom.volume * 144
+        check(lib.gbx_gauge_upload(self.h, ptr(a, C.c_double)), "gbx_gauge_upload")
+        return self
+
+    def upload_ref(self, ref, nsimd):
+        if ref.dtype == np.float64:
+            check(lib.gbx_gauge_upload_ref_f64(self.h, ptr(ref, C.c_double), nsimd), "gbx_gauge_upload_ref_f64")
+        else:
+            check(lib.gbx_gauge_upload_ref_f32(self.h, ptr(ref, C.c_float), nsimd), "gbx_gauge_upload_ref_f32")
+        return self
+
+    def close(self):
+        if self.h:
+            lib.gbx_gauge_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class SpinorField:
+    def __init__(self, geom, Ls, nrhs=1, prec="f64"):
+        self.geom = geom
+        self.Ls = int(Ls)
+        self.nrhs = int(nrhs)
+        self.prec = _prec(prec)
+        h = C.c_void_p()
+        check(lib.gbx_spinor_create(geom.h, self.Ls, self.nrhs, self.prec, C.byref(h)), "gbx_spinor_create")
+        self.h = h
+
+    @property
+    def shape(self):
+        return (self.nrhs, self.geom.volume, self.Ls, 4, 3, 2)
+
+    def upload(self, lex):
+        a = np.ascontiguousarray(lex, dtype=np.float64)
+        assert a.size == int(np.prod(self.shape)), (a.shape, self.shape)
+        check(lib.gbx_spinor_upload(self.h, ptr(a, C.c_double)), "gbx_spinor_upload")
+        return self
+
+    def download(self):
+        out = np.empty(self.shape, dtype=np.float64)
+        check(lib.gbx_spinor_download(self.h, ptr(out, C.c_double)), "gbx_spinor_download")
+        return out
+
+    def upload_rhs(self, r, lex):
+        """Right-hand side r from lexicographic float64 [V][Ls][4][3][2]."""
+        a = np.ascontiguousarray(lex, dtype=np.float64)
+        assert a.size == self.geom.volume * self.Ls * 24
+        check(lib.gbx_spinor_upload_rhs(self.h, int(r), ptr(a, C.c_double)), "gbx_spinor_upload_rhs")
+        return self
+
+    def download_rhs(self, r):
+        out = np.empty(self.shape[1:], dtype=np.float64)
+        check(lib.gbx_spinor_download_rhs(self.h, int(r), ptr(out, C.c_double)), "gbx_spinor_download_rhs")
+        return out
+
+    def upload_ref(self, ref, nsimd):
+        if ref.dtype == np.float64:
+            check(lib.gbx_spinor_upload_ref_f64(self.h, ptr(ref, C.c_double), nsimd), "gbx_spinor_upload_ref_f64")
+        else:
+            check(lib.gbx_spinor_upload_ref_f32(self.h, ptr(ref, C.c_float), nsimd), "gbx_spinor_upload_ref_f32")
+        return self
+
+    def download_ref(self, nsimd, dtype=np.float64):
+        out = np.empty(self.geom.volume * self.Ls * 24, dtype=dtype)
+        if dtype == np.float64:
+            check(lib.gbx_spinor_download_ref_f64(self.h, ptr(out, C.c_double), nsimd), "gbx_spinor_download_ref_f64")
+        else:
+            check(lib.gbx_spinor_download_ref_f32(self.h, ptr(out, C.c_float), nsimd), "gbx_spinor_download_ref_f32")
+        return out
+
+    def copy_from(self, src):
+        check(lib.gbx_spinor_copy(self.h, src.h), "gbx_spinor_copy")
+        return self
+
+    def norm2(self):
+        n = C.c_double()
+        check(lib.gbx_spinor_norm2(self.h, C.byref(n)), "gbx_spinor_norm2")
+        return n.value
+
+    def dot(self, other):
+        re, im = C.c_double(), C.c_double()
+        check(lib.gbx_spinor_dot(self.h, other.h, C.byref(re), C.byref(im)), "gbx_spinor_dot")
+        return complex(re.value, im.value)
+
+    def axpy(self, a, x):
+        """self = a*x + self"""
+        check(lib.gbx_spinor_axpy(float(a), x.h, self.h), "gbx_spinor_axpy")
+        return self
+
+    def scale(self, a):
+        check(lib.gbx_spinor_scale(self.h, float(a)), "gbx_spinor_scale")
+        return self
+
+    def zero(self):
+        check(lib.gbx_spinor_zero(self.h), "gbx_spinor_zero")
+        return self
+
+    @property
+    def nbytes(self):
+        b = C.c_uint64()
+        check(lib.gbx_spinor_bytes(self.h, C.byref(b)), "gbx_spinor_bytes")
+        return b.value
+
+    def close(self):
+        if self.h:
+            lib.gbx_spinor_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def kcfg(spw, threads, xcd=1, nt=1, kind=0):
+    return KCfg(spw, threads, xcd, nt, kind)
+
+
+def dhop(U, out, inp, dag=False, cfg=None):
+    """out = Dhop inp (dag: Dhop^dagger) on device-resident fields."""
+    if cfg is None:
+        check(lib.gbx_dhop(U.h, out.h, inp.h, int(bool(dag))), "gbx_dhop")
+    else:
+        check(lib.gbx_dhop_cfg(U.h, out.h, inp.h, int(bool(dag)), C.byref(cfg)), "gbx_dhop_cfg")
+    return out
+
+
+def dhop_axpby(U, out, inp, alpha, beta, dag=False):
+    """out = alpha*inp + beta*Dhop inp."""
+    check(lib.gbx_dhop_axpby(U.h, out.h, inp.h, int(bool(dag)), float(alpha), float(beta)), "gbx_dhop_axpby")
+    return out
+
+
+def dhop_get_cfg(U, inp):
+    k = KCfg()
+    check(lib.gbx_dhop_get_cfg(U.h, inp.h, C.byref(k)), "gbx_dhop_get_cfg")
+    return k
+
+
+def dhop_time(U, out, inp, dag=False, cfg=None, nrep=10):
+    """Average microseconds per application (HIP events on the kernels' stream)."""
+    us = C.c_double()
+    check(lib.gbx_dhop_time(U.h, out.h, inp.h, int(bool(dag)), C.byref(cfg) if cfg is not None else None, int(nrep),
+                            C.byref(us)), "gbx_dhop_time")
+    return us.value
+
+
+def tune(U, out, inp, dag=False, cache_path=None):
+    best = KCfg()
+    us = C.c_double()
+    check(lib.gbx_tune(U.h, out.h, inp.h, int(bool(dag)), cache_path.encode() if cache_path else None, C.byref(best),
+                       C.byref(us)), "gbx_tune")
+    return best, us.value
+
+
+def cg_solve(U_hi, U_lo, x, b, kappa, tol=1e-10, inner_tol=1e-3, max_outer=50, max_inner=2000):
+    """Defect-correction CG for M^dag M x = b, M = 1 - kappa Dhop; U_lo None = FP64 only."""
+    p = CGParams(kappa, tol, inner_tol, max_outer, max_inner)
+    st = CGStats()
+    if U_lo is None:
+        rc = lib.gbx_cg_solve_f64(U_hi.h, x.h, b.h, C.byref(p), C.byref(st))
+    else:
+        rc = lib.gbx_cg_solve(U_hi.h, U_lo.h, x.h, b.h, C.byref(p), C.byref(st))
+    check(rc, "gbx_cg_solve")
+    return st.as_dict()
+
+
+class Batch:
+    """Many independent lattices in one persistent, graph-captured launch."""
+
+    def __init__(self, gauges, outs, ins, dag=False):
+        n = len(gauges)
+        assert n == len(outs) == len(ins)
+        arr = lambda objs: (C.c_void_p * n)(*[o.h for o in objs])
+        h = C.c_void_p()
+        check(lib.gbx_batch_create(n, arr(gauges), arr(outs), arr(ins), int(bool(dag)), C.byref(h)),
+              "gbx_batch_create")
+        self.h = h
+        self._keep = (gauges, outs, ins)
+
+    def run(self):
+        check(lib.gbx_batch_run(self.h), "gbx_batch_run")
+
+    def time(self, nrep=10):
+        us = C.c_double()
+        check(lib.gbx_batch_time(self.h, int(nrep), C.byref(us)), "gbx_batch_time")
+        return us.value
+
+    def info(self):
+        items, wgs = C.c_uint64(), C.c_int()
+        check(lib.gbx_batch_info(self.h, C.byref(items), C.byref(wgs)), "gbx_batch_info")
+        return items.value, wgs.value
+
+    def close(self):
+        if self.h:
+            lib.gbx_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def stream_copy_bw(nbytes=2 << 30, nrep=20):
+    """Achievable HBM bandwidth (GB/s, read + write) of a streaming copy."""
+    g = C.c_double()
+    check(lib.gbx_stream_copy_bw(int(nbytes), int(nrep), C.byref(g)), "gbx_stream_copy_bw")
+    return g.value
+
+
+# ------------------------------------------------------------ work models
+def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
+    """Algorithmic HBM bytes per (5D site, right-hand side): in + out spinor
+    once, the doubled gauge field once per 4D site."""
+    bs = {F64: 8, F32: 4, F16: 2, BF16: 2}[_prec(prec_spinor)]
+    bu = {F64: 8, F32: 4, F16: 2, BF16: 2}[_prec(prec_gauge)]
+    RU = 12 if comp12 else 18
+    return 48 * bs + 8 * RU * bu / (Ls * nrhs)
+
+
+__all__ = [
+    "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
+    "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
+    "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "dhop", "dhop_axpby",
+    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site",
+]

@@ -1,0 +1,259 @@
+/*
+ * gbx.h -- C-ABI of libgbx, an MI355X (gfx950) implementation of GridBench's
+ * domain-wall-fermion Wilson hopping term (the "Hand" DhopSite kernel applied
+ * on every one of Ls fifth-dimension slices that share each gauge link).
+ *
+ * Plain C: opaque handles, plain pointers and sizes, int status codes
+ * (GBX_OK = 0).  No call ever exits the process.  Handles may be used from
+ * different host threads as long as one handle is not used by two threads at
+ * once.  Reference files are cited as path:line of the GridBench sources.
+ */
+#ifndef GBX_H
+#define GBX_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------------- */
+/* Status codes.  The reference has none: a CUDA failure prints and calls  */
+/* exit(0) (Macros.h:46-51) and allocation failure asserts                 */
+/* (AlignedAllocator.h:47-50).                                             */
+/* ---------------------------------------------------------------------- */
+#define GBX_OK           0
+#define GBX_EINVAL       1  /* argument value out of range / inconsistent      */
+#define GBX_ENULL        2  /* required pointer is NULL                        */
+#define GBX_EALIAS       3  /* output buffer aliases an input buffer           */
+#define GBX_ETABLE       4  /* nbr/prm table entry invalid                     */
+#define GBX_ENOMEM       5  /* host or device allocation failed                */
+#define GBX_EHIP         6  /* HIP runtime error                               */
+#define GBX_ENODEV       7  /* no usable GPU                                   */
+#define GBX_EUNSUPPORTED 8  /* valid request this build does not implement     */
+#define GBX_EMISMATCH    9  /* objects of different geometry/shape combined    */
+#define GBX_ENOCONV     10  /* iterative solver did not reach its tolerance    */
+
+const char *gbx_strerror(int status);
+/* Message describing the last failing call made by this thread ("" if none). */
+const char *gbx_last_error(void);
+int gbx_version(void);
+/* Number of usable HIP devices (0 on a host without a GPU; never fails). */
+int gbx_device_count(void);
+
+/* Storage precisions of spinors and gauge links.  Arithmetic is FP64 for
+ * GBX_F64 storage and FP32 for the others. */
+#define GBX_F64  0
+#define GBX_F32  1
+#define GBX_F16  2
+#define GBX_BF16 3
+
+/* ---------------------------------------------------------------------- */
+/* Drop-in replacement for the reference kernel entry                      */
+/*   template<class Simd> double dslash_kernel(int nrep, Simd *U,          */
+/*     Simd *out, Simd *in, uint64_t *nbr, uint64_t nsite, uint64_t Ls,    */
+/*     uint8_t *prm)                  (WilsonKernelsHand.h:17-29)          */
+/* with the Simd type's lane count (SimdVector.h:89-91) and the dagger     */
+/* made explicit.  Host arrays in the reference layout (bench.cc:105-138): */
+/*   U   [nsite][8][3][3][nsimd][re,im]   doubled gauge field              */
+/*   in, out [nsite*Ls][4][3][nsimd][re,im]                                */
+/*   nbr, prm [nsite*Ls*8], entry ss*8+dir (TableGenerate.cc:72-91)        */
+/* U, in, nbr, prm are read-only; out is fully overwritten; out == in (or  */
+/* any overlap) is rejected with GBX_EALIAS.  nrep applications are run;   */
+/* *usec receives the device time of applications 1..nrep-1 in            */
+/* microseconds (the reference's timer also starts at rep 1,               */
+/* WilsonKernelsHandCpu.h:408-409), or of the single one when nrep == 1.   */
+/* Every nbr entry must be < nsite*Ls and every prm entry 0 or 1.          */
+/* If the tables are the canonical periodic tables of some lattice (as     */
+/* TableGenerate.cc writes them) the fast SoA kernel runs; otherwise a     */
+/* table-driven kernel honours the tables exactly.                         */
+/* dag = 0: Dhop; dag = 1: Dhop^dagger = gamma5 Dhop gamma5 (the           */
+/* reference has no dagger kernel).                                        */
+/* ---------------------------------------------------------------------- */
+int gbx_dslash_kernel_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
+                          const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
+int gbx_dslash_kernel_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
+                          const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
+/* Which path the last gbx_dslash_kernel_* call of this thread took:
+ * 1 = fast SoA kernel (tables recognised), 2 = table-driven kernel. */
+int gbx_dslash_last_path(void);
+
+/* ---------------------------------------------------------------------- */
+/* Stencil tables in the reference's form (TableGenerate.cc:72-91, which  */
+/* needs Grid): nbr[mu + 8*(s + Ls*site)] = 5D outer index of the          */
+/* neighbour, prm = 1 when the hop wraps a SIMD-split dimension.  Periodic */
+/* lattice dims[4] = {Lx,Ly,Lz,Lt}, nsimd in {1,2,4,8,16} lanes split over */
+/* T, Z, Y, X in that order.  nbr/prm hold nsite*Ls*8 entries,             */
+/* nsite = Lx*Ly*Lz*Lt/nsimd.                                              */
+/* ---------------------------------------------------------------------- */
+int gbx_table_generate(const int dims[4], int nsimd, uint64_t Ls, uint64_t *nbr, uint8_t *prm);
+/* Recognise canonical tables: on success dims[4] receives the lattice.    */
+/* Returns GBX_OK if canonical, GBX_ETABLE (with a message) if not.        */
+int gbx_table_recognise(int nsimd, uint64_t nsite, uint64_t Ls, const uint64_t *nbr, const uint8_t *prm,
+                        int dims[4]);
+
+/* ---------------------------------------------------------------------- */
+/* Seeded synthetic inputs (host side; bit-reproducible on any x86 host).  */
+/* Counter-based generator: every value is a function of (seed, stream,    */
+/* index) only, so any sub-range can be produced independently.            */
+/* ---------------------------------------------------------------------- */
+/* n i.i.d. N(0,1) reals, values index0 .. index0+n-1 of stream `stream`. */
+int gbx_gen_gaussian(uint64_t seed, uint64_t stream, uint64_t index0, uint64_t n, double *out);
+/* Forward links U_mu(x), lexicographic [V][4][3][3][re,im]: rows 0 and 1  */
+/* complex N(0,1), Gram-Schmidt orthonormalised; row 2 = conj(r0 x r1),   */
+/* so U is unitary with det 1.                                             */
+int gbx_gen_su3_links(const int dims[4], uint64_t seed, double *links);
+/* Doubled field U[x][mu] = U_mu(x), U[x][mu+4] = U_mu(x-mu)^dagger,      */
+/* lexicographic [V][8][3][3][re,im] (the field TableGenerate.cc:54 dumps). */
+int gbx_double_links(const int dims[4], const double *links, double *doubled);
+/* Gaussian spinor, lexicographic [nrhs][V][Ls][4][3][re,im]; right-hand   */
+/* side r uses stream r.                                                   */
+int gbx_gen_spinor(const int dims[4], int Ls, int nrhs, uint64_t seed, double *psi);
+
+/* ---------------------------------------------------------------------- */
+/* Device-resident objects.                                                */
+/* ---------------------------------------------------------------------- */
+typedef struct gbx_geom_s *gbx_geom;     /* 4D periodic lattice + device site order */
+typedef struct gbx_gauge_s *gbx_gauge;   /* doubled gauge field on the GPU           */
+typedef struct gbx_spinor_s *gbx_spinor; /* Ls x nrhs 5D spinor field on the GPU     */
+
+/* block (may be NULL = default) is the 4D traversal block: sites are stored */
+/* and visited block by block, which keeps stencil neighbours in cache.     */
+int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out);
+int gbx_geom_destroy(gbx_geom g);
+int gbx_geom_volume(gbx_geom g, uint64_t *vol);
+/* The geometry's stream (hipStream_t) on which all its work runs. */
+int gbx_geom_stream(gbx_geom g, void **stream);
+int gbx_sync(gbx_geom g);
+
+/* comp12 = 1 stores 12 reals per link (two rows) and rebuilds the third   */
+/* row in the kernel; the links must then be SU(3).                        */
+int gbx_gauge_create(gbx_geom g, int prec, int comp12, gbx_gauge *out);
+int gbx_gauge_destroy(gbx_gauge u);
+/* Doubled field, lexicographic FP64 [V][8][3][3][re,im]. */
+int gbx_gauge_upload(gbx_gauge u, const double *doubled);
+/* Doubled field in the reference host layout [nsite][8][3][3][nsimd][2]. */
+int gbx_gauge_upload_ref_f64(gbx_gauge u, const double *ref, int nsimd);
+int gbx_gauge_upload_ref_f32(gbx_gauge u, const float *ref, int nsimd);
+
+int gbx_spinor_create(gbx_geom g, int Ls, int nrhs, int prec, gbx_spinor *out);
+int gbx_spinor_destroy(gbx_spinor x);
+int gbx_spinor_bytes(gbx_spinor x, uint64_t *bytes);
+/* Lexicographic FP64 [nrhs][V][Ls][4][3][re,im]. */
+int gbx_spinor_upload(gbx_spinor x, const double *lex);
+int gbx_spinor_download(gbx_spinor x, double *lex);
+/* One right-hand side r, lexicographic FP64 [V][Ls][4][3][re,im]. */
+int gbx_spinor_upload_rhs(gbx_spinor x, int r, const double *lex);
+int gbx_spinor_download_rhs(gbx_spinor x, int r, double *lex);
+/* Reference host layout [nsite*Ls][4][3][nsimd][re,im] (nrhs must be 1). */
+int gbx_spinor_upload_ref_f64(gbx_spinor x, const double *ref, int nsimd);
+int gbx_spinor_download_ref_f64(gbx_spinor x, double *ref, int nsimd);
+int gbx_spinor_upload_ref_f32(gbx_spinor x, const float *ref, int nsimd);
+int gbx_spinor_download_ref_f32(gbx_spinor x, float *ref, int nsimd);
+/* dst = src converted between precisions (same geometry, Ls, nrhs). */
+int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src);
+
+/* ---------------------------------------------------------------------- */
+/* The hopping term on device-resident fields: out = Dhop in (dag = 0) or  */
+/* Dhop^dagger in (dag = 1).  Same operator as the reference kernel        */
+/* (WilsonKernelsHandCpu.h:426-446): on every s-slice and right-hand side, */
+/* Dhop psi(x) = sum_mu (1-g_mu) U_mu(x) psi(x+mu)                          */
+/*             + (1+g_mu) U_mu(x-mu)^dagger psi(x-mu).                      */
+/* The gauge and spinor precisions must match (FP64 with FP64, else any   */
+/* of FP32/FP16/BF16 for both).  Asynchronous on the geometry's stream.   */
+/* ---------------------------------------------------------------------- */
+int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag);
+/* out = alpha*in + beta*Dhop(in): with alpha = 1, beta = -kappa this is   */
+/* M = 1 - kappa*Dhop in one pass.                                         */
+int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta);
+
+/* Explicit launch configuration of the VALU kernel. */
+typedef struct {
+  int spw;     /* 4D sites per workgroup (their links are staged in LDS) */
+  int threads; /* workgroup size, multiple of 64, <= 1024               */
+  int xcd;     /* 1: give each XCD a contiguous range of sites           */
+  int nt;      /* 1: non-temporal output stores                          */
+  int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS)    */
+} gbx_kcfg;
+int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg);
+/* Configuration gbx_dhop would use now (autotuned if gbx_tune ran). */
+int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg);
+/* Average device time per application over nrep applications after one  */
+/* warm-up, from HIP events recorded on the stream the kernels run on.    */
+int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg, int nrep,
+                  double *usec);
+
+/* Autotuner: times candidate configurations (sites per workgroup,         */
+/* workgroup size, XCD ordering, store kind, VALU vs MFMA) on these        */
+/* fields and records the fastest for this (precision, comp12, Ls, nrhs,   */
+/* geometry) signature; gbx_dhop uses it from then on.  Optional cache     */
+/* file (JSON lines) is read first and appended to (may be NULL).          */
+int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
+             double *best_usec);
+
+/* ---------------------------------------------------------------------- */
+/* Vector algebra on spinors (used by the solver; deterministic order).    */
+/* ---------------------------------------------------------------------- */
+/* Re<x,y> and |x|^2 accumulated in FP64 with a fixed reduction tree.     */
+int gbx_spinor_dot(gbx_spinor x, gbx_spinor y, double *re, double *im);
+int gbx_spinor_norm2(gbx_spinor x, double *n2);
+/* y = a*x + y */
+int gbx_spinor_axpy(double a, gbx_spinor x, gbx_spinor y);
+int gbx_spinor_scale(gbx_spinor x, double a);
+int gbx_spinor_zero(gbx_spinor x);
+
+/* ---------------------------------------------------------------------- */
+/* Defect-correction CG for  M^dagger M x = b,  M = 1 - kappa Dhop.       */
+/* The outer loop runs in FP64 on the GPU (true residual, reliable update */
+/* of x, global reductions); each inner CG runs with gauge and vectors in */
+/* the precision of u_lo (FP32, FP16 or BF16, FP32 arithmetic) on a       */
+/* rescaled residual.  u_hi must be FP64; x and b FP64, nrhs = 1.         */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  double kappa;       /* hopping parameter, |kappa| < 1/8                 */
+  double tol;         /* target |b - M^dag M x| / |b|                     */
+  double inner_tol;   /* relative reduction per inner solve (e.g. 1e-3)   */
+  int max_outer;
+  int max_inner;
+} gbx_cg_params;
+typedef struct {
+  int outer_iters;
+  int inner_iters;    /* total over all inner solves                      */
+  double true_resid;  /* final |b - M^dag M x| / |b| computed in FP64     */
+  double seconds;     /* wall time of the solve                           */
+  int hi_applies;     /* FP64 M^dag M applications                         */
+  int lo_applies;     /* low-precision M^dag M applications                */
+} gbx_cg_stats;
+int gbx_cg_solve(gbx_gauge u_hi, gbx_gauge u_lo, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p,
+                 gbx_cg_stats *st);
+/* The same solve in FP64 only (no inner low-precision solves). */
+int gbx_cg_solve_f64(gbx_gauge u, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p, gbx_cg_stats *st);
+
+/* ---------------------------------------------------------------------- */
+/* Batched application over many independent lattices (each its own      */
+/* geometry, gauge field, Ls and nrhs; FP32/FP16/BF16 storage) in one      */
+/* persistent launch captured in a HIP graph.                              */
+/* ---------------------------------------------------------------------- */
+typedef struct gbx_batch_s *gbx_batch;
+int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag,
+                     gbx_batch *b);
+int gbx_batch_destroy(gbx_batch b);
+/* Replays the graph once on the batch's stream and waits for it. */
+int gbx_batch_run(gbx_batch b);
+int gbx_batch_time(gbx_batch b, int nrep, double *usec);
+/* Work items and persistent workgroups of the launch. */
+int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups);
+
+/* ---------------------------------------------------------------------- */
+/* HBM bandwidth probe: device-to-device copy of `bytes` with the          */
+/* library's own streaming kernel; returns GB/s (1e9 B/s) counting read + */
+/* write bytes.                                                            */
+/* ---------------------------------------------------------------------- */
+int gbx_stream_copy_bw(uint64_t bytes, int nrep, double *gbps);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* GBX_H */

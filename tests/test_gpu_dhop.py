@@ -1,0 +1,228 @@
+"""GPU parity of the hopping kernels against the oracle (the CPU restatement
+of the reference Hand kernel, WilsonKernelsHandCpu.h:385-451).
+
+Tolerances (relative L2 over the whole output, vs the FP64 oracle):
+  FP64 storage           <= 1e-14   (FMA contraction on the GPU only)
+  FP32 storage           <= 1e-6
+  FP16 storage           <= 2e-3    (10-bit mantissa inputs, FP32 arithmetic)
+  BF16 storage           <= 1.6e-2  (7-bit mantissa inputs, FP32 arithmetic)
+"""
+import numpy as np
+import pytest
+
+from conftest import rel_l2
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f64": 1e-14, "f32": 1e-6, "f16": 2e-3, "bf16": 1.6e-2}
+
+
+def run_dhop(gbx, p, prec, dag=False, comp12=False, block=None, cfg=None, psi=None):
+    geom = gbx.Geometry(p.dims, block)
+    U = gbx.GaugeField(geom, prec, comp12).upload(p.dbl)
+    src = gbx.SpinorField(geom, p.Ls, p.nrhs, prec).upload(p.psi if psi is None else psi)
+    dst = gbx.SpinorField(geom, p.Ls, p.nrhs, prec)
+    gbx.dhop(U, dst, src, dag=dag, cfg=cfg)
+    return dst.download()
+
+
+@pytest.mark.parametrize("dims,Ls", [((4, 4, 4, 4), 4), ((8, 8, 8, 8), 8), ((4, 6, 8, 2), 3), ((2, 2, 2, 2), 1),
+                                     ((6, 4, 2, 10), 5)])
+@pytest.mark.parametrize("dag", [False, True])
+def test_f64_vs_oracle(gpu, gbx, problem, dims, Ls, dag):
+    p = problem(dims, Ls)
+    got = run_dhop(gbx, p, "f64", dag=dag)[0]
+    want = p.oracle_lex(dag=dag)
+    err = rel_l2(got, want)
+    print(f"dims={dims} Ls={Ls} dag={dag} rel_l2={err:.3e}")
+    assert err <= TOL["f64"]
+
+
+@pytest.mark.parametrize("prec", ["f32", "f16", "bf16"])
+@pytest.mark.parametrize("comp12", [False, True])
+@pytest.mark.parametrize("dag", [False, True])
+def test_low_precision_vs_f64_oracle(gpu, gbx, problem, prec, comp12, dag):
+    p = problem((8, 4, 4, 8), 6)
+    got = run_dhop(gbx, p, prec, dag=dag, comp12=comp12)[0]
+    want = p.oracle_lex(dag=dag)
+    err = rel_l2(got, want)
+    print(f"prec={prec} comp12={comp12} dag={dag} rel_l2_vs_f64_oracle={err:.3e}")
+    assert err <= TOL[prec]
+
+
+def test_f32_vs_f32_oracle(gpu, gbx, problem):
+    p = problem((4, 4, 4, 8), 4)
+    got = run_dhop(gbx, p, "f32")[0]
+    want = p.oracle_lex(dtype=np.float32)
+    assert rel_l2(got, want) <= 1e-6
+
+
+def test_f64_comp12(gpu, gbx, problem):
+    p = problem((4, 4, 6, 4), 4)
+    got = run_dhop(gbx, p, "f64", comp12=True)[0]
+    assert rel_l2(got, p.oracle_lex()) <= 1e-14
+
+
+@pytest.mark.parametrize("nrhs", [2, 5])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_multi_rhs(gpu, gbx, problem, nrhs, prec):
+    p = problem((4, 4, 4, 4), 4, nrhs=nrhs)
+    got = run_dhop(gbx, p, prec)
+    for r in range(nrhs):
+        err = rel_l2(got[r], p.oracle_lex(rhs=r))
+        print(f"rhs={r} rel_l2={err:.3e}")
+        assert err <= TOL[prec]
+
+
+@pytest.mark.parametrize("block", [(1, 1, 1, 1), (4, 2, 2, 2), (8, 4, 4, 4), (2, 4, 1, 8)])
+def test_traversal_blocks(gpu, gbx, problem, block):
+    p = problem((8, 4, 4, 8), 4)
+    got = run_dhop(gbx, p, "f64", block=block)[0]
+    assert rel_l2(got, p.oracle_lex()) <= 1e-14
+
+
+@pytest.mark.parametrize("spw,threads,xcd,nt", [(1, 64, 0, 0), (3, 64, 1, 1), (16, 256, 1, 0), (7, 128, 0, 1),
+                                                (64, 1024, 1, 1), (5, 512, 1, 1)])
+def test_launch_configs_identical(gpu, gbx, problem, spw, threads, xcd, nt):
+    """Every launch configuration computes the same bits (the work split
+    changes, the per-point arithmetic does not)."""
+    p = problem((4, 6, 4, 6), 4)
+    base = run_dhop(gbx, p, "f64")
+    got = run_dhop(gbx, p, "f64", cfg=gbx.kcfg(spw, threads, xcd, nt))
+    assert np.array_equal(base, got)
+
+
+def test_axpby(gpu, gbx, problem):
+    """out = a*in + b*Dhop(in) (the M = 1 - kappa Dhop form)."""
+    p = problem((4, 4, 4, 4), 4)
+    geom = gbx.Geometry(p.dims)
+    U = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    src = gbx.SpinorField(geom, p.Ls, 1, "f64").upload(p.psi)
+    dst = gbx.SpinorField(geom, p.Ls, 1, "f64")
+    for dag in (False, True):
+        gbx.dhop_axpby(U, dst, src, 1.0, -0.12, dag=dag)
+        want = p.psi[0] - 0.12 * p.oracle_lex(dag=dag)
+        assert rel_l2(dst.download()[0], want) <= 1e-14
+
+
+def test_errors_on_device_objects(gpu, gbx, problem):
+    p = problem((4, 4, 4, 4), 2)
+    geom = gbx.Geometry(p.dims)
+    U = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    a = gbx.SpinorField(geom, 2, 1, "f64").upload(p.psi)
+    b32 = gbx.SpinorField(geom, 2, 1, "f32")
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dhop(U, a, a)
+    assert e.value.status == 3  # GBX_EALIAS
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dhop(U, b32, a)
+    assert e.value.status == 9  # GBX_EMISMATCH
+    other = gbx.SpinorField(gbx.Geometry((4, 4, 4, 4)), 2, 1, "f64")
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dhop(U, other, a)
+    assert e.value.status == 9
+    with pytest.raises(gbx.GbxError):
+        gbx.dhop(U, gbx.SpinorField(geom, 2, 1, "f64"), a, cfg=gbx.kcfg(1, 100))  # threads not a multiple of 64
+
+
+def test_upload_download_roundtrip(gpu, gbx, problem):
+    p = problem((4, 6, 2, 4), 3, nrhs=2)
+    geom = gbx.Geometry(p.dims)
+    x = gbx.SpinorField(geom, 3, 2, "f64").upload(p.psi)
+    assert np.array_equal(x.download(), p.psi)
+    for prec, tol in (("f32", 1e-7), ("f16", 1e-3), ("bf16", 8e-3)):
+        y = gbx.SpinorField(geom, 3, 2, prec).copy_from(x)
+        assert rel_l2(y.download(), p.psi) <= tol
+        z = gbx.SpinorField(geom, 3, 2, prec).upload(p.psi)
+        assert np.array_equal(y.download(), z.download())
+
+
+# ----------------------------------------------------------- drop-in entry
+def ref_case(gbx, orc, dims, Ls, nsimd, dtype):
+    links = gbx.gen_su3_links(dims, 101)
+    dbl = gbx.double_links(dims, links)
+    psi = gbx.gen_spinor(dims, Ls, 1, 102)[0]
+    U = orc.pack_gauge(dims, nsimd, dbl).astype(dtype)
+    inp = orc.pack_spinor(dims, nsimd, Ls, psi).astype(dtype)
+    nbr, prm = orc.tables(dims, nsimd, Ls)
+    return U, inp, nbr, prm, int(np.prod(dims)) // nsimd
+
+
+@pytest.mark.parametrize("nsimd", [1, 2, 4, 8, 16])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("dag", [False, True])
+def test_dslash_kernel_dropin_canonical(gpu, gbx, orc, nsimd, dtype, dag):
+    dims, Ls = (4, 4, 4, 8), 4
+    U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, nsimd, dtype)
+    out = np.zeros_like(inp)
+    usec = gbx.dslash_kernel(3, U, out, inp, nbr, nsite, Ls, prm, nsimd, dag=dag)
+    assert gbx.dslash_last_path() == 1
+    want = orc.hand(U, inp, nbr, prm, nsite, Ls, nsimd, dag=dag)
+    tol = 1e-14 if dtype == np.float64 else 1e-6
+    assert rel_l2(out, want) <= tol
+    assert usec >= 0.0
+
+
+@pytest.mark.parametrize("nsimd", [1, 4, 8])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+def test_dslash_kernel_dropin_arbitrary_tables(gpu, gbx, orc, nsimd, dtype):
+    """Non-canonical tables (outer sites relabelled by a permutation) take
+    the table-driven kernel and must still match the oracle on the same
+    tables and arrays."""
+    dims, Ls = (4, 4, 4, 8), 3
+    U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, nsimd, dtype)
+    rng = np.random.default_rng(7)
+    perm = rng.permutation(nsite)  # new index of old outer site
+    inv = np.argsort(perm)
+    Ur = U.reshape(nsite, -1)[inv].reshape(-1).copy()
+    inr = inp.reshape(nsite, Ls, -1)[inv].reshape(-1).copy()
+    nb = nbr.reshape(nsite, Ls, 8)[inv]
+    nb = (perm[nb // Ls] * Ls + nb % Ls).astype(np.uint64).reshape(-1).copy()
+    pr = prm.reshape(nsite, Ls, 8)[inv].reshape(-1).copy()
+    out = np.zeros_like(inr)
+    gbx.dslash_kernel(2, Ur, out, inr, nb, nsite, Ls, pr, nsimd)
+    assert gbx.dslash_last_path() == 2
+    want = orc.hand(orc.aligned_copy(Ur), orc.aligned_copy(inr), orc.aligned_copy(nb), orc.aligned_copy(pr), nsite, Ls,
+                    nsimd)
+    tol = 1e-14 if dtype == np.float64 else 1e-6
+    assert rel_l2(out, want) <= tol
+
+
+def test_dslash_kernel_rejects_bad_tables(gpu, gbx, orc):
+    dims, Ls, nsimd = (4, 4, 4, 4), 2, 1
+    U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, nsimd, np.float64)
+    out = np.zeros_like(inp)
+    bad = nbr.copy()
+    bad[17] = nsite * Ls
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dslash_kernel(1, U, out, inp, bad, nsite, Ls, prm, nsimd)
+    assert e.value.status == 4  # GBX_ETABLE
+    badp = prm.copy()
+    badp[3] = 2
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dslash_kernel(1, U, out, inp, nbr, nsite, Ls, badp, nsimd)
+    assert e.value.status == 4
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dslash_kernel(1, U, inp, inp, nbr, nsite, Ls, prm, nsimd)
+    assert e.value.status == 3  # GBX_EALIAS
+
+
+# ----------------------------------------------------------- full size
+def test_full_size_24x4_ls16_f64(gpu, gbx, problem):
+    """24^4 x Ls16 FP64 (the benchmark workload) against the oracle run on the
+    host cores, plus size-independent properties."""
+    p = problem((24, 24, 24, 24), 16, gseed=201, sseed=202)
+    geom = gbx.Geometry(p.dims)
+    U = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    src = gbx.SpinorField(geom, 16, 1, "f64").upload(p.psi)
+    dst = gbx.SpinorField(geom, 16, 1, "f64")
+    gbx.dhop(U, dst, src)
+    got = dst.download()[0]
+    want = p.oracle_lex(nthreads=16)
+    err = rel_l2(got, want)
+    print(f"24^4 x 16 FP64 rel_l2 vs oracle = {err:.3e}")
+    assert err <= 1e-14
+    # linearity: Dhop(2 psi) = 2 Dhop(psi) exactly (scaling by 2 is exact)
+    src.scale(2.0)
+    gbx.dhop(U, dst, src)
+    assert np.array_equal(dst.download()[0], 2.0 * got)
