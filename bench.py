@@ -81,6 +81,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="only the FP64 1-RHS headline")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-tune", action="store_true",
+                    help="use the default launch configuration (profiling runs: no tuner launches)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hopping term on one "
@@ -104,9 +106,12 @@ def main():
     x64 = gbx.SpinorField(geom, LS, 1, "f64").upload_rhs(0, psi)
     y64 = gbx.SpinorField(geom, LS, 1, "f64")
 
-    log("tuning FP64 1-RHS")
-    best, best_us = gbx.tune(U64, y64, x64)
-    log(f"tuned: {best.as_dict()} {best_us:.1f} us")
+    if args.no_tune:
+        best = gbx.dhop_get_cfg(U64, x64)
+    else:
+        log("tuning FP64 1-RHS")
+        best, best_us = gbx.tune(U64, y64, x64)
+        log(f"tuned: {best.as_dict()} {best_us:.1f} us")
 
     # ------------------------------------------------ timed headline region
     for _ in range(args.warmup):

@@ -38,7 +38,9 @@ def check(status, where="libgbx"):
 
 
 class KCfg(C.Structure):
-    _fields_ = [("spw", C.c_int), ("threads", C.c_int), ("xcd", C.c_int), ("nt", C.c_int), ("kind", C.c_int)]
+    _fields_ = [("spw", C.c_int), ("threads", C.c_int), ("xcd", C.c_int), ("nt", C.c_int), ("kind", C.c_int),
+                ("mc", C.c_int), ("pf", C.c_int), ("wgpc", C.c_int),
+                ("cblk", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -78,6 +80,7 @@ PROTOTYPES = {
     "gbx_double_links": (i32, [pi32, pdbl, pdbl]),
     "gbx_gen_spinor": (i32, [pi32, i32, i32, u64, pdbl]),
     "gbx_geom_create": (i32, [pi32, pi32, C.POINTER(vp)]),
+    "gbx_geom_create_levels": (i32, [pi32, i32, pi32, C.POINTER(vp)]),
     "gbx_geom_destroy": (i32, [vp]),
     "gbx_geom_volume": (i32, [vp, pu64]),
     "gbx_geom_stream": (i32, [vp, C.POINTER(vp)]),

@@ -201,8 +201,9 @@ int apply_mdagm(const gbx_gauge_s *u, gbx_spinor_s *y, const gbx_spinor_s *x, gb
 {
   gbx_kcfg k;
   gbx_dhop_get_cfg((gbx_gauge)u, (gbx_spinor)x, &k);
-  if (k.kind != 0) k = gbx_kcfg{dhop_default_cfg(u, x).spw, dhop_default_cfg(u, x).threads, 1, 0, 0};
-  dhop_cfg c{k.spw, k.threads, k.xcd, 0};
+  dhop_cfg c = dhop_default_cfg(u, x);
+  if (k.kind == 0) c = dhop_cfg{k.spw, k.threads, k.xcd, 0, k.mc, 0, k.wgpc, k.cblk};
+  c.nt = 0;  // the solver reads its outputs back at once: keep them cached
   GBX_TRY(dhop_launch(u, t, x, 0, c, 1.0, -kappa, x, st));
   GBX_TRY(dhop_launch(u, y, t, 1, c, 1.0, -kappa, t, st));
   return GBX_OK;

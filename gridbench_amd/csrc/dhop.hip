@@ -19,12 +19,17 @@
 
 namespace {
 
-template <int P, bool DAG, bool C12, bool AXPBY>
-__global__ void __launch_bounds__(1024) k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr,
-                                               const void *__restrict__ in, void *__restrict__ out,
-                                               const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw,
-                                               uint32_t nwg, uint32_t per_xcd, int xcd, int nt,
-                                               typename dev::Traits<P>::acc alpha, typename dev::Traits<P>::acc beta)
+// Logical workgroup L covers sites q0 .. q0+spw-1 (site block L % nwg_sites)
+// and minor indices m0 .. m0+mc-1 (minor chunk L / nwg_sites): with mc < M
+// the lattice is swept once per minor chunk, so the 8-neighbour working set
+// is that of an mc-wide field (the links are re-read once per chunk).
+template <int P, bool DAG, bool C12, bool AXPBY, bool PF>
+__global__ void __launch_bounds__(PF ? 256 : 1024)
+    k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
+           void *__restrict__ out, const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mc,
+           uint32_t nwg_sites, uint32_t cblk, uint32_t nchunk, uint32_t nwg, uint32_t per_xcd, int xcd, int nt,
+           typename dev::Traits<P>::acc alpha,
+           typename dev::Traits<P>::acc beta)
 {
   typedef typename dev::Traits<P>::acc A;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -33,45 +38,63 @@ __global__ void __launch_bounds__(1024) k_dhop(const void *__restrict__ gauge, c
 
   const uint32_t L = xcd ? (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
   if (L >= nwg) return;  // uniform over the workgroup, before any barrier
-  const uint32_t q0 = L * spw;
-  const uint32_t nsl = min(spw, nslot - q0);
+  // cblk site groups are visited for every minor chunk before the next
+  // cblk groups (chunk-inner order); cblk == nwg_sites sweeps the lattice
+  // once per chunk (chunk-outer order).
+  const uint32_t per = cblk * nchunk, grp = L / per, r = L - grp * per;
+  const uint32_t chunk = r / cblk, b = grp * cblk + (r - chunk * cblk);
+  if (b >= nwg_sites) return;
+  const uint32_t q0 = b * spw, m0 = chunk * mc;
+  const uint32_t nsl = min(spw, nslot - q0), mcc = min(mc, M - m0);
   dev::stage_links<P, C12>(gauge, nbr, q0, nsl, sU, sN);
 
-  const uint32_t items = nsl * M;
+  const uint32_t items = nsl * mcc;
   for (uint32_t w = threadIdx.x; w < items; w += blockDim.x) {
-    const uint32_t sl = w / M, m = w - sl * M;
+    const uint32_t sl = w / mcc, m = m0 + (w - sl * mcc);
     A rr[12], ri[12];
-    dev::dhop_site<P, DAG>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    dev::dhop_site<P, DAG, PF>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
     dev::store_site<P, AXPBY>(out, center, q0 + sl, M, m, rr, ri, alpha, beta, nt != 0);
   }
 }
 
-template <int P, bool DAG, bool C12, bool AXPBY>
-void launch_t(dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge, const uint32_t *nbr, const void *in,
-              void *out, const void *center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t nwg, uint32_t per_xcd,
-              int xcd, int nt, double alpha, double beta)
+struct LaunchArgs {
+  dim3 grid, block;
+  size_t lds;
+  hipStream_t st;
+  const void *gauge;
+  const uint32_t *nbr;
+  const void *in;
+  void *out;
+  const void *center;
+  uint32_t nslot, M, spw, mc, nwg_sites, cblk, nchunk, nwg, per_xcd;
+  int xcd, nt;
+  double alpha, beta;
+};
+
+template <int P, bool DAG, bool C12, bool AXPBY, bool PF>
+void launch_t(const LaunchArgs &a)
 {
   typedef typename dev::Traits<P>::acc A;
-  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY>), grid, block, lds, st, gauge, nbr, in, out, center, nslot, M, spw, nwg,
-                     per_xcd, xcd, nt, (A)alpha, (A)beta);
+  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out, a.center,
+                     a.nslot, a.M, a.spw, a.mc, a.nwg_sites, a.cblk, a.nchunk, a.nwg, a.per_xcd, a.xcd, a.nt, (A)a.alpha,
+                     (A)a.beta);
+}
+
+template <int P, bool DAG, bool C12>
+void launch_dc(bool axpby, bool pf, const LaunchArgs &a)
+{
+  if (axpby) launch_t<P, DAG, C12, true, false>(a);
+  else if (pf) launch_t<P, DAG, C12, false, true>(a);
+  else launch_t<P, DAG, C12, false, false>(a);
 }
 
 template <int P>
-void launch_p(bool dag, bool c12, bool axpby, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge,
-              const uint32_t *nbr, const void *in, void *out, const void *center, uint32_t nslot, uint32_t M, uint32_t spw,
-              uint32_t nwg, uint32_t per_xcd, int xcd, int nt, double alpha, double beta)
+void launch_p(bool dag, bool c12, bool axpby, bool pf, const LaunchArgs &a)
 {
-#define GBX_L(D, C, X) \
-  launch_t<P, D, C, X>(grid, block, lds, st, gauge, nbr, in, out, center, nslot, M, spw, nwg, per_xcd, xcd, nt, alpha, beta)
-  if (!dag && !c12 && !axpby) GBX_L(false, false, false);
-  else if (!dag && !c12 && axpby) GBX_L(false, false, true);
-  else if (!dag && c12 && !axpby) GBX_L(false, true, false);
-  else if (!dag && c12 && axpby) GBX_L(false, true, true);
-  else if (dag && !c12 && !axpby) GBX_L(true, false, false);
-  else if (dag && !c12 && axpby) GBX_L(true, false, true);
-  else if (dag && c12 && !axpby) GBX_L(true, true, false);
-  else GBX_L(true, true, true);
-#undef GBX_L
+  if (!dag && !c12) launch_dc<P, false, false>(axpby, pf, a);
+  else if (!dag && c12) launch_dc<P, false, true>(axpby, pf, a);
+  else if (dag && !c12) launch_dc<P, true, false>(axpby, pf, a);
+  else launch_dc<P, true, true>(axpby, pf, a);
 }
 
 size_t lds_bytes(int prec, int spw) { return (size_t)spw * 8 * 18 * (prec == GBX_F64 ? 8 : 4) + (size_t)spw * 8 * 4; }
@@ -86,9 +109,15 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
 {
   char buf[160];
   const gbx_geom_s *g = u->geom;
-  snprintf(buf, sizeof buf, "p%d_c%d_Ls%d_n%d_%dx%dx%dx%d_b%dx%dx%dx%d", in->prec, u->comp12, in->Ls, in->nrhs, g->dims[0],
-           g->dims[1], g->dims[2], g->dims[3], g->block[0], g->block[1], g->block[2], g->block[3]);
-  return buf;
+  snprintf(buf, sizeof buf, "p%d_c%d_Ls%d_n%d_%dx%dx%dx%d", in->prec, u->comp12, in->Ls, in->nrhs, g->dims[0], g->dims[1],
+           g->dims[2], g->dims[3]);
+  std::string s = buf;
+  for (int l = 0; l < g->nlevels; l++) {
+    snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
+             g->levels[l * 4 + 3]);
+    s += buf;
+  }
+  return s;
 }
 
 void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c)
@@ -137,6 +166,10 @@ dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in)
   }
   c.xcd = 1;
   c.nt = 1;
+  c.mc = 0;
+  c.pf = 0;
+  c.wgpc = 0;
+  c.cblk = 0;
   return c;
 }
 
@@ -144,7 +177,10 @@ int dhop_cfg_check(const dhop_cfg &c, const gbx_gauge_s *u, const gbx_spinor_s *
 {
   (void)in;
   GBX_CHECK(c.spw >= 1 && c.spw <= 4096, GBX_EINVAL, "kernel config: spw=%d", c.spw);
-  GBX_CHECK(c.threads >= 64 && c.threads <= 1024 && c.threads % 64 == 0, GBX_EINVAL, "kernel config: threads=%d", c.threads);
+  GBX_CHECK(c.threads >= 64 && c.threads <= (c.pf ? 256 : 1024) && c.threads % 64 == 0, GBX_EINVAL,
+            "kernel config: threads=%d (pf=%d)", c.threads, c.pf);
+  GBX_CHECK(c.mc >= 0, GBX_EINVAL, "kernel config: mc=%d", c.mc);
+  GBX_CHECK(c.wgpc >= 0 && c.wgpc <= 64, GBX_EINVAL, "kernel config: wgpc=%d", c.wgpc);
   GBX_CHECK(lds_bytes(u->prec, c.spw) <= 160 * 1024, GBX_EINVAL, "kernel config: spw=%d needs %zu B of LDS", c.spw,
             lds_bytes(u->prec, c.spw));
   return GBX_OK;
@@ -174,29 +210,42 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
     GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == in->geom, GBX_EMISMATCH,
               "dhop_axpby: center field shape differs");
   }
-  const uint32_t nslot = (uint32_t)u->geom->vol;
-  const uint32_t nwg = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
-  const uint32_t per_xcd = (nwg + 7) / 8;
-  const dim3 grid(c.xcd ? per_xcd * 8 : nwg), block((unsigned)c.threads);
-  const size_t lds = lds_bytes(u->prec, c.spw);
-  const void *cen = axpby ? center->d : nullptr;
+  LaunchArgs a;
+  a.nslot = (uint32_t)u->geom->vol;
+  a.M = in->M;
+  a.spw = (uint32_t)c.spw;
+  a.mc = (c.mc <= 0 || (uint32_t)c.mc > in->M) ? in->M : (uint32_t)c.mc;
+  a.nwg_sites = (a.nslot + a.spw - 1) / a.spw;
+  a.nchunk = (a.M + a.mc - 1) / a.mc;
+  a.cblk = (c.cblk <= 0 || (uint32_t)c.cblk >= a.nwg_sites) ? a.nwg_sites : (uint32_t)c.cblk;
+  const uint64_t ngrp = (a.nwg_sites + a.cblk - 1) / a.cblk;
+  const uint64_t nwg64 = ngrp * a.cblk * a.nchunk;
+  GBX_CHECK(nwg64 < (1ull << 31), GBX_EINVAL, "dhop: %llu workgroups", (unsigned long long)nwg64);
+  a.nwg = (uint32_t)nwg64;
+  a.per_xcd = (a.nwg + 7) / 8;
+  a.grid = dim3(c.xcd ? a.per_xcd * 8 : a.nwg);
+  a.block = dim3((unsigned)c.threads);
+  a.lds = lds_bytes(u->prec, c.spw);
+  if (c.wgpc > 0) {
+    const size_t pad = ((size_t)160 * 1024 / (size_t)c.wgpc) & ~(size_t)511;
+    if (pad > a.lds) a.lds = pad;
+  }
+  a.st = st;
+  a.gauge = u->d;
+  a.nbr = u->geom->nbr;
+  a.in = in->d;
+  a.out = out->d;
+  a.center = axpby ? center->d : nullptr;
+  a.xcd = c.xcd;
+  a.nt = c.nt;
+  a.alpha = alpha;
+  a.beta = beta;
+  const bool pf = c.pf != 0 && !axpby;
   switch (u->prec) {
-  case GBX_F64:
-    launch_p<GBX_F64>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
-                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
-    break;
-  case GBX_F32:
-    launch_p<GBX_F32>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
-                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
-    break;
-  case GBX_F16:
-    launch_p<GBX_F16>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
-                      (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
-    break;
-  default:
-    launch_p<GBX_BF16>(dag, u->comp12, axpby, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, cen, nslot, in->M,
-                       (uint32_t)c.spw, nwg, per_xcd, c.xcd, c.nt, alpha, beta);
-    break;
+  case GBX_F64: launch_p<GBX_F64>(dag, u->comp12, axpby, pf, a); break;
+  case GBX_F32: launch_p<GBX_F32>(dag, u->comp12, axpby, pf, a); break;
+  case GBX_F16: launch_p<GBX_F16>(dag, u->comp12, axpby, pf, a); break;
+  default: launch_p<GBX_BF16>(dag, u->comp12, axpby, pf, a); break;
   }
   GBX_HIP(hipGetLastError());
   return GBX_OK;
@@ -214,6 +263,10 @@ static gbx_kcfg to_kcfg(const dhop_cfg &c)
   k.xcd = c.xcd;
   k.nt = c.nt;
   k.kind = 0;
+  k.mc = c.mc;
+  k.pf = c.pf;
+  k.wgpc = c.wgpc;
+  k.cblk = c.cblk;
   return k;
 }
 
@@ -224,6 +277,10 @@ static dhop_cfg from_kcfg(const gbx_kcfg &k)
   c.threads = k.threads;
   c.xcd = k.xcd;
   c.nt = k.nt;
+  c.mc = k.mc;
+  c.pf = k.pf;
+  c.wgpc = k.wgpc;
+  c.cblk = k.cblk;
   return c;
 }
 

@@ -12,6 +12,9 @@ namespace dev {
 
 enum { PXP = 0, PYP, PZP, PTP, PXM, PYM, PZM, PTM };
 
+__device__ __forceinline__ float dfma(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double dfma(double a, double b, double c) { return __builtin_fma(a, b, c); }
+
 // Projector kind used on leg `dir`.
 template <bool DAG, int DIR> struct LegProj {
   static constexpr int value = DAG ? DIR : (DIR ^ 4);
@@ -57,19 +60,44 @@ __device__ __forceinline__ void project(const A *v, A *hr, A *hi)
 }
 
 // u[sp*3+i] = sum_j U[i][j] h[sp*3+j]; U row-major complex, 18 reals.
+// FP64: one multiply + five fused multiply-adds per real output (840 FP64
+// VALU ops per 5D site instead of 1032 for the mul/fma/add form the
+// compiler derives from the plain expression).  FP32: the plain expression,
+// whose register schedule is far better for the 16-bit-storage kernels.
 template <typename A>
 __device__ __forceinline__ void su3_mul(const A *U, const A *hr, const A *hi, A *ur, A *ui)
 {
+  if constexpr (sizeof(A) == 4) {
+#pragma unroll
+    for (int sp = 0; sp < 2; sp++)
+#pragma unroll
+      for (int i = 0; i < 3; i++) {
+        A re = U[(i * 3) * 2] * hr[sp * 3] - U[(i * 3) * 2 + 1] * hi[sp * 3];
+        A im = U[(i * 3) * 2] * hi[sp * 3] + U[(i * 3) * 2 + 1] * hr[sp * 3];
+#pragma unroll
+        for (int j = 1; j < 3; j++) {
+          re += U[(i * 3 + j) * 2] * hr[sp * 3 + j] - U[(i * 3 + j) * 2 + 1] * hi[sp * 3 + j];
+          im += U[(i * 3 + j) * 2] * hi[sp * 3 + j] + U[(i * 3 + j) * 2 + 1] * hr[sp * 3 + j];
+        }
+        ur[sp * 3 + i] = re;
+        ui[sp * 3 + i] = im;
+      }
+    return;
+  }
 #pragma unroll
   for (int sp = 0; sp < 2; sp++)
 #pragma unroll
     for (int i = 0; i < 3; i++) {
-      A re = U[(i * 3) * 2] * hr[sp * 3] - U[(i * 3) * 2 + 1] * hi[sp * 3];
-      A im = U[(i * 3) * 2] * hi[sp * 3] + U[(i * 3) * 2 + 1] * hr[sp * 3];
+      A re = U[(i * 3) * 2] * hr[sp * 3];
+      A im = U[(i * 3) * 2] * hi[sp * 3];
+      re = dfma(-U[(i * 3) * 2 + 1], hi[sp * 3], re);
+      im = dfma(U[(i * 3) * 2 + 1], hr[sp * 3], im);
 #pragma unroll
       for (int j = 1; j < 3; j++) {
-        re += U[(i * 3 + j) * 2] * hr[sp * 3 + j] - U[(i * 3 + j) * 2 + 1] * hi[sp * 3 + j];
-        im += U[(i * 3 + j) * 2] * hi[sp * 3 + j] + U[(i * 3 + j) * 2 + 1] * hr[sp * 3 + j];
+        re = dfma(U[(i * 3 + j) * 2], hr[sp * 3 + j], re);
+        im = dfma(U[(i * 3 + j) * 2], hi[sp * 3 + j], im);
+        re = dfma(-U[(i * 3 + j) * 2 + 1], hi[sp * 3 + j], re);
+        im = dfma(U[(i * 3 + j) * 2 + 1], hr[sp * 3 + j], im);
       }
       ur[sp * 3 + i] = re;
       ui[sp * 3 + i] = im;
@@ -165,19 +193,21 @@ __device__ __forceinline__ void stage_links(const void *__restrict__ gauge, cons
   __syncthreads();
 }
 
-// One stencil leg for minor index m: gather neighbour `nb`, project,
-// multiply by U (18 reals in LDS), reconstruct into r.
-template <int P, int K, bool FIRST>
-__device__ __forceinline__ void leg(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
-                                    const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
-                                    typename Traits<P>::acc *ri)
+// Gather the 24 reals of neighbour slot `nb`, minor index m.
+template <int P>
+__device__ __forceinline__ void load_nb(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
+                                        typename Traits<P>::acc *v)
 {
-  typedef typename Traits<P>::acc A;
   constexpr int NCH = Traits<P>::NCH, R = Traits<P>::R;
-  A v[24];
   const uint64_t base = (uint64_t)nb * NCH * M + m;
 #pragma unroll
-  for (int k = 0; k < NCH; k++) load_chunk<P, A>(in, base + (uint64_t)k * M, v + k * R);
+  for (int k = 0; k < NCH; k++) load_chunk<P, typename Traits<P>::acc>(in, base + (uint64_t)k * M, v + k * R);
+}
+
+// project, multiply by U (18 reals in LDS), reconstruct into r
+template <int K, bool FIRST, typename A>
+__device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri)
+{
   A hr[6], hi[6], ur[6], ui[6];
   project<K>(v, hr, hi);
   su3_mul(U, hr, hi, ur, ui);
@@ -185,19 +215,48 @@ __device__ __forceinline__ void leg(const void *__restrict__ in, uint32_t nb, ui
 }
 
 // Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
-template <int P, bool DAG>
+// PF = false: gather and compute leg by leg.  PF = true: the gather of leg
+// d+1 is issued before leg d is computed (two gather buffers in registers),
+// which keeps a second set of loads in flight per wavefront.
+template <int P, bool DAG, bool PF>
 __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
                                           const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
                                           typename Traits<P>::acc *ri)
 {
-  leg<P, LegProj<DAG, 0>::value, true>(in, nb[0], M, m, U + 0 * 18, rr, ri);
-  leg<P, LegProj<DAG, 1>::value, false>(in, nb[1], M, m, U + 1 * 18, rr, ri);
-  leg<P, LegProj<DAG, 2>::value, false>(in, nb[2], M, m, U + 2 * 18, rr, ri);
-  leg<P, LegProj<DAG, 3>::value, false>(in, nb[3], M, m, U + 3 * 18, rr, ri);
-  leg<P, LegProj<DAG, 4>::value, false>(in, nb[4], M, m, U + 4 * 18, rr, ri);
-  leg<P, LegProj<DAG, 5>::value, false>(in, nb[5], M, m, U + 5 * 18, rr, ri);
-  leg<P, LegProj<DAG, 6>::value, false>(in, nb[6], M, m, U + 6 * 18, rr, ri);
-  leg<P, LegProj<DAG, 7>::value, false>(in, nb[7], M, m, U + 7 * 18, rr, ri);
+  typedef typename Traits<P>::acc A;
+  if (!PF) {
+    A v[24];
+#define GBX_LEG(D, FIRST)                                                  \
+    load_nb<P>(in, nb[D], M, m, v);                                        \
+    leg_compute<LegProj<DAG, D>::value, FIRST>(v, U + D * 18, rr, ri);
+    GBX_LEG(0, true)
+    GBX_LEG(1, false)
+    GBX_LEG(2, false)
+    GBX_LEG(3, false)
+    GBX_LEG(4, false)
+    GBX_LEG(5, false)
+    GBX_LEG(6, false)
+    GBX_LEG(7, false)
+#undef GBX_LEG
+  } else {
+    A v0[24], v1[24];
+    load_nb<P>(in, nb[0], M, m, v0);
+    load_nb<P>(in, nb[1], M, m, v1);
+    leg_compute<LegProj<DAG, 0>::value, true>(v0, U + 0 * 18, rr, ri);
+    load_nb<P>(in, nb[2], M, m, v0);
+    leg_compute<LegProj<DAG, 1>::value, false>(v1, U + 1 * 18, rr, ri);
+    load_nb<P>(in, nb[3], M, m, v1);
+    leg_compute<LegProj<DAG, 2>::value, false>(v0, U + 2 * 18, rr, ri);
+    load_nb<P>(in, nb[4], M, m, v0);
+    leg_compute<LegProj<DAG, 3>::value, false>(v1, U + 3 * 18, rr, ri);
+    load_nb<P>(in, nb[5], M, m, v1);
+    leg_compute<LegProj<DAG, 4>::value, false>(v0, U + 4 * 18, rr, ri);
+    load_nb<P>(in, nb[6], M, m, v0);
+    leg_compute<LegProj<DAG, 5>::value, false>(v1, U + 5 * 18, rr, ri);
+    load_nb<P>(in, nb[7], M, m, v1);
+    leg_compute<LegProj<DAG, 6>::value, false>(v0, U + 6 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 7>::value, false>(v1, U + 7 * 18, rr, ri);
+  }
 }
 
 // Write the 4-spinor result (optionally out = alpha*center + beta*result).

@@ -63,7 +63,9 @@ inline int spinor_nchunk(int p) { return 24 * prec_bytes(p) / 16; }
 struct gbx_geom_s {
   int dims[4];
   uint64_t vol;           // 4D sites
-  int block[4];           // traversal block shape
+  int block[4];           // innermost traversal block shape
+  int nlevels = 1;        // traversal blocking levels
+  int levels[4 * GBX_MAX_LEVELS] = {0};
   int device;
   hipStream_t stream;
   // slot <-> lexicographic site permutation (host and device copies)
@@ -102,6 +104,10 @@ struct dhop_cfg {
   int threads;   // workgroup size
   int xcd;       // 1: XCD-aware workgroup order (contiguous slot ranges per XCD)
   int nt;        // 1: non-temporal output stores
+  int mc;        // minor indices per workgroup (0 = all Ls*nrhs)
+  int pf;        // 1: two-buffer gather pipeline (workgroups of <= 256 threads)
+  int wgpc;      // cap on resident workgroups per CU via padded LDS (0 = none)
+  int cblk;      // site groups per minor-chunk cycle (0 = whole lattice per chunk)
 };
 
 // out = alpha*in_c + beta*Dhop(in) where in_c is `center` (may be null when alpha == 0).

@@ -297,35 +297,43 @@ void geom_release(gbx_geom_s *g)
 extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out)
 {
   GBX_CHECK(dims && out, GBX_ENULL, "gbx_geom_create: NULL argument");
+  int blk[4];
+  for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : (d == 0 ? dims[0] : largest_divisor_le(dims[d] > 0 ? dims[d] : 1, 4));
+  return gbx_geom_create_levels(dims, 1, blk, out);
+}
+
+// Slot order: lexicographic over the outermost blocks, then, inside each
+// block of level l, lexicographic over its level l-1 sub-blocks, down to
+// lexicographic order of the sites of a level-0 block.
+extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out)
+{
+  GBX_CHECK(dims && out && blocks, GBX_ENULL, "gbx_geom_create_levels: NULL argument");
   *out = nullptr;
+  GBX_CHECK(nlevels >= 1 && nlevels <= GBX_MAX_LEVELS, GBX_EINVAL, "gbx_geom_create_levels: nlevels=%d", nlevels);
   uint64_t vol = 1;
   for (int d = 0; d < 4; d++) {
     GBX_CHECK(dims[d] >= 1 && dims[d] <= 1024, GBX_EINVAL, "gbx_geom_create: dims[%d]=%d", d, dims[d]);
     vol *= (uint64_t)dims[d];
   }
   GBX_CHECK(vol <= (1ull << 30), GBX_EINVAL, "gbx_geom_create: volume %llu too large", (unsigned long long)vol);
-  int blk[4];
-  for (int d = 0; d < 4; d++) {
-    if (block) {
-      GBX_CHECK(block[d] >= 1 && dims[d] % block[d] == 0, GBX_EINVAL, "gbx_geom_create: block[%d]=%d does not divide %d", d,
-                block[d], dims[d]);
-      blk[d] = block[d];
-    } else {
-      blk[d] = d == 0 ? dims[0] : largest_divisor_le(dims[d], 4);
+  for (int l = 0; l < nlevels; l++)
+    for (int d = 0; d < 4; d++) {
+      const int b = blocks[l * 4 + d], outer = l + 1 < nlevels ? blocks[(l + 1) * 4 + d] : dims[d];
+      GBX_CHECK(b >= 1 && outer % b == 0, GBX_EINVAL, "gbx_geom_create: level %d block[%d]=%d does not divide %d", l, d, b,
+                outer);
     }
-  }
   GBX_TRY(gbx_require_device());
   gbx_geom_s *g = new gbx_geom_s();
+  g->nlevels = nlevels;
   for (int d = 0; d < 4; d++) {
     g->dims[d] = dims[d];
-    g->block[d] = blk[d];
+    g->block[d] = blocks[d];
   }
+  for (int l = 0; l < nlevels * 4; l++) g->levels[l] = blocks[l];
   g->vol = vol;
   GBX_HIP(hipGetDevice(&g->device));
   g->site_of_slot_h.resize(vol);
   g->slot_of_site_h.resize(vol);
-  const int nb[4] = {dims[0] / blk[0], dims[1] / blk[1], dims[2] / blk[2], dims[3] / blk[3]};
-  const uint64_t bvol = (uint64_t)blk[0] * blk[1] * blk[2] * blk[3];
   for (uint64_t site = 0; site < vol; site++) {
     int x[4];
     uint64_t r = site;
@@ -333,12 +341,25 @@ extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *
       x[d] = (int)(r % (uint64_t)dims[d]);
       r /= (uint64_t)dims[d];
     }
-    uint64_t bi = 0, wi = 0;
-    for (int d = 3; d >= 0; d--) {
-      bi = bi * (uint64_t)nb[d] + (uint64_t)(x[d] / blk[d]);
-      wi = wi * (uint64_t)blk[d] + (uint64_t)(x[d] % blk[d]);
+    // outermost grid of top-level blocks
+    const int *top = blocks + (nlevels - 1) * 4;
+    uint64_t slot = 0;
+    for (int d = 3; d >= 0; d--) slot = slot * (uint64_t)(dims[d] / top[d]) + (uint64_t)(x[d] / top[d]);
+    for (int l = nlevels - 1; l >= 1; l--) {
+      const int *bo = blocks + l * 4, *bi = blocks + (l - 1) * 4;
+      uint64_t w = 0, n = 1;
+      for (int d = 3; d >= 0; d--) {
+        w = w * (uint64_t)(bo[d] / bi[d]) + (uint64_t)((x[d] % bo[d]) / bi[d]);
+        n *= (uint64_t)(bo[d] / bi[d]);
+      }
+      slot = slot * n + w;
     }
-    const uint64_t slot = bi * bvol + wi;
+    uint64_t w = 0, n = 1;
+    for (int d = 3; d >= 0; d--) {
+      w = w * (uint64_t)blocks[d] + (uint64_t)(x[d] % blocks[d]);
+      n *= (uint64_t)blocks[d];
+    }
+    slot = slot * n + w;
     g->slot_of_site_h[site] = (uint32_t)slot;
     g->site_of_slot_h[slot] = (uint32_t)site;
   }

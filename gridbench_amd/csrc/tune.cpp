@@ -40,43 +40,73 @@ int time_cfg(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, in
   return rc;
 }
 
-std::vector<gbx_kcfg> candidates(const gbx_gauge_s *u, const gbx_spinor_s *in)
+// Minor-chunk sizes worth trying: all of M, and divisors of M down to 8.
+std::vector<int> minor_chunks(uint32_t M)
+{
+  std::vector<int> v{0};
+  for (uint32_t c = M / 2; c >= 8; c /= 2)
+    if (M % c == 0) v.push_back((int)c);
+  if (M > 16 && M % 12 == 0) v.push_back(12);
+  return v;
+}
+
+// Stage 1: the work shape -- minor chunk, sites per workgroup, workgroup
+// size, gather pipeline -- with XCD ordering and streaming stores on.
+std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s *in)
 {
   std::vector<gbx_kcfg> v;
   const uint32_t M = in->M;
   const int spws[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
-  for (int spw : spws) {
-    const uint64_t items = (uint64_t)spw * M;
-    if (items > 8192) continue;
-    std::vector<int> thr;
-    if (items <= 1024 && items % 64 == 0) thr.push_back((int)items);
-    for (int t : {128, 256, 512}) {
-      if ((uint64_t)t < items && (items % t == 0 || items > 4u * (uint64_t)t)) thr.push_back(t);
-    }
-    if (items < 1024 && items % 64 != 0) thr.push_back((int)((items + 63) / 64 * 64));
-    for (int t : thr) {
-      dhop_cfg c{spw, t, 1, 1};
-      if (dhop_cfg_check(c, u, in) != GBX_OK) continue;
-      for (int xcd = 0; xcd <= 1; xcd++)
-        for (int nt = 0; nt <= 1; nt++) {
-          gbx_kcfg k{spw, t, xcd, nt, 0};
-          v.push_back(k);
+  for (int mc : minor_chunks(M)) {
+    const uint32_t mw = mc ? (uint32_t)mc : M;  // minor indices per workgroup
+    for (int spw : spws) {
+      const uint64_t items = (uint64_t)spw * mw;
+      if (items > 4096 || items < 64) continue;
+      std::vector<int> thr;
+      if (items <= 1024 && items % 64 == 0) thr.push_back((int)items);
+      for (int t : {128, 256, 512})
+        if ((uint64_t)t < items && items % t == 0 && (uint64_t)t != items) thr.push_back(t);
+      if (items < 1024 && items % 64 != 0) thr.push_back((int)((items + 63) / 64 * 64));
+      for (int t : thr)
+        for (int pf = 0; pf <= 1; pf++) {
+          dhop_cfg c{spw, t, 1, 1, mc, pf, 0, 0};
+          if (dhop_cfg_check(c, u, in) != GBX_OK) continue;
+          v.push_back(gbx_kcfg{spw, t, 1, 1, 0, mc, pf, 0, 0});
         }
     }
   }
   gbx_clear_error();
   if (mfma_supported(u, in)) {
-    for (int spw : {1, 2, 4})
-      for (int xcd = 0; xcd <= 1; xcd++) v.push_back(gbx_kcfg{spw, 256, xcd, 1, 1});
+    for (int spw : {1, 2, 4}) v.push_back(gbx_kcfg{spw, 256, 1, 1, 1, 0, 0, 0, 0});
   }
+  return v;
+}
+
+// Stage 2: around the best shape -- XCD ordering, store kind, and the cap on
+// resident workgroups per CU (fewer sites in flight per XCD = a working set
+// its 4 MB L2 holds, at the cost of latency hiding).
+std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
+{
+  std::vector<gbx_kcfg> v;
+  for (int xcd = 0; xcd <= 1; xcd++)
+    for (int nt = 0; nt <= 1; nt++)
+      for (int wgpc : {0, 2, 3, 4, 6, 8, 12, 16}) {
+        gbx_kcfg k = b;
+        k.xcd = xcd;
+        k.nt = nt;
+        k.wgpc = wgpc;
+        v.push_back(k);
+      }
   return v;
 }
 
 bool parse_line(const std::string &line, std::string &sig, gbx_kcfg &k, double &us)
 {
   char s[200];
-  if (sscanf(line.c_str(), "{\"sig\": \"%199[^\"]\", \"spw\": %d, \"threads\": %d, \"xcd\": %d, \"nt\": %d, \"kind\": %d, \"usec\": %lf}",
-             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &us) != 7)
+  if (sscanf(line.c_str(),
+             "{\"sig\": \"%199[^\"]\", \"spw\": %d, \"threads\": %d, \"xcd\": %d, \"nt\": %d, \"kind\": %d, \"mc\": %d, "
+             "\"pf\": %d, \"wgpc\": %d, \"usec\": %lf}",
+             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &k.mc, &k.pf, &k.wgpc, &us) != 10)
     return false;
   sig = s;
   return true;
@@ -90,7 +120,7 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
   GBX_CHECK(u && out && in && best && best_usec, GBX_ENULL, "gbx_tune: NULL argument");
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_tune: dag=%d", dag);
   const std::string sig = dhop_signature(u, in);
-  *best = gbx_kcfg{0, 0, 0, 0, 0};
+  *best = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
   *best_usec = 0.0;
   if (cache_path) {
     std::ifstream f(cache_path);
@@ -113,30 +143,34 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
       gbx_clear_error();
     }
   }
-  const std::vector<gbx_kcfg> cands = candidates(u, in);
+  const std::vector<gbx_kcfg> cands = shape_candidates(u, in);
   GBX_CHECK(!cands.empty(), GBX_EINVAL, "gbx_tune: no valid configuration");
-  // Repetitions: about 20 ms of kernel time per candidate, at least 2.
+  // Repetitions: about 10 ms of kernel time per candidate, at least 2.
   double us0 = 0.0;
   gbx_kcfg def;
   gbx_dhop_get_cfg(u, in, &def);
   GBX_TRY(time_cfg(u, out, in, dag, def, 1, &us0));
-  int nrep = (int)(20000.0 / (us0 > 1.0 ? us0 : 1.0));
+  int nrep = (int)(10000.0 / (us0 > 1.0 ? us0 : 1.0));
   if (nrep < 2) nrep = 2;
   if (nrep > 50) nrep = 50;
   double bus = 1e300;
   gbx_kcfg bk = def;
-  for (const gbx_kcfg &k : cands) {
-    double us;
-    if (time_cfg(u, out, in, dag, k, nrep, &us) != GBX_OK) {
-      gbx_clear_error();
-      continue;
+  auto search = [&](const std::vector<gbx_kcfg> &list) {
+    for (const gbx_kcfg &k : list) {
+      double us;
+      if (time_cfg(u, out, in, dag, k, nrep, &us) != GBX_OK) {
+        gbx_clear_error();
+        continue;
+      }
+      if (us < bus) {
+        bus = us;
+        bk = k;
+      }
     }
-    if (us < bus) {
-      bus = us;
-      bk = k;
-    }
-  }
+  };
+  search(cands);
   GBX_CHECK(bus < 1e300, GBX_EHIP, "gbx_tune: every candidate failed");
+  if (bk.kind == 0) search(refine_candidates(bk));
   *best = bk;
   *best_usec = bus;
   dhop_remember_cfg(sig, bk);
@@ -144,7 +178,8 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
     std::ofstream f(cache_path, std::ios::app);
     if (f)
       f << "{\"sig\": \"" << sig << "\", \"spw\": " << bk.spw << ", \"threads\": " << bk.threads << ", \"xcd\": " << bk.xcd
-        << ", \"nt\": " << bk.nt << ", \"kind\": " << bk.kind << ", \"usec\": " << bus << "}\n";
+        << ", \"nt\": " << bk.nt << ", \"kind\": " << bk.kind << ", \"mc\": " << bk.mc << ", \"pf\": " << bk.pf
+        << ", \"wgpc\": " << bk.wgpc << ", \"usec\": " << bus << "}\n";
   }
   return GBX_OK;
 }

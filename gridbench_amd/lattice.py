@@ -112,11 +112,20 @@ def dslash_last_path():
 
 # ----------------------------------------------------------- device objects
 class Geometry:
-    def __init__(self, dims, block=None):
+    """Periodic 4D lattice on the GPU.  `block` is the traversal block; or
+    `levels` = [innermost block, ..., outermost block] for a multi-level
+    (blocks-of-blocks) site order."""
+
+    def __init__(self, dims, block=None, levels=None):
         self.dims = tuple(int(d) for d in dims)
         h = C.c_void_p()
-        check(lib.gbx_geom_create(ivec(self.dims), ivec(block) if block is not None else None, C.byref(h)),
-              "gbx_geom_create")
+        if levels is not None:
+            flat = [int(b) for lv in levels for b in lv]
+            check(lib.gbx_geom_create_levels(ivec(self.dims), len(levels), ivec(flat), C.byref(h)),
+                  "gbx_geom_create_levels")
+        else:
+            check(lib.gbx_geom_create(ivec(self.dims), ivec(block) if block is not None else None, C.byref(h)),
+                  "gbx_geom_create")
         self.h = h
         self.volume = int(np.prod(self.dims))
 
@@ -267,8 +276,8 @@ class SpinorField:
             pass
 
 
-def kcfg(spw, threads, xcd=1, nt=1, kind=0):
-    return KCfg(spw, threads, xcd, nt, kind)
+def kcfg(spw, threads, xcd=1, nt=1, kind=0, mc=0, pf=0, wgpc=0, cblk=0):
+    return KCfg(spw, threads, xcd, nt, kind, mc, pf, wgpc, cblk)
 
 
 def dhop(U, out, inp, dag=False, cfg=None):

@@ -121,6 +121,10 @@ typedef struct gbx_spinor_s *gbx_spinor; /* Ls x nrhs 5D spinor field on the GPU
 /* block (may be NULL = default) is the 4D traversal block: sites are stored */
 /* and visited block by block, which keeps stencil neighbours in cache.     */
 int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out);
+/* Multi-level traversal order: blocks[l*4 .. l*4+3] is the level-l block,  */
+/* innermost first; each level divides the next and the last divides dims.  */
+#define GBX_MAX_LEVELS 4
+int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out);
 int gbx_geom_destroy(gbx_geom g);
 int gbx_geom_volume(gbx_geom g, uint64_t *vol);
 /* The geometry's stream (hipStream_t) on which all its work runs. */
@@ -175,6 +179,14 @@ typedef struct {
   int xcd;     /* 1: give each XCD a contiguous range of sites           */
   int nt;      /* 1: non-temporal output stores                          */
   int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS)    */
+  int mc;      /* minor indices (s, rhs) per workgroup; 0 = all Ls*nrhs  */
+  int pf;      /* 1: gather of leg d+1 issued before leg d is computed   */
+  int wgpc;    /* cap on resident workgroups per CU (0 = none): the LDS  */
+               /* request is padded to 160 KiB / wgpc, which shrinks the */
+               /* set of sites in flight per XCD to what its L2 holds    */
+  int cblk;    /* with mc < Ls*nrhs: workgroup site-groups visited for   */
+               /* every minor chunk before moving on (0 = one sweep of   */
+               /* the lattice per chunk); keeps a block's links in L2    */
 } gbx_kcfg;
 int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg);
 /* Configuration gbx_dhop would use now (autotuned if gbx_tune ran). */
