@@ -97,16 +97,23 @@ std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
         k.wgpc = wgpc;
         v.push_back(k);
       }
+  if (b.mc > 0)  // chunk-inner visiting order: site groups per chunk cycle
+    for (int cblk : {16, 64, 256, 1024}) {
+      gbx_kcfg k = b;
+      k.cblk = cblk;
+      v.push_back(k);
+    }
   return v;
 }
 
 bool parse_line(const std::string &line, std::string &sig, gbx_kcfg &k, double &us)
 {
   char s[200];
+  k = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (sscanf(line.c_str(),
              "{\"sig\": \"%199[^\"]\", \"spw\": %d, \"threads\": %d, \"xcd\": %d, \"nt\": %d, \"kind\": %d, \"mc\": %d, "
-             "\"pf\": %d, \"wgpc\": %d, \"usec\": %lf}",
-             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &k.mc, &k.pf, &k.wgpc, &us) != 10)
+             "\"pf\": %d, \"wgpc\": %d, \"cblk\": %d, \"usec\": %lf}",
+             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &k.mc, &k.pf, &k.wgpc, &k.cblk, &us) != 11)
     return false;
   sig = s;
   return true;
@@ -179,7 +186,7 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
     if (f)
       f << "{\"sig\": \"" << sig << "\", \"spw\": " << bk.spw << ", \"threads\": " << bk.threads << ", \"xcd\": " << bk.xcd
         << ", \"nt\": " << bk.nt << ", \"kind\": " << bk.kind << ", \"mc\": " << bk.mc << ", \"pf\": " << bk.pf
-        << ", \"wgpc\": " << bk.wgpc << ", \"usec\": " << bus << "}\n";
+        << ", \"wgpc\": " << bk.wgpc << ", \"cblk\": " << bk.cblk << ", \"usec\": " << bus << "}\n";
   }
   return GBX_OK;
 }

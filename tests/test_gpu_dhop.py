@@ -226,3 +226,86 @@ def test_full_size_24x4_ls16_f64(gpu, gbx, problem):
     src.scale(2.0)
     gbx.dhop(U, dst, src)
     assert np.array_equal(dst.download()[0], 2.0 * got)
+
+
+# ----------------------------------------------------------- tuner, orders
+def test_tuner_picks_a_config_with_identical_results(gpu, gbx, problem, tmp_path):
+    p = problem((8, 8, 8, 8), 8)
+    geom = gbx.Geometry(p.dims)
+    U = gbx.GaugeField(geom, "f32").upload(p.dbl)
+    x = gbx.SpinorField(geom, 8, 1, "f32").upload(p.psi)
+    y = gbx.SpinorField(geom, 8, 1, "f32")
+    gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, 0, 0))
+    base = y.download()
+    cache = str(tmp_path / "tune.jsonl")
+    best, us = gbx.tune(U, y, x, cache_path=cache)
+    assert us > 0 and best.threads % 64 == 0
+    assert gbx.dhop_get_cfg(U, x).as_dict() == best.as_dict()
+    y.zero()
+    gbx.dhop(U, y, x)  # now runs the tuned configuration
+    assert np.array_equal(y.download(), base)
+    best2, _ = gbx.tune(U, y, x, cache_path=cache)  # served from the cache file
+    assert best2.as_dict() == best.as_dict()
+
+
+@pytest.mark.parametrize("levels", [[(2, 2, 2, 2)], [(2, 2, 2, 2), (4, 4, 4, 4)], [(1, 2, 1, 2), (2, 4, 4, 4), (4, 4, 8, 8)]])
+def test_multilevel_site_orders(gpu, gbx, problem, levels):
+    p = problem((4, 4, 8, 8), 4)
+    geom = gbx.Geometry(p.dims, levels=levels)
+    U = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    x = gbx.SpinorField(geom, 4, 1, "f64").upload(p.psi)
+    y = gbx.SpinorField(geom, 4, 1, "f64")
+    gbx.dhop(U, y, x)
+    assert rel_l2(y.download()[0], p.oracle_lex()) <= 1e-14
+
+
+@pytest.mark.parametrize("mc,cblk,pf", [(2, 0, 0), (4, 3, 1), (8, 7, 0), (4, 1000, 1)])
+def test_minor_chunk_configs(gpu, gbx, problem, mc, cblk, pf):
+    p = problem((4, 4, 4, 6), 4, nrhs=2)
+    geom = gbx.Geometry(p.dims)
+    U = gbx.GaugeField(geom, "f32").upload(p.dbl)
+    x = gbx.SpinorField(geom, 4, 2, "f32").upload(p.psi)
+    y = gbx.SpinorField(geom, 4, 2, "f32")
+    gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, 0, 0))
+    base = y.download()
+    y.zero()
+    gbx.dhop(U, y, x, cfg=gbx.kcfg(8, 64, 1, 1, 0, mc, pf, 4, cblk))
+    assert np.array_equal(y.download(), base)
+
+
+def test_cpp_host_bench(gpu):
+    """The C++ host program drives dslash_kernel<Simd> through gbx_dslash.hpp
+    and checks the result against the device-resident path."""
+    import os
+    import subprocess
+    exe = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "gridbench_amd", "dslash_bench")
+    for args in (["8", "8", "20", "d", "1"], ["8", "4", "5", "d", "8"], ["8", "8", "10", "f", "16"]):
+        r = subprocess.run([exe] + args, capture_output=True, text=True, timeout=120)
+        print(r.stdout)
+        assert r.returncode == 0, r.stdout + r.stderr
+        assert "path 1" in r.stdout
+
+
+def test_full_size_16rhs_f32_vs_f64(gpu, gbx):
+    """24^4 x Ls16 x 16 right-hand sides in FP32 storage: every right-hand side
+    against the FP64 kernel on the same input (<= 1e-6 relative)."""
+    dims, Ls, nrhs = (24, 24, 24, 24), 16, 16
+    V = 24 ** 4
+    geom = gbx.Geometry(dims)
+    dbl = gbx.gen_doubled_gauge(dims, 201)
+    U32 = gbx.GaugeField(geom, "f32").upload(dbl)
+    U64 = gbx.GaugeField(geom, "f64").upload(dbl)
+    x = gbx.SpinorField(geom, Ls, nrhs, "f32")
+    x64 = gbx.SpinorField(geom, Ls, 1, "f64")
+    y64 = gbx.SpinorField(geom, Ls, 1, "f64")
+    rhs = [0, 7, 15]
+    for r in range(nrhs):
+        x.upload_rhs(r, gbx.gen_gaussian(202, r, 0, V * Ls * 24))
+    y = gbx.SpinorField(geom, Ls, nrhs, "f32")
+    gbx.dhop(U32, y, x)
+    for r in rhs:
+        x64.upload_rhs(0, gbx.gen_gaussian(202, r, 0, V * Ls * 24))
+        gbx.dhop(U64, y64, x64)
+        err = rel_l2(y.download_rhs(r), y64.download_rhs(0))
+        print(f"rhs {r}: rel_l2(FP32 16-RHS vs FP64) = {err:.2e}")
+        assert err <= 1e-6
