@@ -8,9 +8,24 @@ one right-hand side, seeded SU(3) links and Gaussian spinor, periodic.  One
 before the timed region; the timed region is K applications bracketed by
 stream synchronisations.
 
-Also reported (same lattice, same seeds; `variants`): FP32 1-RHS, FP32 and
-FP16 storage with 16 right-hand sides, DhopDag, each with its throughput per
-right-hand side relative to FP64 1-RHS and its error against the FP64 result.
+Also reported, each with its own seeds:
+  variants   24^4 x Ls16: DhopDag FP64; FP32 1-RHS; FP32/FP16/BF16 storage
+             with 16 right-hand sides (+ FP16 with 12-real links); 24^4 x
+             Ls12 x 16 RHS in FP32 and BF16.  Throughput per right-hand side
+             relative to FP64 1-RHS, error of right-hand side 0 against the
+             FP64 kernel on the same input.
+  mixed      32^4 x Ls16: Dhop error of FP32/BF16/FP16/FP16+12-real storage
+             against FP64 on the same input.
+  cg         32^4 x Ls16: defect-correction CG on M^dag M x = b,
+             M = 1 - 0.12 Dhop, b Gaussian, to 1e-10 relative residual, inner
+             solves in FP32/BF16/FP16 (and FP64 only, for comparison).
+  batch      40 lattices L^4 x Ls x Nrhs (L in {8,12,16,20,24}, Ls in
+             {8,12,16}, Nrhs in 1..8, drawn with seed 505), FP32, one
+             persistent graph-captured launch; aggregate GFLOP/s and the
+             per-lattice error against FP64.
+
+Launch configurations come from the autotuner; its results for this machine
+are cached in gridbench_amd/tuned_mi355x.jsonl (re-validated on every run).
 
 Prints ONE JSON line.
 """
@@ -29,7 +44,8 @@ DIMS = (24, 24, 24, 24)
 LS = 16
 GAUGE_SEED, SPINOR_SEED = 201, 202  # seeds of the 24^4 x 16 configuration
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, AMD-published
-FP64_VECTOR_TFLOPS = 78.6
+KAPPA = 0.12
+TUNE_CACHE = os.path.join(ROOT, "gridbench_amd", "tuned_mi355x.jsonl")
 
 
 def log(msg):
@@ -37,8 +53,8 @@ def log(msg):
 
 
 def traffic_from_profiles():
-    """HBM bytes per launch of the FP64 kernel, from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-calibrated)."""
+    """HBM bytes per launch of the FP64 kernel, from the committed rocprofv3
+    PMC summary (TCC read/write requests, copy-calibrated)."""
     path = os.path.join(ROOT, "profiles", "dhop_f64_24x24x24x24_ls16_pmc.md")
     try:
         with open(path) as f:
@@ -73,6 +89,189 @@ def cpu_baseline(dbl, psi, threads, budget_s=12.0):
                       f"{threads} threads, -O3 -march=x86-64-v3)", "ms_per_application": round(dt * 1e3, 2)}
 
 
+def rel_err_device(gbx, geom, got, want64):
+    """|got - want| / |want| over every right-hand side, on the device (got in
+    any precision, want64 FP64 of the same shape)."""
+    d = gbx.SpinorField(geom, want64.Ls, want64.nrhs, "f64").copy_from(got)
+    d.axpy(-1.0, want64)
+    e = float(np.sqrt(d.norm2() / want64.norm2()))
+    d.close()
+    return e
+
+
+class Tuner:
+    def __init__(self, gbx, cache, retune):
+        self.gbx, self.cache = gbx, cache
+        if retune and cache and os.path.exists(cache):
+            os.remove(cache)
+
+    def __call__(self, U, y, x, dag=False):
+        return self.gbx.tune(U, y, x, dag=dag, cache_path=self.cache)
+
+
+def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64):
+    V = int(np.prod(DIMS))
+    out = []
+
+    def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False):
+        g = geom
+        sites5 = V * Ls
+        U = U64 if prec == "f64" else gbx.GaugeField(g, prec, comp12).upload(dbl)
+        x = gbx.SpinorField(g, Ls, nrhs, prec)
+        if Ls == LS:
+            x.fill_gaussian(SPINOR_SEED + 1000)  # right-hand sides 1.. ; 0 is the headline spinor
+            x.upload_rhs(0, psi)
+            want = ref64_dag if dag else ref64
+        else:
+            x.fill_gaussian(SPINOR_SEED + 2000)
+            x64 = gbx.SpinorField(g, Ls, nrhs, "f64").copy_from(x)
+            w = gbx.SpinorField(g, Ls, nrhs, "f64")
+            gbx.dhop(U64, w, x64, dag=dag)
+        y = gbx.SpinorField(g, Ls, nrhs, prec)
+        cfg, _ = tune(U, y, x, dag=dag)
+        us = gbx.dhop_time(U, y, x, dag=dag, nrep=20)
+        gbx.dhop(U, y, x, dag=dag)
+        if Ls == LS:
+            got = y.download_rhs(0)
+            err = float(np.linalg.norm((got - want).ravel()) / np.linalg.norm(want.ravel()))
+            err_what = "rhs 0 vs FP64 kernel on the headline spinor"
+        else:
+            err = rel_err_device(gbx, g, y, w)
+            err_what = "all rhs vs FP64 kernel on the same (rounded) input"
+            for f in (x64, w):
+                f.close()
+        b = gbx.bytes_per_site(prec, prec, Ls, nrhs, comp12) * sites5 * nrhs
+        v = {"name": name, "prec": prec, "nrhs": nrhs, "Ls": Ls, "dag": dag, "comp12": comp12,
+             "us_per_launch": round(us, 2), "gflops": round(1320.0 * sites5 * nrhs / (us * 1e-6) / 1e9, 1),
+             "hbm_gbps_alg": round(b / (us * 1e-6) / 1e9, 1),
+             "per_rhs_speedup_vs_f64_1rhs": round(us_f64 * (Ls / LS) / (us / nrhs), 3),
+             "rel_l2_vs_f64": float(f"{err:.3e}"), "error_of": err_what, "cfg": cfg.as_dict()}
+        log(f"variant {v}")
+        out.append(v)
+        for f in (x, y):
+            f.close()
+        if U is not U64:
+            U.close()
+
+    variant("f64_1rhs_dag", "f64", 1, dag=True)
+    variant("f32_1rhs", "f32", 1)
+    variant("f32_16rhs", "f32", 16)
+    variant("f16_16rhs", "f16", 16)
+    variant("bf16_16rhs", "bf16", 16)
+    variant("f16_16rhs_comp12", "f16", 16, comp12=True)
+    variant("f32_16rhs_ls12", "f32", 16, Ls=12)
+    variant("bf16_16rhs_ls12", "bf16", 16, Ls=12)
+    return out
+
+
+def mixed_section(gbx, tune):
+    """32^4 x Ls16: Dhop error of each storage format against FP64."""
+    dims, Ls = (32, 32, 32, 32), 16
+    geom = gbx.Geometry(dims)
+    dbl = gbx.gen_doubled_gauge(dims, 301)
+    U64 = gbx.GaugeField(geom, "f64").upload(dbl)
+    x64 = gbx.SpinorField(geom, Ls, 1, "f64").fill_gaussian(302)
+    y64 = gbx.SpinorField(geom, Ls, 1, "f64")
+    tune(U64, y64, x64)
+    us64 = gbx.dhop_time(U64, y64, x64, nrep=20)
+    gbx.dhop(U64, y64, x64)
+    rows = [{"prec": "f64", "comp12": False, "us_per_launch": round(us64, 2), "rel_l2_vs_f64": 0.0}]
+    for prec, comp12 in (("f32", False), ("bf16", False), ("f16", False), ("f16", True), ("f32", True)):
+        U = gbx.GaugeField(geom, prec, comp12).upload(dbl)
+        x = gbx.SpinorField(geom, Ls, 1, prec).copy_from(x64)
+        y = gbx.SpinorField(geom, Ls, 1, prec)
+        tune(U, y, x)
+        us = gbx.dhop_time(U, y, x, nrep=20)
+        gbx.dhop(U, y, x)
+        e_store = rel_err_device(gbx, geom, x, x64)
+        e = rel_err_device(gbx, geom, y, y64)
+        r = {"prec": prec, "comp12": comp12, "us_per_launch": round(us, 2), "rel_l2_vs_f64": float(f"{e:.3e}"),
+             "input_rounding_rel_l2": float(f"{e_store:.3e}")}
+        log(f"mixed {r}")
+        rows.append(r)
+        for f in (U, x, y):
+            f.close()
+    return {"dims": list(dims), "Ls": Ls, "gauge_seed": 301, "spinor_seed_device": 302, "rows": rows}
+
+
+def cg_section(gbx, tune):
+    """32^4 x Ls16 defect-correction CG to 1e-10."""
+    dims, Ls = (32, 32, 32, 32), 16
+    geom = gbx.Geometry(dims)
+    dbl = gbx.gen_doubled_gauge(dims, 301)
+    U64 = gbx.GaugeField(geom, "f64").upload(dbl)
+    b = gbx.SpinorField(geom, Ls, 1, "f64").fill_gaussian(303)
+    x = gbx.SpinorField(geom, Ls, 1, "f64")
+    y = gbx.SpinorField(geom, Ls, 1, "f64")
+    tune(U64, y, b)
+    rows = []
+    for lo, comp12 in (("f32", False), ("bf16", False), ("f16", False), ("f16", True), (None, False)):
+        Ulo = None
+        if lo is not None:
+            Ulo = gbx.GaugeField(geom, lo, comp12).upload(dbl)
+            xs = gbx.SpinorField(geom, Ls, 1, lo).copy_from(b)
+            ys = gbx.SpinorField(geom, Ls, 1, lo)
+            tune(Ulo, ys, xs)
+            xs.close()
+            ys.close()
+        x.zero()
+        st = gbx.cg_solve(U64, Ulo, x, b, KAPPA, tol=1e-10, inner_tol=1e-3,
+                          max_outer=60, max_inner=5000)
+        # the residual once more, independently of the solver: r = b - M^dag M x
+        t = gbx.SpinorField(geom, Ls, 1, "f64")
+        gbx.dhop_axpby(U64, t, x, 1.0, -KAPPA)
+        gbx.dhop_axpby(U64, y, t, 1.0, -KAPPA, dag=True)
+        y.axpy(-1.0, b)
+        rel = float(np.sqrt(y.norm2() / b.norm2()))
+        t.close()
+        r = {"inner": lo or "f64 (no defect correction)", "comp12": comp12, **st,
+             "recomputed_rel_resid": float(f"{rel:.3e}"), "converged": rel <= 1e-10}
+        log(f"cg {r}")
+        rows.append(r)
+        if Ulo is not None:
+            Ulo.close()
+    return {"dims": list(dims), "Ls": Ls, "kappa": KAPPA, "tol": 1e-10, "gauge_seed": 301,
+            "rhs_seed_device": 303, "rows": rows}
+
+
+def batch_section(gbx, nrep):
+    """40 uneven lattices in one persistent graph-captured launch, FP32."""
+    rng = np.random.default_rng(505)
+    specs = [(int(rng.choice([8, 12, 16, 20, 24])), int(rng.choice([8, 12, 16])), int(rng.integers(1, 9)))
+             for _ in range(40)]
+    lats = []
+    for i, (L, Ls, nrhs) in enumerate(specs):
+        dims = (L, L, L, L)
+        g = gbx.Geometry(dims)
+        dbl = gbx.gen_doubled_gauge(dims, 5050 + i)
+        U = gbx.GaugeField(g, "f32").upload(dbl)
+        x = gbx.SpinorField(g, Ls, nrhs, "f32").fill_gaussian(6060 + i)
+        y = gbx.SpinorField(g, Ls, nrhs, "f32")
+        lats.append((g, dbl, U, x, y))
+    b = gbx.Batch([l[2] for l in lats], [l[4] for l in lats], [l[3] for l in lats])
+    items, wgs = b.info()
+    b.run()
+    us = b.time(nrep)
+    flop = sum(1320.0 * L ** 4 * Ls * nrhs for L, Ls, nrhs in specs)
+    byts = sum(gbx.bytes_per_site("f32", "f32", Ls, nrhs) * L ** 4 * Ls * nrhs for L, Ls, nrhs in specs)
+    errs = []
+    for (g, dbl, U, x, y), (L, Ls, nrhs) in zip(lats, specs):
+        U64 = gbx.GaugeField(g, "f64").upload(dbl)
+        x64 = gbx.SpinorField(g, Ls, nrhs, "f64").copy_from(x)
+        y64 = gbx.SpinorField(g, Ls, nrhs, "f64")
+        gbx.dhop(U64, y64, x64)
+        errs.append(float(f"{rel_err_device(gbx, g, y, y64):.3e}"))
+        for f in (U64, x64, y64):
+            f.close()
+    b.close()
+    r = {"lattices": [{"L": L, "Ls": Ls, "nrhs": n} for L, Ls, n in specs], "work_items": items,
+         "persistent_workgroups": wgs, "us_per_launch": round(us, 2), "gflops": round(flop / (us * 1e-6) / 1e9, 1),
+         "hbm_gbps_alg": round(byts / (us * 1e-6) / 1e9, 1), "rel_l2_vs_f64_per_lattice": errs,
+         "max_rel_l2_vs_f64": max(errs)}
+    log(f"batch: {us:.1f} us, {r['gflops']} GFLOP/s, max err {max(errs):.2e}")
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -80,20 +279,27 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="only the FP64 1-RHS headline")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the 32^4 mixed-precision errors")
+    ap.add_argument("--no-cg", action="store_true", help="skip the defect-correction CG")
+    ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
+    ap.add_argument("--headline-only", action="store_true", help="= --no-variants --no-mixed --no-cg --no-batch")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--no-tune", action="store_true",
-                    help="use the default launch configuration (profiling runs: no tuner launches)")
+    ap.add_argument("--tune-cache", default=TUNE_CACHE, help="autotuner cache (JSON lines); '' = tune afresh")
+    ap.add_argument("--retune", action="store_true", help="delete the cache file first")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hopping term on one "
               "GPU's lattice is the whole measured path; the multi-GPU halo-exchange design is described in "
               "DESIGN.md but not built).", file=sys.stderr)
         sys.exit(2)
+    if args.headline_only:
+        args.no_variants = args.no_mixed = args.no_cg = args.no_batch = True
 
     import gridbench_amd as gbx
     if gbx.device_count() < 1:
         print("no GPU visible", file=sys.stderr)
         sys.exit(3)
+    tune = Tuner(gbx, args.tune_cache or None, args.retune)
 
     V = int(np.prod(DIMS))
     sites5 = V * LS
@@ -106,12 +312,8 @@ def main():
     x64 = gbx.SpinorField(geom, LS, 1, "f64").upload_rhs(0, psi)
     y64 = gbx.SpinorField(geom, LS, 1, "f64")
 
-    if args.no_tune:
-        best = gbx.dhop_get_cfg(U64, x64)
-    else:
-        log("tuning FP64 1-RHS")
-        best, best_us = gbx.tune(U64, y64, x64)
-        log(f"tuned: {best.as_dict()} {best_us:.1f} us")
+    best, best_us = tune(U64, y64, x64)
+    log(f"FP64 1-RHS config: {best.as_dict()} {best_us:.1f} us")
 
     # ------------------------------------------------ timed headline region
     for _ in range(args.warmup):
@@ -134,43 +336,17 @@ def main():
     stream_gbps = gbx.stream_copy_bw(4 << 30, 20)
     log(f"STREAM copy: {stream_gbps:.0f} GB/s")
 
-    variants = []
+    extra = {}
     if not args.no_variants:
-        def variant(name, prec, nrhs, dag=False, comp12=False):
-            U = U64 if prec == "f64" else gbx.GaugeField(geom, prec, comp12).upload(dbl)
-            x = gbx.SpinorField(geom, LS, nrhs, prec)
-            for r in range(nrhs):  # right-hand side r = stream r of the spinor seed (gbx_gen_spinor)
-                x.upload_rhs(r, psi if r == 0 else gbx.gen_gaussian(SPINOR_SEED, r, 0, V * LS * 24))
-            y = gbx.SpinorField(geom, LS, nrhs, prec)
-            cfg, tus = gbx.tune(U, y, x, dag=dag)
-            us = gbx.dhop_time(U, y, x, dag=dag, nrep=20)
-            gbx.dhop(U, y, x, dag=dag)
-            got = y.download_rhs(0)
-            if dag:
-                want = ref64_dag
-            else:
-                want = ref64
-            err = float(np.linalg.norm((got - want).ravel()) / np.linalg.norm(want.ravel()))
-            per_rhs_us = us / nrhs
-            b = gbx.bytes_per_site(prec, prec, LS, nrhs, comp12) * sites5 * nrhs
-            v = {"name": name, "prec": prec, "nrhs": nrhs, "dag": dag, "comp12": comp12,
-                 "us_per_launch": round(us, 2), "gflops": round(1320.0 * sites5 * nrhs / (us * 1e-6) / 1e9, 1),
-                 "hbm_gbps_alg": round(b / (us * 1e-6) / 1e9, 1),
-                 "per_rhs_speedup_vs_f64_1rhs": round(us_ev / per_rhs_us, 3),
-                 "rel_l2_vs_f64": float(f"{err:.3e}"), "cfg": cfg.as_dict()}
-            log(f"variant {v}")
-            variants.append(v)
-            for f in (x, y):
-                f.close()
-
         gbx.dhop(U64, y64, x64, dag=True)
         ref64_dag = y64.download_rhs(0)
-        variant("f64_1rhs_dag", "f64", 1, dag=True)
-        variant("f32_1rhs", "f32", 1)
-        variant("f32_16rhs", "f32", 16)
-        variant("f16_16rhs", "f16", 16)
-        variant("bf16_16rhs", "bf16", 16)
-        variant("f16_16rhs_comp12", "f16", 16, comp12=True)
+        extra["variants"] = variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_ev)
+    if not args.no_mixed:
+        extra["mixed_precision"] = mixed_section(gbx, tune)
+    if not args.no_cg:
+        extra["cg"] = cg_section(gbx, tune)
+    if not args.no_batch:
+        extra["batch"] = batch_section(gbx, 20)
 
     cpu = None
     if not args.no_cpu:
@@ -201,7 +377,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profiles(),
                      "algorithmic_bytes_per_launch": bytes_alg, "us_per_launch_events": round(us_ev, 2)},
         "cpu_baseline": cpu,
-        "variants": variants,
+        **extra,
     }
     print(json.dumps(result), flush=True)
 

@@ -102,6 +102,7 @@ PROTOTYPES = {
     "gbx_spinor_upload_ref_f32": (i32, [vp, pflt, i32]),
     "gbx_spinor_download_ref_f32": (i32, [vp, pflt, i32]),
     "gbx_spinor_copy": (i32, [vp, vp]),
+    "gbx_spinor_fill_gaussian": (i32, [vp, u64]),
     "gbx_dhop": (i32, [vp, vp, vp, i32]),
     "gbx_dhop_axpby": (i32, [vp, vp, vp, i32, dbl, dbl]),
     "gbx_dhop_cfg": (i32, [vp, vp, vp, i32, C.POINTER(KCfg)]),

@@ -127,6 +127,51 @@ __global__ void __launch_bounds__(256) k_convert_spinor(const void *__restrict__
   }
 }
 
+__device__ __forceinline__ uint64_t dmix64(uint64_t z)
+{
+  // splitmix64 finaliser (Steele, Lea, Flood 2014)
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+// Gaussian fill: thread = (slot, m); the 12 pairs of its spinor are keyed by
+// the lexicographic 5D index, so the field does not depend on the site order.
+template <int P>
+__global__ void __launch_bounds__(256) k_fill_gaussian(void *__restrict__ dst, const uint32_t *__restrict__ site_of_slot,
+                                                       uint64_t npts, uint32_t M, uint32_t Ls, uint64_t V, uint64_t key)
+{
+  constexpr int R = dev::Traits<P>::R, NCH = dev::Traits<P>::NCH;
+  for (uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; t < npts; t += (uint64_t)gridDim.x * blockDim.x) {
+    const uint64_t q = t / M;
+    const uint32_t m = (uint32_t)(t % M);
+    const uint64_t base = (((uint64_t)(m / Ls) * V + site_of_slot[q]) * Ls + m % Ls) * 12;
+    double v[24];
+#pragma unroll
+    for (int j = 0; j < 12; j++) {
+      const uint64_t h = dmix64(key ^ dmix64(base + (uint64_t)j));
+      const double u1 = ((double)(h >> 11) + 0.5) * (1.0 / 9007199254740992.0);
+      const double u2 = (double)(dmix64(h) >> 11) * (1.0 / 9007199254740992.0);
+      const double r = sqrt(-2.0 * log(u1));
+      double s, c;
+      sincos(6.283185307179586 * u2, &s, &c);
+      v[2 * j] = r * c;
+      v[2 * j + 1] = r * s;
+    }
+#pragma unroll
+    for (int k = 0; k < NCH; k++) dev::store_chunk<P, double>(dst, (q * NCH + k) * M + m, v + k * R, false);
+  }
+}
+
+uint64_t dmix64_host(uint64_t z)
+{
+  z += 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
 uint32_t grid_for(uint64_t n, uint32_t threads = 256)
 {
   uint64_t b = (n + threads - 1) / threads;
@@ -261,6 +306,24 @@ extern "C" int gbx_spinor_scale(gbx_spinor x, double a)
 {
   GBX_CHECK(x && x->d, GBX_ENULL, "gbx_spinor_scale: NULL spinor");
   return spinor_scale_on(x, a, x->geom->stream);
+}
+
+extern "C" int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed)
+{
+  GBX_CHECK(x && x->d, GBX_ENULL, "gbx_spinor_fill_gaussian: NULL spinor");
+  const uint64_t npts = x->geom->vol * (uint64_t)x->M;
+  const uint64_t key = dmix64_host(seed);
+  const dim3 grid(grid_for(npts)), block(256);
+  hipStream_t st = x->geom->stream;
+  const uint32_t *sos = x->geom->site_of_slot;
+  switch (x->prec) {
+  case GBX_F64: hipLaunchKernelGGL(k_fill_gaussian<GBX_F64>, grid, block, 0, st, x->d, sos, npts, x->M, (uint32_t)x->Ls, x->geom->vol, key); break;
+  case GBX_F32: hipLaunchKernelGGL(k_fill_gaussian<GBX_F32>, grid, block, 0, st, x->d, sos, npts, x->M, (uint32_t)x->Ls, x->geom->vol, key); break;
+  case GBX_F16: hipLaunchKernelGGL(k_fill_gaussian<GBX_F16>, grid, block, 0, st, x->d, sos, npts, x->M, (uint32_t)x->Ls, x->geom->vol, key); break;
+  default: hipLaunchKernelGGL(k_fill_gaussian<GBX_BF16>, grid, block, 0, st, x->d, sos, npts, x->M, (uint32_t)x->Ls, x->geom->vol, key); break;
+  }
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
 }
 
 extern "C" int gbx_spinor_zero(gbx_spinor x)

@@ -297,8 +297,10 @@ void geom_release(gbx_geom_s *g)
 extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out)
 {
   GBX_CHECK(dims && out, GBX_ENULL, "gbx_geom_create: NULL argument");
+  // Default: compact 4D blocks of extent <= 6 per dimension (about 1300
+  // sites: the neighbour reuse distance of one block fits an XCD's L2).
   int blk[4];
-  for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : (d == 0 ? dims[0] : largest_divisor_le(dims[d] > 0 ? dims[d] : 1, 4));
+  for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : largest_divisor_le(dims[d] > 0 ? dims[d] : 1, 6);
   return gbx_geom_create_levels(dims, 1, blk, out);
 }
 

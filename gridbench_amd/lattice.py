@@ -235,6 +235,11 @@ class SpinorField:
         check(lib.gbx_spinor_copy(self.h, src.h), "gbx_spinor_copy")
         return self
 
+    def fill_gaussian(self, seed):
+        """N(0,1) entries generated on the GPU (throughput runs; not the host generator's bits)."""
+        check(lib.gbx_spinor_fill_gaussian(self.h, int(seed)), "gbx_spinor_fill_gaussian")
+        return self
+
     def norm2(self):
         n = C.c_double()
         check(lib.gbx_spinor_norm2(self.h, C.byref(n)), "gbx_spinor_norm2")

@@ -157,6 +157,11 @@ int gbx_spinor_upload_ref_f32(gbx_spinor x, const float *ref, int nsimd);
 int gbx_spinor_download_ref_f32(gbx_spinor x, float *ref, int nsimd);
 /* dst = src converted between precisions (same geometry, Ls, nrhs). */
 int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src);
+/* Fill every real with an N(0,1) value generated on the GPU (counter-based */
+/* hash of (seed, rhs, site, s, component), Box-Muller with the device's    */
+/* log/sin/cos).  For throughput runs on fields too large to generate on    */
+/* the host; the values are not the host generator's bits.                  */
+int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 
 /* ---------------------------------------------------------------------- */
 /* The hopping term on device-resident fields: out = Dhop in (dag = 0) or  */

@@ -137,6 +137,28 @@ def test_upload_download_roundtrip(gpu, gbx, problem):
         assert np.array_equal(y.download(), z.download())
 
 
+def test_fill_gaussian(gpu, gbx):
+    """Device Gaussian fill: the field is a function of the lexicographic 5D
+    index (same bits under two site orders), each precision is the FP64 field
+    rounded, seeds differ, and the sample moments are those of N(0,1)."""
+    dims, Ls, nrhs = (8, 4, 6, 4), 4, 3
+    ga, gb = gbx.Geometry(dims), gbx.Geometry(dims, block=(1, 1, 1, 1))
+    a = gbx.SpinorField(ga, Ls, nrhs, "f64").fill_gaussian(77).download()
+    b = gbx.SpinorField(gb, Ls, nrhs, "f64").fill_gaussian(77).download()
+    assert np.array_equal(a, b)
+    c = gbx.SpinorField(ga, Ls, nrhs, "f64").fill_gaussian(78).download()
+    assert not np.any(a == c)
+    lo = gbx.SpinorField(ga, Ls, nrhs, "f32").fill_gaussian(77).download()
+    assert np.array_equal(lo, a.astype(np.float32).astype(np.float64))
+    h = gbx.SpinorField(ga, Ls, nrhs, "f16").fill_gaussian(77).download()
+    assert np.all(np.abs(h - a) <= 2.0 ** -10 * np.abs(a) + 2.0 ** -24)
+    v = a.reshape(-1)
+    n = v.size
+    assert abs(v.mean()) < 5 / np.sqrt(n)
+    assert abs(v.var() - 1.0) < 5 * np.sqrt(2.0 / n)
+    assert abs(np.mean(v ** 4) - 3.0) < 0.1
+
+
 # ----------------------------------------------------------- drop-in entry
 def ref_case(gbx, orc, dims, Ls, nsimd, dtype):
     links = gbx.gen_su3_links(dims, 101)
