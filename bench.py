@@ -109,11 +109,13 @@ class Tuner:
         return self.gbx.tune(U, y, x, dag=dag, cache_path=self.cache)
 
 
-def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64):
+def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, only=None):
     V = int(np.prod(DIMS))
     out = []
 
     def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False):
+        if only and name not in only:
+            return
         g = geom
         sites5 = V * Ls
         U = U64 if prec == "f64" else gbx.GaugeField(g, prec, comp12).upload(dbl)
@@ -283,6 +285,7 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the defect-correction CG")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
     ap.add_argument("--headline-only", action="store_true", help="= --no-variants --no-mixed --no-cg --no-batch")
+    ap.add_argument("--only", default="", help="comma-separated variant names to run (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tune-cache", default=TUNE_CACHE, help="autotuner cache (JSON lines); '' = tune afresh")
     ap.add_argument("--retune", action="store_true", help="delete the cache file first")
@@ -340,7 +343,8 @@ def main():
     if not args.no_variants:
         gbx.dhop(U64, y64, x64, dag=True)
         ref64_dag = y64.download_rhs(0)
-        extra["variants"] = variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_ev)
+        extra["variants"] = variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_ev,
+                                               only=set(filter(None, args.only.split(","))))
     if not args.no_mixed:
         extra["mixed_precision"] = mixed_section(gbx, tune)
     if not args.no_cg:

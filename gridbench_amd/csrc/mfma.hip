@@ -1,21 +1,267 @@
-// MFMA form of the multi-right-hand-side hopping term (placeholder until
-// the matrix-core kernel lands: reports GBX_EUNSUPPORTED).
+// Matrix-core form of the multi-right-hand-side hopping term (FP16 / BF16
+// storage, FP32 accumulation).
+//
+// For one 4D site the whole hopping term is a dense real product
+//
+//   out[24 x M] = sum over legs d of A_d[24 x 24] * in_{nbr(d)}[24 x M],
+//   A_d = (1 -/+ gamma_mu) (x) U_d   (spin (x) colour, complex as 2x2 real),
+//
+// with M = Ls * nrhs minor columns (s-slice, right-hand side).  Stacking the
+// legs along K gives one [24 x 192] * [192 x M] product per site, done with
+// v_mfma_f32_16x16x32_{f16,bf16}: two 16-row tiles (rows 24..31 zero) x six
+// K-steps per 16 columns.  The B operand needs no repacking: a lane of an
+// MFMA holds 8 consecutive K values of one column, and one 16-byte chunk of
+// the SoA layout holds 8 consecutive reals of one (site, minor) spinor, so
+// every B fragment is a single coalesced 16-byte gather (16 lanes read 256
+// contiguous bytes).  Lane group g (lanes 16g .. 16g+15) carries legs 2g and
+// 2g+1: K-step s reads leg 2g + s/3, chunk s%3.
+//
+// The A operand of a site is built once per wavefront in registers from the
+// links staged in LDS: each entry is +-Re or +-Im of one link element (the
+// spin factor is 0, +-1 or +-i), so the 16-bit A holds the stored link
+// exactly (12-real links: the rebuilt third row is rounded once).  The
+// result tile gives a lane 4 consecutive output reals of one column: two
+// 8-byte stores per lane.
+//
+// The reference applies the same operator leg by leg
+// (WilsonKernelsHandCpu.h:426-446 with the projections and reconstructions
+// of lines 94-303); the gamma matrices below are those spin projections
+// written as matrices.
+#include "dhop_dev.h"
 #include "gbx_internal.h"
+
+namespace {
+
+using dev::f4;
+
+template <int P> struct MTraits;
+template <> struct MTraits<GBX_F16> {
+  typedef _Float16 S;
+  typedef dev::h8 V8;
+  typedef _Float16 V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ f4 mma(V8 a, V8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0); }
+};
+template <> struct MTraits<GBX_BF16> {
+  typedef __bf16 S;
+  typedef dev::b8 V8;
+  typedef __bf16 V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ f4 mma(V8 a, V8 b, f4 c) { return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0); }
+};
+
+// gamma_mu (mu = x, y, z, t), chiral basis: row a has its single nonzero in
+// column gcol(mu, a), value gre + i gim with (gre, gim) from gval.
+__device__ __forceinline__ int gcol(int mu, int a) { return mu < 2 ? 3 - a : a ^ 2; }
+__device__ __forceinline__ void gval(int mu, int a, int &re, int &im)
+{
+  const bool outer = (a == 0 || a == 3);
+  switch (mu) {
+  case 0: re = 0; im = a < 2 ? 1 : -1; break;   // x: h0 = p0 + i p3 for 1 + gamma_x
+  case 1: re = outer ? -1 : 1; im = 0; break;   // y: h0 = p0 - p3
+  case 2: re = 0; im = outer ? 1 : -1; break;   // z: h0 = p0 + i p2
+  default: re = 1; im = 0; break;               // t: h0 = p0 + p2
+  }
+}
+
+// Work unit of a wavefront: one site of the workgroup x `mw` minor columns
+// (mw a multiple of 16).  Workgroup = `spw` sites; its waves take units
+// round-robin, site-major, and rebuild A only when the site changes.
+template <int P, bool DAG, bool C12, bool PF>
+__global__ void __launch_bounds__(256)
+    k_dhop_mfma(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
+                void *__restrict__ out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t nwg,
+                uint32_t per_xcd, int xcd, int nt)
+{
+  typedef MTraits<P> T;
+  typedef typename T::V8 V8;
+  typedef typename T::V4 V4;
+  typedef typename T::S S;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float *sU = (float *)smem;
+  uint32_t *sN = (uint32_t *)(sU + (size_t)spw * 8 * 18);
+
+  const uint32_t L = xcd ? (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (L >= nwg) return;  // uniform over the workgroup, before the barrier
+  const uint32_t q0 = L * spw;
+  const uint32_t nsl = min(spw, nslot - q0);
+  dev::stage_links<P, C12>(gauge, nbr, q0, nsl, sU, sN);
+
+  const int lane = threadIdx.x & 63, g = lane >> 4, i = lane & 15;
+  const uint32_t wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+  // Output row of this lane in tile r: o = 16 r + i = 6 a + 2 c + sigma.
+  int ra[2], rc[2], rs[2], rp[2][2], wre[2][2], wim[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; r++) {
+    const int o = 16 * r + i;
+    ra[r] = o < 24 ? o / 6 : -1;
+    rc[r] = o < 24 ? (o % 6) / 2 : 0;
+    rs[r] = o & 1;
+#pragma unroll
+    for (int h = 0; h < 2; h++) {
+      const int d = 2 * g + h;
+      const int K = DAG ? d : (d ^ 4);  // projector kind: 0-3 (1 + gamma), 4-7 (1 - gamma)
+      const int mu = K & 3, sg = K < 4 ? 1 : -1;
+      int gr = 0, gi = 0;
+      if (ra[r] >= 0) gval(mu, ra[r], gr, gi);
+      rp[r][h] = ra[r] >= 0 ? gcol(mu, ra[r]) : -1;
+      wre[r][h] = sg * gr;
+      wim[r][h] = sg * gi;
+    }
+  }
+
+  const uint32_t cpw = M / mw;  // wave units per site
+  const uint32_t units = nsl * cpw;
+  const V8 *vin = (const V8 *)in;
+  V4 *vout = (V4 *)out;
+  uint32_t cur = 0xffffffffu;
+  V8 Af[2][6];
+  uint32_t nb0 = 0, nb1 = 0;
+  for (uint32_t w = wave; w < units; w += nwave) {
+    const uint32_t sl = w / cpw, m0 = (w - sl * cpw) * mw;
+    if (sl != cur) {
+      cur = sl;
+      nb0 = sN[sl * 8 + 2 * g];
+      nb1 = sN[sl * 8 + 2 * g + 1];
+#pragma unroll
+      for (int r = 0; r < 2; r++)
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+          const float *u = sU + (sl * 8 + 2 * g + h) * 18 + rc[r] * 6;
+          float dg[6], of[6];
+#pragma unroll
+          for (int e = 0; e < 3; e++) {
+            const float ur = u[2 * e], ui = u[2 * e + 1];
+            // real form of z acting on (re, im) of the input: row sigma
+            // 0 -> (Re z, -Im z), row 1 -> (Im z, Re z)
+            dg[2 * e] = rs[r] ? ui : ur;
+            dg[2 * e + 1] = rs[r] ? ur : -ui;
+            const float zr = (float)wre[r][h] * ur - (float)wim[r][h] * ui;
+            const float zi = (float)wre[r][h] * ui + (float)wim[r][h] * ur;
+            of[2 * e] = rs[r] ? zi : zr;
+            of[2 * e + 1] = rs[r] ? zr : -zi;
+          }
+#pragma unroll
+          for (int c = 0; c < 3; c++) {
+            V8 a;
+#pragma unroll
+            for (int j = 0; j < 8; j++) {
+              const int ep = 8 * c + j, b = ep / 6, k = ep % 6;
+              const float v = b == ra[r] ? dg[k] : (b == rp[r][h] ? of[k] : 0.f);
+              a[j] = (S)v;
+            }
+            Af[r][3 * h + c] = a;
+          }
+        }
+    }
+    const uint64_t ob0 = ((uint64_t)(q0 + sl) * 3 + (uint32_t)(g >> 1)) * M;
+    const uint64_t ob1 = ((uint64_t)(q0 + sl) * 3 + 2) * M;
+    const uint64_t ib0 = (uint64_t)nb0 * 3 * M, ib1 = (uint64_t)nb1 * 3 * M;
+    V8 B[6];
+    auto gather = [&](V8 *dst, uint32_t m) {
+#pragma unroll
+      for (int s = 0; s < 6; s++) dst[s] = vin[(s < 3 ? ib0 : ib1) + (uint64_t)(s % 3) * M + m];
+    };
+    gather(B, m0 + i);
+    for (uint32_t t = 0; t < mw; t += 16) {
+      const uint32_t m = m0 + t + i;
+      V8 Bn[6];
+      if (PF && t + 16 < mw) gather(Bn, m + 16);
+      f4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int s = 0; s < 6; s++) {
+        acc0 = T::mma(Af[0][s], B[s], acc0);
+        acc1 = T::mma(Af[1][s], B[s], acc1);
+      }
+      V4 o0, o1;
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        o0[j] = (S)acc0[j];
+        o1[j] = (S)acc1[j];
+      }
+      V4 *p0 = vout + (ob0 + m) * 2 + (g & 1);
+      V4 *p1 = vout + (ob1 + m) * 2 + (g & 1);
+      if (nt) __builtin_nontemporal_store(o0, p0);
+      else *p0 = o0;
+      if (g < 2) {
+        if (nt) __builtin_nontemporal_store(o1, p1);
+        else *p1 = o1;
+      }
+      if (t + 16 < mw) {
+        if (PF) {
+#pragma unroll
+          for (int s = 0; s < 6; s++) B[s] = Bn[s];
+        } else {
+          gather(B, m + 16);
+        }
+      }
+    }
+  }
+}
+
+template <int P, bool DAG, bool C12>
+void launch_pf(bool pf, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge, const uint32_t *nbr,
+               const void *in, void *out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t nwg,
+               uint32_t per_xcd, int xcd, int nt)
+{
+  if (pf)
+    hipLaunchKernelGGL((k_dhop_mfma<P, DAG, C12, true>), grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg,
+                       per_xcd, xcd, nt);
+  else
+    hipLaunchKernelGGL((k_dhop_mfma<P, DAG, C12, false>), grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw,
+                       nwg, per_xcd, xcd, nt);
+}
+
+template <int P>
+void launch_prec(bool dag, bool c12, bool pf, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge,
+                 const uint32_t *nbr, const void *in, void *out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw,
+                 uint32_t nwg, uint32_t per_xcd, int xcd, int nt)
+{
+  if (!dag && !c12) launch_pf<P, false, false>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
+  else if (!dag) launch_pf<P, false, true>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
+  else if (!c12) launch_pf<P, true, false>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
+  else launch_pf<P, true, true>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
+}
+
+}  // namespace
 
 int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in)
 {
-  (void)u;
-  (void)in;
-  return 0;
+  return u && in && (in->prec == GBX_F16 || in->prec == GBX_BF16) && in->M % 16 == 0;
 }
 
+// kind = 1 configuration: spw sites per workgroup, threads = 64 .. 256
+// (one wavefront per 64), mc = minor columns per wavefront unit (a multiple
+// of 16 dividing M; 0 = all of M), pf = gather the next 16 columns before
+// the current MFMAs, xcd / nt / wgpc as for the VALU kernel (cblk unused).
 int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st)
 {
-  (void)u;
-  (void)out;
-  (void)in;
-  (void)dag;
-  (void)c;
-  (void)st;
-  return gbx_fail(GBX_EUNSUPPORTED, "MFMA kernel not available for this configuration");
+  GBX_CHECK(u->prec == in->prec && out->prec == in->prec, GBX_EMISMATCH, "dhop (MFMA): precisions differ");
+  GBX_CHECK(mfma_supported(u, in), GBX_EUNSUPPORTED,
+            "dhop (MFMA): needs FP16 or BF16 storage and Ls*nrhs a multiple of 16 (precision %d, Ls*nrhs %u)", in->prec,
+            in->M);
+  GBX_CHECK(c.threads >= 64 && c.threads <= 256 && c.threads % 64 == 0, GBX_EINVAL, "dhop (MFMA): threads=%d",
+            c.threads);
+  GBX_CHECK(c.spw >= 1 && c.spw <= 1024, GBX_EINVAL, "dhop (MFMA): spw=%d", c.spw);
+  const uint32_t mw = c.mc <= 0 ? in->M : (uint32_t)c.mc;
+  GBX_CHECK(mw % 16 == 0 && in->M % mw == 0, GBX_EINVAL, "dhop (MFMA): mc=%d must be a multiple of 16 dividing %u", c.mc,
+            in->M);
+  GBX_CHECK(c.wgpc >= 0 && c.wgpc <= 64, GBX_EINVAL, "dhop (MFMA): wgpc=%d", c.wgpc);
+  size_t lds = (size_t)c.spw * 8 * 18 * 4 + (size_t)c.spw * 8 * 4;
+  GBX_CHECK(lds <= 160 * 1024, GBX_EINVAL, "dhop (MFMA): spw=%d needs %zu B of LDS", c.spw, lds);
+  if (c.wgpc > 0) {
+    const size_t pad = ((size_t)160 * 1024 / (size_t)c.wgpc) & ~(size_t)511;
+    if (pad > lds) lds = pad;
+  }
+  const uint32_t nslot = (uint32_t)u->geom->vol;
+  const uint32_t nwg = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
+  const uint32_t per_xcd = (nwg + 7) / 8;
+  const dim3 grid(c.xcd ? per_xcd * 8 : nwg), block((unsigned)c.threads);
+  const bool c12 = u->comp12 != 0, pf = c.pf != 0;
+  if (in->prec == GBX_F16)
+    launch_prec<GBX_F16>(dag != 0, c12, pf, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, nslot, in->M,
+                         (uint32_t)c.spw, mw, nwg, per_xcd, c.xcd, c.nt);
+  else
+    launch_prec<GBX_BF16>(dag != 0, c12, pf, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, nslot, in->M,
+                          (uint32_t)c.spw, mw, nwg, per_xcd, c.xcd, c.nt);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
 }

@@ -77,7 +77,16 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
   }
   gbx_clear_error();
   if (mfma_supported(u, in)) {
-    for (int spw : {1, 2, 4}) v.push_back(gbx_kcfg{spw, 256, 1, 1, 1, 0, 0, 0, 0});
+    // MFMA form: minor columns per wavefront unit (multiples of 16 dividing
+    // M), sites per workgroup, wavefronts per workgroup, next-tile gather.
+    for (uint32_t mw = M; mw >= 16; mw /= 2) {
+      if (M % mw || mw % 16) continue;
+      for (int spw : {1, 2, 4, 8})
+        for (int t : {128, 256}) {
+          if ((uint64_t)spw * (M / mw) * 64 < (uint64_t)t) continue;  // fewer units than wavefronts
+          for (int pf = 0; pf <= 1; pf++) v.push_back(gbx_kcfg{spw, t, 1, 1, 1, mw == M ? 0 : (int)mw, pf, 0, 0});
+        }
+    }
   }
   return v;
 }
@@ -97,7 +106,7 @@ std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
         k.wgpc = wgpc;
         v.push_back(k);
       }
-  if (b.mc > 0)  // chunk-inner visiting order: site groups per chunk cycle
+  if (b.mc > 0 && b.kind == 0)  // chunk-inner visiting order: site groups per chunk cycle
     for (int cblk : {16, 64, 256, 1024}) {
       gbx_kcfg k = b;
       k.cblk = cblk;
@@ -177,7 +186,7 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
   };
   search(cands);
   GBX_CHECK(bus < 1e300, GBX_EHIP, "gbx_tune: every candidate failed");
-  if (bk.kind == 0) search(refine_candidates(bk));
+  search(refine_candidates(bk));
   *best = bk;
   *best_usec = bus;
   dhop_remember_cfg(sig, bk);
