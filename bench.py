@@ -11,9 +11,11 @@ stream synchronisations.
 Also reported, each with its own seeds:
   variants   24^4 x Ls16: DhopDag FP64; FP32 1-RHS; FP32/FP16/BF16 storage
              with 16 right-hand sides (+ FP16 with 12-real links); 24^4 x
-             Ls12 x 16 RHS in FP32 and BF16.  Throughput per right-hand side
-             relative to FP64 1-RHS, error of right-hand side 0 against the
-             FP64 kernel on the same input.
+             Ls12 x 16 RHS in FP32 and BF16 (gauge seed 401, spinors 402).
+             Throughput per right-hand side relative to FP64 1-RHS (per 5D
+             site: the Ls12 forms are scaled by 12/16), error against the
+             FP64 kernel on the same input; for 16-bit storage the MFMA form
+             and the best VALU configuration are both reported.
   mixed      32^4 x Ls16: Dhop error of FP32/BF16/FP16/FP16+12-real storage
              against FP64 on the same input.
   cg         32^4 x Ls16: defect-correction CG on M^dag M x = b,
@@ -43,6 +45,7 @@ sys.path.insert(0, ROOT)
 DIMS = (24, 24, 24, 24)
 LS = 16
 GAUGE_SEED, SPINOR_SEED = 201, 202  # seeds of the 24^4 x 16 configuration
+C4_GAUGE_SEED, C4_SPINOR_SEED = 401, 402  # 24^4 x Ls12 x 16 right-hand sides
 HBM_PEAK_GBPS = 8000.0  # MI355X HBM3E, AMD-published
 KAPPA = 0.12
 TUNE_CACHE = os.path.join(ROOT, "gridbench_amd", "tuned_mi355x.jsonl")
@@ -105,30 +108,38 @@ class Tuner:
         if retune and cache and os.path.exists(cache):
             os.remove(cache)
 
-    def __call__(self, U, y, x, dag=False):
-        return self.gbx.tune(U, y, x, dag=dag, cache_path=self.cache)
+    def __call__(self, U, y, x, dag=False, kinds=None):
+        return self.gbx.tune(U, y, x, dag=dag, cache_path=self.cache, kinds=kinds)
 
 
 def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, only=None):
     V = int(np.prod(DIMS))
     out = []
+    c4 = {}  # the 24^4 x Ls12 x 16-RHS configuration: its own gauge field (seed 401), spinors seed 402
 
-    def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False):
+    def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False, compare=False):
         if only and name not in only:
             return
         g = geom
         sites5 = V * Ls
-        U = U64 if prec == "f64" else gbx.GaugeField(g, prec, comp12).upload(dbl)
+        if Ls == LS:
+            gauge, Uref = dbl, U64
+        else:
+            if not c4:
+                c4["dbl"] = gbx.gen_doubled_gauge(DIMS, C4_GAUGE_SEED)
+                c4["U64"] = gbx.GaugeField(g, "f64").upload(c4["dbl"])
+            gauge, Uref = c4["dbl"], c4["U64"]
+        U = Uref if prec == "f64" else gbx.GaugeField(g, prec, comp12).upload(gauge)
         x = gbx.SpinorField(g, Ls, nrhs, prec)
         if Ls == LS:
             x.fill_gaussian(SPINOR_SEED + 1000)  # right-hand sides 1.. ; 0 is the headline spinor
             x.upload_rhs(0, psi)
             want = ref64_dag if dag else ref64
         else:
-            x.fill_gaussian(SPINOR_SEED + 2000)
+            x.fill_gaussian(C4_SPINOR_SEED)
             x64 = gbx.SpinorField(g, Ls, nrhs, "f64").copy_from(x)
             w = gbx.SpinorField(g, Ls, nrhs, "f64")
-            gbx.dhop(U64, w, x64, dag=dag)
+            gbx.dhop(Uref, w, x64, dag=dag)
         y = gbx.SpinorField(g, Ls, nrhs, prec)
         cfg, _ = tune(U, y, x, dag=dag)
         us = gbx.dhop_time(U, y, x, dag=dag, nrep=20)
@@ -142,27 +153,39 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             err_what = "all rhs vs FP64 kernel on the same (rounded) input"
             for f in (x64, w):
                 f.close()
+        forms = None
+        if compare:  # the matrix-core form against the best VALU configuration, both autotuned
+            forms = {}
+            for kind in ("valu", "mfma"):
+                try:
+                    kc, _ = tune(U, y, x, dag=dag, kinds=kind)
+                    forms[kind] = {"us_per_launch": round(gbx.dhop_time(U, y, x, dag=dag, cfg=kc, nrep=20), 2),
+                                   "cfg": kc.as_dict()}
+                except gbx.GbxError as e:
+                    forms[kind] = {"unsupported": str(e)}
         b = gbx.bytes_per_site(prec, prec, Ls, nrhs, comp12) * sites5 * nrhs
         v = {"name": name, "prec": prec, "nrhs": nrhs, "Ls": Ls, "dag": dag, "comp12": comp12,
              "us_per_launch": round(us, 2), "gflops": round(1320.0 * sites5 * nrhs / (us * 1e-6) / 1e9, 1),
              "hbm_gbps_alg": round(b / (us * 1e-6) / 1e9, 1),
              "per_rhs_speedup_vs_f64_1rhs": round(us_f64 * (Ls / LS) / (us / nrhs), 3),
              "rel_l2_vs_f64": float(f"{err:.3e}"), "error_of": err_what, "cfg": cfg.as_dict()}
+        if forms:
+            v["forms"] = forms
         log(f"variant {v}")
         out.append(v)
         for f in (x, y):
             f.close()
-        if U is not U64:
+        if U is not Uref:
             U.close()
 
     variant("f64_1rhs_dag", "f64", 1, dag=True)
     variant("f32_1rhs", "f32", 1)
     variant("f32_16rhs", "f32", 16)
-    variant("f16_16rhs", "f16", 16)
-    variant("bf16_16rhs", "bf16", 16)
+    variant("f16_16rhs", "f16", 16, compare=True)
+    variant("bf16_16rhs", "bf16", 16, compare=True)
     variant("f16_16rhs_comp12", "f16", 16, comp12=True)
-    variant("f32_16rhs_ls12", "f32", 16, Ls=12)
-    variant("bf16_16rhs_ls12", "bf16", 16, Ls=12)
+    variant("f32_16rhs_ls12", "f32", 16, Ls=12, compare=True)
+    variant("bf16_16rhs_ls12", "bf16", 16, Ls=12, compare=True)
     return out
 
 

@@ -109,6 +109,7 @@ PROTOTYPES = {
     "gbx_dhop_get_cfg": (i32, [vp, vp, C.POINTER(KCfg)]),
     "gbx_dhop_time": (i32, [vp, vp, vp, i32, C.POINTER(KCfg), i32, pdbl]),
     "gbx_tune": (i32, [vp, vp, vp, i32, C.c_char_p, C.POINTER(KCfg), pdbl]),
+    "gbx_tune_kinds": (i32, [vp, vp, vp, i32, i32, C.c_char_p, C.POINTER(KCfg), pdbl]),
     "gbx_spinor_dot": (i32, [vp, vp, pdbl, pdbl]),
     "gbx_spinor_norm2": (i32, [vp, pdbl]),
     "gbx_spinor_axpy": (i32, [dbl, vp, vp]),

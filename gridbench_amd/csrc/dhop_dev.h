@@ -60,14 +60,15 @@ __device__ __forceinline__ void project(const A *v, A *hr, A *hi)
 }
 
 // u[sp*3+i] = sum_j U[i][j] h[sp*3+j]; U row-major complex, 18 reals.
-// FP64: one multiply + five fused multiply-adds per real output (840 FP64
-// VALU ops per 5D site instead of 1032 for the mul/fma/add form the
-// compiler derives from the plain expression).  FP32: the plain expression,
-// whose register schedule is far better for the 16-bit-storage kernels.
-template <typename A>
+// CHAIN (FP64 and FP32 storage): one multiply + five fused multiply-adds per
+// real output (72 ops per leg, 840 VALU ops per 5D site instead of 1032 for
+// the mul/fma/add form the compiler derives from the plain expression).
+// !CHAIN (16-bit storage): the plain expression, whose register schedule
+// keeps those kernels free of spills.
+template <bool CHAIN, typename A>
 __device__ __forceinline__ void su3_mul(const A *U, const A *hr, const A *hi, A *ur, A *ui)
 {
-  if constexpr (sizeof(A) == 4) {
+  if constexpr (!CHAIN) {
 #pragma unroll
     for (int sp = 0; sp < 2; sp++)
 #pragma unroll
@@ -205,12 +206,12 @@ __device__ __forceinline__ void load_nb(const void *__restrict__ in, uint32_t nb
 }
 
 // project, multiply by U (18 reals in LDS), reconstruct into r
-template <int K, bool FIRST, typename A>
+template <int K, bool FIRST, bool CHAIN, typename A>
 __device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri)
 {
   A hr[6], hi[6], ur[6], ui[6];
   project<K>(v, hr, hi);
-  su3_mul(U, hr, hi, ur, ui);
+  su3_mul<CHAIN>(U, hr, hi, ur, ui);
   recon<K, FIRST>(ur, ui, rr, ri);
 }
 
@@ -224,11 +225,12 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
                                           typename Traits<P>::acc *ri)
 {
   typedef typename Traits<P>::acc A;
+  constexpr bool CH = (P == GBX_F64 || P == GBX_F32);
   if (!PF) {
     A v[24];
 #define GBX_LEG(D, FIRST)                                                  \
     load_nb<P>(in, nb[D], M, m, v);                                        \
-    leg_compute<LegProj<DAG, D>::value, FIRST>(v, U + D * 18, rr, ri);
+    leg_compute<LegProj<DAG, D>::value, FIRST, CH>(v, U + D * 18, rr, ri);
     GBX_LEG(0, true)
     GBX_LEG(1, false)
     GBX_LEG(2, false)
@@ -242,20 +244,20 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
     A v0[24], v1[24];
     load_nb<P>(in, nb[0], M, m, v0);
     load_nb<P>(in, nb[1], M, m, v1);
-    leg_compute<LegProj<DAG, 0>::value, true>(v0, U + 0 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 0>::value, true, CH>(v0, U + 0 * 18, rr, ri);
     load_nb<P>(in, nb[2], M, m, v0);
-    leg_compute<LegProj<DAG, 1>::value, false>(v1, U + 1 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 1>::value, false, CH>(v1, U + 1 * 18, rr, ri);
     load_nb<P>(in, nb[3], M, m, v1);
-    leg_compute<LegProj<DAG, 2>::value, false>(v0, U + 2 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 2>::value, false, CH>(v0, U + 2 * 18, rr, ri);
     load_nb<P>(in, nb[4], M, m, v0);
-    leg_compute<LegProj<DAG, 3>::value, false>(v1, U + 3 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 3>::value, false, CH>(v1, U + 3 * 18, rr, ri);
     load_nb<P>(in, nb[5], M, m, v1);
-    leg_compute<LegProj<DAG, 4>::value, false>(v0, U + 4 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 4>::value, false, CH>(v0, U + 4 * 18, rr, ri);
     load_nb<P>(in, nb[6], M, m, v0);
-    leg_compute<LegProj<DAG, 5>::value, false>(v1, U + 5 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 5>::value, false, CH>(v1, U + 5 * 18, rr, ri);
     load_nb<P>(in, nb[7], M, m, v1);
-    leg_compute<LegProj<DAG, 6>::value, false>(v0, U + 6 * 18, rr, ri);
-    leg_compute<LegProj<DAG, 7>::value, false>(v1, U + 7 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 6>::value, false, CH>(v0, U + 6 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 7>::value, false, CH>(v1, U + 7 * 18, rr, ri);
   }
 }
 

@@ -58,7 +58,7 @@ __global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *
     }                                                                                                                    \
     T hr[6], hi[6], ur[6], ui[6];                                                                                        \
     dev::project<dev::LegProj<DAG, DIR>::value>(v, hr, hi);                                                              \
-    dev::su3_mul(Ul, hr, hi, ur, ui);                                                                                    \
+    dev::su3_mul<true>(Ul, hr, hi, ur, ui);                                                                                    \
     dev::recon<dev::LegProj<DAG, DIR>::value, FIRST>(ur, ui, rr, ri);                                                    \
   }
   GBX_TLEG(0, 3, true)

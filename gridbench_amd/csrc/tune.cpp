@@ -52,12 +52,12 @@ std::vector<int> minor_chunks(uint32_t M)
 
 // Stage 1: the work shape -- minor chunk, sites per workgroup, workgroup
 // size, gather pipeline -- with XCD ordering and streaming stores on.
-std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s *in)
+std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s *in, int kinds)
 {
   std::vector<gbx_kcfg> v;
   const uint32_t M = in->M;
   const int spws[] = {1, 2, 4, 8, 12, 16, 24, 32, 48, 64};
-  for (int mc : minor_chunks(M)) {
+  for (int mc : (kinds & GBX_KIND_VALU) ? minor_chunks(M) : std::vector<int>{}) {
     const uint32_t mw = mc ? (uint32_t)mc : M;  // minor indices per workgroup
     for (int spw : spws) {
       const uint64_t items = (uint64_t)spw * mw;
@@ -76,7 +76,7 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
     }
   }
   gbx_clear_error();
-  if (mfma_supported(u, in)) {
+  if ((kinds & GBX_KIND_MFMA) && mfma_supported(u, in)) {
     // MFMA form: minor columns per wavefront unit (multiples of 16 dividing
     // M), sites per workgroup, wavefronts per workgroup, next-tile gather.
     for (uint32_t mw = M; mw >= 16; mw /= 2) {
@@ -130,12 +130,14 @@ bool parse_line(const std::string &line, std::string &sig, gbx_kcfg &k, double &
 
 }  // namespace
 
-extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
-                        double *best_usec)
+extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kinds, const char *cache_path,
+                              gbx_kcfg *best, double *best_usec)
 {
   GBX_CHECK(u && out && in && best && best_usec, GBX_ENULL, "gbx_tune: NULL argument");
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_tune: dag=%d", dag);
-  const std::string sig = dhop_signature(u, in);
+  GBX_CHECK(kinds >= 1 && kinds <= 3, GBX_EINVAL, "gbx_tune: kinds=%d", kinds);
+  const bool all = kinds == (GBX_KIND_VALU | GBX_KIND_MFMA);
+  const std::string sig = dhop_signature(u, in) + (all ? "" : (kinds == GBX_KIND_VALU ? "#valu" : "#mfma"));
   *best = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
   *best_usec = 0.0;
   if (cache_path) {
@@ -152,19 +154,18 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
       // validate against this build before trusting it
       double us2;
       if (time_cfg(u, out, in, dag, *best, 2, &us2) == GBX_OK) {
-        dhop_remember_cfg(sig, *best);
+        if (all) dhop_remember_cfg(sig, *best);
         *best_usec = us2;
         return GBX_OK;
       }
       gbx_clear_error();
     }
   }
-  const std::vector<gbx_kcfg> cands = shape_candidates(u, in);
-  GBX_CHECK(!cands.empty(), GBX_EINVAL, "gbx_tune: no valid configuration");
+  const std::vector<gbx_kcfg> cands = shape_candidates(u, in, kinds);
+  GBX_CHECK(!cands.empty(), GBX_EUNSUPPORTED, "gbx_tune: no valid configuration of kinds %d for this problem", kinds);
   // Repetitions: about 10 ms of kernel time per candidate, at least 2.
   double us0 = 0.0;
-  gbx_kcfg def;
-  gbx_dhop_get_cfg(u, in, &def);
+  const gbx_kcfg def = cands.front();
   GBX_TRY(time_cfg(u, out, in, dag, def, 1, &us0));
   int nrep = (int)(10000.0 / (us0 > 1.0 ? us0 : 1.0));
   if (nrep < 2) nrep = 2;
@@ -189,7 +190,7 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
   search(refine_candidates(bk));
   *best = bk;
   *best_usec = bus;
-  dhop_remember_cfg(sig, bk);
+  if (all) dhop_remember_cfg(sig, bk);
   if (cache_path) {
     std::ofstream f(cache_path, std::ios::app);
     if (f)
@@ -198,4 +199,10 @@ extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, con
         << ", \"wgpc\": " << bk.wgpc << ", \"cblk\": " << bk.cblk << ", \"usec\": " << bus << "}\n";
   }
   return GBX_OK;
+}
+
+extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
+                        double *best_usec)
+{
+  return gbx_tune_kinds(u, out, in, dag, GBX_KIND_VALU | GBX_KIND_MFMA, cache_path, best, best_usec);
 }

@@ -314,11 +314,18 @@ def dhop_time(U, out, inp, dag=False, cfg=None, nrep=10):
     return us.value
 
 
-def tune(U, out, inp, dag=False, cache_path=None):
+def tune(U, out, inp, dag=False, cache_path=None, kinds=None):
+    """Autotune; kinds=None searches both kernel forms and sets what gbx_dhop
+    uses, kinds="valu" / "mfma" restricts the search (result not adopted)."""
     best = KCfg()
     us = C.c_double()
-    check(lib.gbx_tune(U.h, out.h, inp.h, int(bool(dag)), cache_path.encode() if cache_path else None, C.byref(best),
-                       C.byref(us)), "gbx_tune")
+    cp = cache_path.encode() if cache_path else None
+    if kinds is None:
+        check(lib.gbx_tune(U.h, out.h, inp.h, int(bool(dag)), cp, C.byref(best), C.byref(us)), "gbx_tune")
+    else:
+        k = {"valu": 1, "mfma": 2}[kinds]
+        check(lib.gbx_tune_kinds(U.h, out.h, inp.h, int(bool(dag)), k, cp, C.byref(best), C.byref(us)),
+              "gbx_tune_kinds")
     return best, us.value
 
 

@@ -208,6 +208,14 @@ int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx
 /* file (JSON lines) is read first and appended to (may be NULL).          */
 int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
              double *best_usec);
+/* The same search restricted to kernel kinds: bit 0 = VALU (kind 0), bit 1 */
+/* = MFMA (kind 1); kinds = 3 is gbx_tune.  A restricted result is cached  */
+/* under its own key and does not change what gbx_dhop uses.  For          */
+/* comparing the two forms on one problem.                                 */
+#define GBX_KIND_VALU 1
+#define GBX_KIND_MFMA 2
+int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kinds, const char *cache_path,
+                   gbx_kcfg *best, double *best_usec);
 
 /* ---------------------------------------------------------------------- */
 /* Vector algebra on spinors (used by the solver; deterministic order).    */

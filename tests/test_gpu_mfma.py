@@ -108,3 +108,24 @@ def test_tuner_considers_mfma(gpu, gbx, problem, tmp_path):
     got = y.download()
     for r in range(4):
         assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= TOL["f16"]
+
+
+def test_tune_restricted_to_one_kind(gpu, gbx, problem, tmp_path):
+    """gbx_tune_kinds: each restricted search returns a configuration of its
+    own kind, both agree with the oracle, neither changes what gbx_dhop uses;
+    FP32 storage has no MFMA candidates (GBX_EUNSUPPORTED)."""
+    p = problem((8, 4, 4, 8), 8, nrhs=2)
+    geom, U, x, y = fields(gbx, p, "bf16")
+    before = gbx.dhop_get_cfg(U, x).as_dict()
+    for kind, code in (("valu", 0), ("mfma", 1)):
+        cfg, us = gbx.tune(U, y, x, kinds=kind, cache_path=str(tmp_path / "k.jsonl"))
+        assert cfg.kind == code and us > 0
+        gbx.dhop(U, y, x, cfg=cfg)
+        got = y.download()
+        for r in range(2):
+            assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= TOL["bf16"]
+    assert gbx.dhop_get_cfg(U, x).as_dict() == before
+    geom, U, x, y = fields(gbx, p, "f32")
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.tune(U, y, x, kinds="mfma")
+    assert e.value.status == 8
