@@ -112,7 +112,7 @@ class Tuner:
         return self.gbx.tune(U, y, x, dag=dag, cache_path=self.cache, kinds=kinds)
 
 
-def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, only=None):
+def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, only=None, compare_forms=True):
     V = int(np.prod(DIMS))
     out = []
     c4 = {}  # the 24^4 x Ls12 x 16-RHS configuration: its own gauge field (seed 401), spinors seed 402
@@ -154,7 +154,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             for f in (x64, w):
                 f.close()
         forms = None
-        if compare:  # the matrix-core form against the best VALU configuration, both autotuned
+        if compare and compare_forms:  # the matrix-core form against the best VALU configuration, both autotuned
             forms = {}
             for kind in ("valu", "mfma"):
                 try:
@@ -309,6 +309,7 @@ def main():
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
     ap.add_argument("--headline-only", action="store_true", help="= --no-variants --no-mixed --no-cg --no-batch")
     ap.add_argument("--only", default="", help="comma-separated variant names to run (profiling)")
+    ap.add_argument("--no-compare", action="store_true", help="skip the MFMA-vs-VALU form comparison (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--tune-cache", default=TUNE_CACHE, help="autotuner cache (JSON lines); '' = tune afresh")
     ap.add_argument("--retune", action="store_true", help="delete the cache file first")
@@ -367,7 +368,8 @@ def main():
         gbx.dhop(U64, y64, x64, dag=True)
         ref64_dag = y64.download_rhs(0)
         extra["variants"] = variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_ev,
-                                               only=set(filter(None, args.only.split(","))))
+                                               only=set(filter(None, args.only.split(","))),
+                                               compare_forms=not args.no_compare)
     if not args.no_mixed:
         extra["mixed_precision"] = mixed_section(gbx, tune)
     if not args.no_cg:
