@@ -81,6 +81,8 @@ PROTOTYPES = {
     "gbx_gen_spinor": (i32, [pi32, i32, i32, u64, pdbl]),
     "gbx_geom_create": (i32, [pi32, pi32, C.POINTER(vp)]),
     "gbx_geom_create_levels": (i32, [pi32, i32, pi32, C.POINTER(vp)]),
+    "gbx_geom_create_rb": (i32, [pi32, pi32, C.POINTER(vp), C.POINTER(vp)]),
+    "gbx_geom_parity": (i32, [vp, pi32]),
     "gbx_geom_destroy": (i32, [vp]),
     "gbx_geom_volume": (i32, [vp, pu64]),
     "gbx_geom_stream": (i32, [vp, C.POINTER(vp)]),
@@ -105,6 +107,7 @@ PROTOTYPES = {
     "gbx_spinor_fill_gaussian": (i32, [vp, u64]),
     "gbx_dhop": (i32, [vp, vp, vp, i32]),
     "gbx_dhop_axpby": (i32, [vp, vp, vp, i32, dbl, dbl]),
+    "gbx_dhop_xpay": (i32, [vp, vp, vp, vp, i32, dbl, dbl]),
     "gbx_dhop_cfg": (i32, [vp, vp, vp, i32, C.POINTER(KCfg)]),
     "gbx_dhop_get_cfg": (i32, [vp, vp, C.POINTER(KCfg)]),
     "gbx_dhop_time": (i32, [vp, vp, vp, i32, C.POINTER(KCfg), i32, pdbl]),
@@ -117,6 +120,8 @@ PROTOTYPES = {
     "gbx_spinor_zero": (i32, [vp]),
     "gbx_cg_solve": (i32, [vp, vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]),
     "gbx_cg_solve_f64": (i32, [vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]),
+    "gbx_cg_solve_eo": (i32, [C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(CGParams),
+                              C.POINTER(CGStats)]),
     "gbx_batch_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32, C.POINTER(vp)]),
     "gbx_batch_destroy": (i32, [vp]),
     "gbx_batch_run": (i32, [vp]),

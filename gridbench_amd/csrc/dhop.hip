@@ -112,6 +112,7 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
   snprintf(buf, sizeof buf, "p%d_c%d_Ls%d_n%d_%dx%dx%dx%d", in->prec, u->comp12, in->Ls, in->nrhs, g->dims[0], g->dims[1],
            g->dims[2], g->dims[3]);
   std::string s = buf;
+  if (g->parity >= 0) s += g->parity ? "_odd" : "_even";
   for (int l = 0; l < g->nlevels; l++) {
     snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
              g->levels[l * 4 + 3]);
@@ -191,7 +192,10 @@ static int check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const g
   GBX_CHECK(u && out && in, GBX_ENULL, "dhop: NULL handle");
   GBX_CHECK(u->d && out->d && in->d, GBX_ENULL, "dhop: unallocated field");
   GBX_CHECK(out != in && out->d != in->d, GBX_EALIAS, "dhop: out aliases in");
-  GBX_CHECK(u->geom == out->geom && u->geom == in->geom, GBX_EMISMATCH, "dhop: fields on different geometries");
+  GBX_CHECK(u->geom == out->geom, GBX_EMISMATCH, "dhop: gauge and output fields on different geometries");
+  GBX_CHECK(dhop_input_geom_ok(out->geom, in->geom), GBX_EMISMATCH,
+            out->geom->parity < 0 ? "dhop: input field on a different geometry"
+                                  : "dhop: on a red-black half the input must live on the partner half of the same pair");
   GBX_CHECK(out->Ls == in->Ls && out->nrhs == in->nrhs, GBX_EMISMATCH, "dhop: in/out shapes differ");
   GBX_CHECK(out->prec == in->prec && u->prec == in->prec, GBX_EMISMATCH,
             "dhop: precisions differ (gauge %d, in %d, out %d)", u->prec, in->prec, out->prec);
@@ -207,8 +211,8 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   if (axpby) {
     GBX_CHECK(center && center->d, GBX_ENULL, "dhop_axpby: NULL center field");
     GBX_CHECK(center->d != out->d, GBX_EALIAS, "dhop_axpby: out aliases the center field");
-    GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == in->geom, GBX_EMISMATCH,
-              "dhop_axpby: center field shape differs");
+    GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == out->geom, GBX_EMISMATCH,
+              "dhop_axpby: the added field must have the output's geometry, precision and shape");
   }
   LaunchArgs a;
   a.nslot = (uint32_t)u->geom->vol;
@@ -314,14 +318,22 @@ extern "C" int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag)
   return dhop_run(u, out, in, dag, k, u->geom->stream);
 }
 
-extern "C" int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta)
+extern "C" int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta)
 {
   GBX_TRY(check_operands(u, out, in));
-  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_dhop_axpby: dag must be 0 or 1");
+  GBX_CHECK(x, GBX_ENULL, "gbx_dhop_xpay: NULL field x");
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_dhop_xpay: dag must be 0 or 1");
   gbx_kcfg k;
   gbx_dhop_get_cfg(u, in, &k);
   if (k.kind != 0) k = to_kcfg(dhop_default_cfg(u, in));
-  return dhop_launch(u, out, in, dag, from_kcfg(k), alpha, beta, in, u->geom->stream);
+  return dhop_launch(u, out, in, dag, from_kcfg(k), alpha, beta, x, u->geom->stream);
+}
+
+extern "C" int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta)
+{
+  GBX_CHECK(!u || u->geom->parity < 0, GBX_EINVAL,
+            "gbx_dhop_axpby: on a red-black half in and out have different parities; use gbx_dhop_xpay");
+  return gbx_dhop_xpay(u, out, in, in, dag, alpha, beta);
 }
 
 extern "C" int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg, int nrep, double *usec)

@@ -73,7 +73,30 @@ struct gbx_geom_s {
   uint32_t *site_of_slot = nullptr, *slot_of_site = nullptr;
   uint32_t *nbr = nullptr;  // [slot][8] neighbour slots, direction order Xp Yp Zp Tp Xm Ym Zm Tm
   int refcount = 1;
+  // Red-black halves.  parity -1: the whole lattice.  parity 0 / 1: the even /
+  // odd sites ((x+y+z+t) mod 2) of a pair made by gbx_geom_create_rb; then vol
+  // = V/2, site_of_slot holds checkerboarded lexicographic indices ((x/2)
+  // fastest), nbr holds slots of the partner half (same pair_id, other
+  // parity), and gsite_of_slot holds each slot's full-lattice lexicographic
+  // site (for the gauge field, which is uploaded from the full lattice).
+  int parity = -1;
+  uint64_t pair_id = 0;
+  uint32_t *gsite_of_slot = nullptr;
+  // The odd half runs on the even half's stream (work on the two halves of a
+  // pair reads across them, so it is ordered on one stream) and holds a
+  // reference to it; nullptr when the geometry owns its stream.
+  gbx_geom_s *stream_owner = nullptr;
 };
+
+// Full-lattice lexicographic site of each slot (the gauge upload's index).
+inline const uint32_t *gauge_sites(const gbx_geom_s *g) { return g->gsite_of_slot ? g->gsite_of_slot : g->site_of_slot; }
+// True when `in` holds the sites Dhop on `out_geom` reads: the same geometry
+// for a whole lattice, the partner half for a red-black half.
+inline bool dhop_input_geom_ok(const gbx_geom_s *out_geom, const gbx_geom_s *in_geom)
+{
+  if (out_geom->parity < 0) return in_geom == out_geom;
+  return in_geom->pair_id == out_geom->pair_id && in_geom->parity == 1 - out_geom->parity;
+}
 
 struct gbx_gauge_s {
   gbx_geom_s *geom;

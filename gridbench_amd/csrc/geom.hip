@@ -182,6 +182,7 @@ __global__ void k_pack_gauge_ref(const T *__restrict__ ref, void *__restrict__ d
 // ------------------------------------------------------------- host helpers
 int ref_geom(const gbx_geom_s *g, int nsimd, RefGeom &rg)
 {
+  GBX_CHECK(g->parity < 0, GBX_EUNSUPPORTED, "the reference's virtual-node layout describes a whole lattice, not a red-black half");
   rg.nsimd = nsimd;
   for (int d = 0; d < 4; d++) rg.simd[d] = 1;
   GBX_CHECK(nsimd == 1 || nsimd == 2 || nsimd == 4 || nsimd == 8 || nsimd == 16, GBX_EINVAL,
@@ -290,8 +291,11 @@ void geom_release(gbx_geom_s *g)
   (void)hipFree(g->site_of_slot);
   (void)hipFree(g->slot_of_site);
   (void)hipFree(g->nbr);
-  if (g->stream) (void)hipStreamDestroy(g->stream);
+  (void)hipFree(g->gsite_of_slot);
+  gbx_geom_s *owner = g->stream_owner;
+  if (g->stream && !owner) (void)hipStreamDestroy(g->stream);
   delete g;
+  if (owner) geom_release(owner);
 }
 
 extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out)
@@ -304,13 +308,10 @@ extern "C" int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *
   return gbx_geom_create_levels(dims, 1, blk, out);
 }
 
-// Slot order: lexicographic over the outermost blocks, then, inside each
-// block of level l, lexicographic over its level l-1 sub-blocks, down to
-// lexicographic order of the sites of a level-0 block.
-extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out)
+namespace {
+
+int check_dims_levels(const int dims[4], int nlevels, const int *blocks, uint64_t *vol_out)
 {
-  GBX_CHECK(dims && out && blocks, GBX_ENULL, "gbx_geom_create_levels: NULL argument");
-  *out = nullptr;
   GBX_CHECK(nlevels >= 1 && nlevels <= GBX_MAX_LEVELS, GBX_EINVAL, "gbx_geom_create_levels: nlevels=%d", nlevels);
   uint64_t vol = 1;
   for (int d = 0; d < 4; d++) {
@@ -324,18 +325,17 @@ extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int 
       GBX_CHECK(b >= 1 && outer % b == 0, GBX_EINVAL, "gbx_geom_create: level %d block[%d]=%d does not divide %d", l, d, b,
                 outer);
     }
-  GBX_TRY(gbx_require_device());
-  gbx_geom_s *g = new gbx_geom_s();
-  g->nlevels = nlevels;
-  for (int d = 0; d < 4; d++) {
-    g->dims[d] = dims[d];
-    g->block[d] = blocks[d];
-  }
-  for (int l = 0; l < nlevels * 4; l++) g->levels[l] = blocks[l];
-  g->vol = vol;
-  GBX_HIP(hipGetDevice(&g->device));
-  g->site_of_slot_h.resize(vol);
-  g->slot_of_site_h.resize(vol);
+  *vol_out = vol;
+  return GBX_OK;
+}
+
+// Slot order: lexicographic over the outermost blocks, then, inside each
+// block of level l, lexicographic over its level l-1 sub-blocks, down to
+// lexicographic order of the sites of a level-0 block.  Returns the
+// full-lattice slot of every lexicographic site.
+std::vector<uint32_t> blocked_slots(const int dims[4], int nlevels, const int *blocks, uint64_t vol)
+{
+  std::vector<uint32_t> slot_of_site(vol);
   for (uint64_t site = 0; site < vol; site++) {
     int x[4];
     uint64_t r = site;
@@ -361,38 +361,172 @@ extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int 
       w = w * (uint64_t)blocks[d] + (uint64_t)(x[d] % blocks[d]);
       n *= (uint64_t)blocks[d];
     }
-    slot = slot * n + w;
-    g->slot_of_site_h[site] = (uint32_t)slot;
-    g->site_of_slot_h[slot] = (uint32_t)site;
+    slot_of_site[site] = (uint32_t)(slot * n + w);
   }
-  std::vector<uint32_t> nbr(vol * 8);
-  uint64_t stride[4] = {1, (uint64_t)dims[0], (uint64_t)dims[0] * dims[1], (uint64_t)dims[0] * dims[1] * dims[2]};
-  for (uint64_t q = 0; q < vol; q++) {
-    const uint64_t site = g->site_of_slot_h[q];
-    for (int mu = 0; mu < 8; mu++) {
-      const int d = mu & 3;
-      const int x = (int)((site / stride[d]) % (uint64_t)dims[d]);
-      uint64_t ns;
-      if (mu < 4) ns = (x + 1 == dims[d]) ? site - (uint64_t)x * stride[d] : site + stride[d];
-      else ns = (x == 0) ? site + (uint64_t)(dims[d] - 1) * stride[d] : site - stride[d];
-      nbr[q * 8 + mu] = g->slot_of_site_h[ns];
-    }
+  return slot_of_site;
+}
+
+// Lexicographic site of the neighbour of `site` in direction mu (periodic).
+uint64_t nbr_site(uint64_t site, int mu, const int dims[4])
+{
+  const uint64_t stride[4] = {1, (uint64_t)dims[0], (uint64_t)dims[0] * dims[1], (uint64_t)dims[0] * dims[1] * dims[2]};
+  const int d = mu & 3;
+  const int x = (int)((site / stride[d]) % (uint64_t)dims[d]);
+  if (mu < 4) return (x + 1 == dims[d]) ? site - (uint64_t)x * stride[d] : site + stride[d];
+  return (x == 0) ? site + (uint64_t)(dims[d] - 1) * stride[d] : site - stride[d];
+}
+
+// Checkerboarded lexicographic index of a full-lattice site: (x/2) fastest.
+uint32_t half_index(uint64_t site, const int dims[4])
+{
+  const uint64_t x0 = site % (uint64_t)dims[0];
+  return (uint32_t)((site / (uint64_t)dims[0]) * (uint64_t)(dims[0] / 2) + x0 / 2);
+}
+
+int site_parity(uint64_t site, const int dims[4])
+{
+  int p = 0;
+  for (int d = 0; d < 4; d++) {
+    p += (int)(site % (uint64_t)dims[d]);
+    site /= (uint64_t)dims[d];
   }
-  int st = GBX_OK;
+  return p & 1;
+}
+
+gbx_geom_s *geom_new(const int dims[4], int nlevels, const int *blocks, uint64_t vol, int device)
+{
+  gbx_geom_s *g = new gbx_geom_s();
+  g->nlevels = nlevels;
+  for (int d = 0; d < 4; d++) {
+    g->dims[d] = dims[d];
+    g->block[d] = blocks[d];
+  }
+  for (int l = 0; l < nlevels * 4; l++) g->levels[l] = blocks[l];
+  g->vol = vol;
+  g->device = device;
+  return g;
+}
+
+// Stream (its own, or that of stream_owner) and device copies of the
+// tables; releases g on failure.
+int geom_finish(gbx_geom_s *g, const std::vector<uint32_t> &nbr, const std::vector<uint32_t> *gsite)
+{
+  const uint64_t vol = g->vol;
   auto fail = [&](int s) {
     geom_release(g);
     return s;
   };
-  if (hipStreamCreate(&g->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "hipStreamCreate failed"));
+  if (g->stream_owner) g->stream = g->stream_owner->stream;
+  else if (hipStreamCreate(&g->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "hipStreamCreate failed"));
   if (hipMalloc(&g->site_of_slot, vol * 4) != hipSuccess || hipMalloc(&g->slot_of_site, vol * 4) != hipSuccess ||
-      hipMalloc(&g->nbr, vol * 32) != hipSuccess)
+      hipMalloc(&g->nbr, vol * 32) != hipSuccess || (gsite && hipMalloc(&g->gsite_of_slot, vol * 4) != hipSuccess))
     return fail(gbx_fail(GBX_ENOMEM, "gbx_geom_create: device allocation failed"));
   if (hipMemcpy(g->site_of_slot, g->site_of_slot_h.data(), vol * 4, hipMemcpyHostToDevice) != hipSuccess ||
       hipMemcpy(g->slot_of_site, g->slot_of_site_h.data(), vol * 4, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(g->nbr, nbr.data(), vol * 32, hipMemcpyHostToDevice) != hipSuccess)
+      hipMemcpy(g->nbr, nbr.data(), vol * 32, hipMemcpyHostToDevice) != hipSuccess ||
+      (gsite && hipMemcpy(g->gsite_of_slot, gsite->data(), vol * 4, hipMemcpyHostToDevice) != hipSuccess))
     return fail(gbx_fail(GBX_EHIP, "gbx_geom_create: upload failed"));
-  (void)st;
+  return GBX_OK;
+}
+
+uint64_t g_next_pair_id = 0;
+std::mutex g_pair_mu;
+
+}  // namespace
+
+extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out)
+{
+  GBX_CHECK(dims && out && blocks, GBX_ENULL, "gbx_geom_create_levels: NULL argument");
+  *out = nullptr;
+  uint64_t vol;
+  GBX_TRY(check_dims_levels(dims, nlevels, blocks, &vol));
+  GBX_TRY(gbx_require_device());
+  int device;
+  GBX_HIP(hipGetDevice(&device));
+  gbx_geom_s *g = geom_new(dims, nlevels, blocks, vol, device);
+  g->slot_of_site_h = blocked_slots(dims, nlevels, blocks, vol);
+  g->site_of_slot_h.resize(vol);
+  for (uint64_t site = 0; site < vol; site++) g->site_of_slot_h[g->slot_of_site_h[site]] = (uint32_t)site;
+  std::vector<uint32_t> nbr(vol * 8);
+  for (uint64_t q = 0; q < vol; q++)
+    for (int mu = 0; mu < 8; mu++) nbr[q * 8 + mu] = g->slot_of_site_h[nbr_site(g->site_of_slot_h[q], mu, dims)];
+  GBX_TRY(geom_finish(g, nbr, nullptr));
   *out = g;
+  return GBX_OK;
+}
+
+// Red-black pair: the even and odd sites of the lattice, each in the blocked
+// order of the full lattice restricted to its parity (so a half's
+// consecutive slots stay spatially compact), each half's neighbour table
+// pointing into the other half.
+extern "C" int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geom *even, gbx_geom *odd)
+{
+  GBX_CHECK(dims && even && odd, GBX_ENULL, "gbx_geom_create_rb: NULL argument");
+  *even = *odd = nullptr;
+  for (int d = 0; d < 4; d++)
+    GBX_CHECK(dims[d] >= 2 && dims[d] % 2 == 0, GBX_EINVAL, "gbx_geom_create_rb: dims[%d]=%d must be even", d, dims[d]);
+  int blk[4];
+  for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : largest_divisor_le(dims[d], 6);
+  uint64_t vol;
+  GBX_TRY(check_dims_levels(dims, 1, blk, &vol));
+  GBX_TRY(gbx_require_device());
+  int device;
+  GBX_HIP(hipGetDevice(&device));
+  const std::vector<uint32_t> full_slot = blocked_slots(dims, 1, blk, vol);
+  std::vector<uint32_t> full_site(vol);
+  for (uint64_t site = 0; site < vol; site++) full_site[full_slot[site]] = (uint32_t)site;
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> lk(g_pair_mu);
+    id = ++g_next_pair_id;
+  }
+  const uint64_t half = vol / 2;
+  gbx_geom_s *h[2];
+  std::vector<uint32_t> gsite[2];
+  for (int p = 0; p < 2; p++) {
+    h[p] = geom_new(dims, 1, blk, half, device);
+    h[p]->parity = p;
+    h[p]->pair_id = id;
+    h[p]->site_of_slot_h.reserve(half);
+    h[p]->slot_of_site_h.assign(half, 0);
+    gsite[p].reserve(half);
+  }
+  for (uint64_t q = 0; q < vol; q++) {
+    const uint64_t site = full_site[q];
+    const int p = site_parity(site, dims);
+    const uint32_t hi = half_index(site, dims);
+    h[p]->slot_of_site_h[hi] = (uint32_t)h[p]->site_of_slot_h.size();
+    h[p]->site_of_slot_h.push_back(hi);
+    gsite[p].push_back((uint32_t)site);
+  }
+  std::vector<uint32_t> nbr[2];
+  for (int p = 0; p < 2; p++) {
+    nbr[p].resize(half * 8);
+    for (uint64_t q = 0; q < half; q++)
+      for (int mu = 0; mu < 8; mu++)
+        nbr[p][q * 8 + mu] = h[1 - p]->slot_of_site_h[half_index(nbr_site(gsite[p][q], mu, dims), dims)];
+  }
+  int rc = geom_finish(h[0], nbr[0], &gsite[0]);  // releases h[0] on failure
+  if (rc != GBX_OK) {
+    geom_release(h[1]);
+    return rc;
+  }
+  h[1]->stream_owner = h[0];
+  geom_retain(h[0]);
+  rc = geom_finish(h[1], nbr[1], &gsite[1]);  // releases h[1] (and its hold on h[0]) on failure
+  if (rc != GBX_OK) {
+    geom_release(h[0]);
+    return rc;
+  }
+  *even = h[0];
+  *odd = h[1];
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_parity(gbx_geom g, int *parity)
+{
+  GBX_CHECK(g && parity, GBX_ENULL, "gbx_geom_parity: NULL argument");
+  *parity = g->parity;
   return GBX_OK;
 }
 
@@ -461,12 +595,13 @@ extern "C" int gbx_gauge_upload(gbx_gauge u, const double *doubled)
   gbx_geom_s *g = u->geom;
   std::lock_guard<std::mutex> lk(g_staging_mu);
   void *stage;
-  const size_t in_bytes = g->vol * 8 * 18 * sizeof(double);
+  const uint64_t full_vol = g->parity < 0 ? g->vol : 2 * g->vol;  // a red-black half reads the whole lattice
+  const size_t in_bytes = full_vol * 8 * 18 * sizeof(double);
   GBX_TRY(staging_buffer(in_bytes, &stage));
   GBX_HIP(hipMemcpyAsync(stage, doubled, in_bytes, hipMemcpyHostToDevice, g->stream));
   const int Rl = u->comp12 ? 12 : 18;
   const uint64_t total = g->vol * 8 * (uint64_t)Rl;
-  dispatch_prec<PackGaugeLex>(u->prec, (const double *)stage, u->d, (const uint32_t *)g->site_of_slot, total, Rl, g->stream);
+  dispatch_prec<PackGaugeLex>(u->prec, (const double *)stage, u->d, gauge_sites(g), total, Rl, g->stream);
   GBX_HIP(hipGetLastError());
   GBX_HIP(hipStreamSynchronize(g->stream));
   return GBX_OK;

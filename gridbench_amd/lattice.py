@@ -114,12 +114,15 @@ def dslash_last_path():
 class Geometry:
     """Periodic 4D lattice on the GPU.  `block` is the traversal block; or
     `levels` = [innermost block, ..., outermost block] for a multi-level
-    (blocks-of-blocks) site order."""
+    (blocks-of-blocks) site order.  Geometry.rb_pair(dims) gives the even and
+    odd halves of a red-black pair (parity -1 / 0 / 1; volume = sites held)."""
 
-    def __init__(self, dims, block=None, levels=None):
+    def __init__(self, dims, block=None, levels=None, _handle=None):
         self.dims = tuple(int(d) for d in dims)
         h = C.c_void_p()
-        if levels is not None:
+        if _handle is not None:
+            h = _handle
+        elif levels is not None:
             flat = [int(b) for lv in levels for b in lv]
             check(lib.gbx_geom_create_levels(ivec(self.dims), len(levels), ivec(flat), C.byref(h)),
                   "gbx_geom_create_levels")
@@ -127,7 +130,21 @@ class Geometry:
             check(lib.gbx_geom_create(ivec(self.dims), ivec(block) if block is not None else None, C.byref(h)),
                   "gbx_geom_create")
         self.h = h
-        self.volume = int(np.prod(self.dims))
+        v = C.c_uint64()
+        check(lib.gbx_geom_volume(h, C.byref(v)), "gbx_geom_volume")
+        self.volume = int(v.value)
+        par = C.c_int()
+        check(lib.gbx_geom_parity(h, C.byref(par)), "gbx_geom_parity")
+        self.parity = int(par.value)
+
+    @classmethod
+    def rb_pair(cls, dims, block=None):
+        """(even, odd) half-lattice geometries of one red-black pair."""
+        dims = tuple(int(d) for d in dims)
+        he, ho = C.c_void_p(), C.c_void_p()
+        check(lib.gbx_geom_create_rb(ivec(dims), ivec(block) if block is not None else None, C.byref(he),
+                                     C.byref(ho)), "gbx_geom_create_rb")
+        return cls(dims, _handle=he), cls(dims, _handle=ho)
 
     def sync(self):
         check(lib.gbx_sync(self.h), "gbx_sync")
@@ -154,9 +171,10 @@ class GaugeField:
         self.h = h
 
     def upload(self, doubled):
-        """Doubled field, lexicographic float64 [V][8][3][3][2]."""
+        """Doubled field of the WHOLE lattice, lexicographic float64
+        [V][8][3][3][2] (a red-black half keeps its own sites' links)."""
         a = np.ascontiguousarray(doubled, dtype=np.float64)
-        assert a.size == self.geom.volume * 144
+        assert a.size == int(np.prod(self.geom.dims)) * 144
         check(lib.gbx_gauge_upload(self.h, ptr(a, C.c_double)), "gbx_gauge_upload")
         return self
 
@@ -294,6 +312,11 @@ def dhop(U, out, inp, dag=False, cfg=None):
     return out
 
 
+def dhop_xpay(U, out, x, inp, alpha, beta, dag=False):
+    """out = alpha*x + beta*Dhop(inp), x on out's geometry (red-black halves too)."""
+    check(lib.gbx_dhop_xpay(U.h, out.h, x.h, inp.h, int(bool(dag)), float(alpha), float(beta)), "gbx_dhop_xpay")
+
+
 def dhop_axpby(U, out, inp, alpha, beta, dag=False):
     """out = alpha*inp + beta*Dhop inp."""
     check(lib.gbx_dhop_axpby(U.h, out.h, inp.h, int(bool(dag)), float(alpha), float(beta)), "gbx_dhop_axpby")
@@ -339,6 +362,52 @@ def cg_solve(U_hi, U_lo, x, b, kappa, tol=1e-10, inner_tol=1e-3, max_outer=50, m
         rc = lib.gbx_cg_solve(U_hi.h, U_lo.h, x.h, b.h, C.byref(p), C.byref(st))
     check(rc, "gbx_cg_solve")
     return st.as_dict()
+
+
+def cg_solve_eo(U_hi, U_lo, x, b, kappa, tol=1e-10, inner_tol=1e-3, max_outer=50, max_inner=2000):
+    """Even-odd (Schur complement) solve of M x = b, M = 1 - kappa Dhop, on a
+    red-black pair.  U_hi, x, b: (even, odd) pairs of FP64 objects; U_lo an
+    (even, odd) pair of FP32/FP16/BF16 gauge fields or None (FP64 only)."""
+    p = CGParams(kappa, tol, inner_tol, max_outer, max_inner)
+    st = CGStats()
+    arr = lambda objs: (C.c_void_p * 2)(*[o.h for o in objs])
+    check(lib.gbx_cg_solve_eo(arr(U_hi), arr(U_lo) if U_lo is not None else None, arr(x), arr(b), C.byref(p),
+                              C.byref(st)), "gbx_cg_solve_eo")
+    return st.as_dict()
+
+
+def rb_split(dims, lex):
+    """Whole-lattice field [..., V, ...] (site axis -1 of the leading shape is
+    the lexicographic site) -> (even, odd) halves in checkerboarded
+    lexicographic order ((x/2) fastest).  `lex` shape [nrhs, V, ...]."""
+    L0 = int(dims[0])
+    V = int(np.prod(dims))
+    site = np.arange(V)
+    x = [(site // int(np.prod(dims[:d]))) % int(dims[d]) for d in range(4)]
+    par = (x[0] + x[1] + x[2] + x[3]) % 2
+    hidx = (site // L0) * (L0 // 2) + x[0] // 2
+    out = []
+    for p in (0, 1):
+        sel = np.nonzero(par == p)[0]
+        h = np.empty((lex.shape[0], V // 2) + lex.shape[2:], dtype=lex.dtype)
+        h[:, hidx[sel]] = lex[:, sel]
+        out.append(h)
+    return out[0], out[1]
+
+
+def rb_merge(dims, even, odd):
+    """Inverse of rb_split."""
+    L0 = int(dims[0])
+    V = int(np.prod(dims))
+    site = np.arange(V)
+    x = [(site // int(np.prod(dims[:d]))) % int(dims[d]) for d in range(4)]
+    par = (x[0] + x[1] + x[2] + x[3]) % 2
+    hidx = (site // L0) * (L0 // 2) + x[0] // 2
+    lex = np.empty((even.shape[0], V) + even.shape[2:], dtype=even.dtype)
+    for p, h in ((0, even), (1, odd)):
+        sel = np.nonzero(par == p)[0]
+        lex[:, sel] = h[:, hidx[sel]]
+    return lex
 
 
 class Batch:
@@ -400,5 +469,6 @@ __all__ = [
     "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "dhop", "dhop_axpby",
-    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site",
+    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
+    "cg_solve_eo", "rb_split", "rb_merge",
 ]

@@ -125,6 +125,19 @@ int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out);
 /* innermost first; each level divides the next and the last divides dims.  */
 #define GBX_MAX_LEVELS 4
 int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out);
+/* Red-black (even-odd) pair: the sites of parity (x+y+z+t) mod 2 = 0 and 1  */
+/* as two half-lattice geometries (every dim even).  The 4D checkerboard of */
+/* the reference's unused red-black grids (TableGenerate.cc:26,28), with Ls */
+/* not checkerboarded.  On a half:                                          */
+/*  - spinor host arrays are [nrhs][V/2][Ls][24] in checkerboarded          */
+/*    lexicographic order: index (x/2) + (Lx/2)*(y + Ly*(z + Lz*t));          */
+/*  - gbx_gauge_upload reads the FULL-lattice doubled field [V][8][18] and   */
+/*    keeps the links of the half's own sites;                              */
+/*  - gbx_dhop(u, out, in) with out on one half and in on the other applies */
+/*    the hopping term between them: DhopEO (out even) / DhopOE (out odd).  */
+int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geom *even, gbx_geom *odd);
+/* -1 for a whole lattice, 0 / 1 for the even / odd half of a pair. */
+int gbx_geom_parity(gbx_geom g, int *parity);
 int gbx_geom_destroy(gbx_geom g);
 int gbx_geom_volume(gbx_geom g, uint64_t *vol);
 /* The geometry's stream (hipStream_t) on which all its work runs. */
@@ -176,6 +189,9 @@ int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag);
 /* out = alpha*in + beta*Dhop(in): with alpha = 1, beta = -kappa this is   */
 /* M = 1 - kappa*Dhop in one pass.                                         */
 int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta);
+/* out = alpha*x + beta*Dhop(in) with x on out's geometry: on a red-black   */
+/* half, x is a field of out's parity (e.g. x - kappa^2 DhopEO DhopOE x).    */
+int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta);
 
 /* Explicit launch configuration of the VALU kernel. */
 typedef struct {
@@ -254,6 +270,20 @@ int gbx_cg_solve(gbx_gauge u_hi, gbx_gauge u_lo, gbx_spinor x, gbx_spinor b, con
                  gbx_cg_stats *st);
 /* The same solve in FP64 only (no inner low-precision solves). */
 int gbx_cg_solve_f64(gbx_gauge u, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p, gbx_cg_stats *st);
+/* Even-odd preconditioned solve of M x = b (not the normal equations),    */
+/* M = 1 - kappa*Dhop, on a red-black pair, by the Schur complement on the  */
+/* even sites:                                                             */
+/*   b'_e = b_e + kappa DhopEO b_o                                          */
+/*   (1 - kappa^2 DhopEO DhopOE) x_e = b'_e   defect-correction CGNR: FP64  */
+/*       residual and updates outside, FP32/FP16/BF16 inner solves          */
+/*   x_o = b_o + kappa DhopOE x_e                                           */
+/* u_hi[2] (FP64) and u_lo[2] (NULL array for FP64 only) are the gauge      */
+/* fields of the even and odd halves; x[2], b[2] FP64 fields, even first.   */
+/* st->true_resid = |b - M x| / |b| over the whole lattice; hi_applies and  */
+/* lo_applies count applications of the Schur operator.  Reaches the same  */
+/* solution as gbx_cg_solve on M^dag M x = M^dag b in about half the work.  */
+int gbx_cg_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x[2], const gbx_spinor b[2],
+                    const gbx_cg_params *p, gbx_cg_stats *st);
 
 /* ---------------------------------------------------------------------- */
 /* Batched application over many independent lattices (each its own      */
