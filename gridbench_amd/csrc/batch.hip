@@ -131,7 +131,8 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   uint32_t max_spw = 1;
   for (int i = 0; i < n; i++) {
     GBX_CHECK(u[i] && out[i] && in[i], GBX_ENULL, "gbx_batch_create: NULL handle at %d", i);
-    GBX_CHECK(u[i]->geom == in[i]->geom && u[i]->geom == out[i]->geom, GBX_EMISMATCH, "gbx_batch_create: lattice %d mixes geometries", i);
+    GBX_CHECK(u[i]->geom == out[i]->geom && dhop_input_geom_ok(out[i]->geom, in[i]->geom), GBX_EMISMATCH,
+              "gbx_batch_create: lattice %d mixes geometries (a red-black half reads its partner half)", i);
     GBX_CHECK(u[i]->prec == prec && in[i]->prec == prec && out[i]->prec == prec, GBX_EMISMATCH,
               "gbx_batch_create: lattice %d precision differs from lattice 0", i);
     GBX_CHECK(in[i]->M == out[i]->M && in[i]->d != out[i]->d, GBX_EMISMATCH, "gbx_batch_create: lattice %d in/out mismatch", i);

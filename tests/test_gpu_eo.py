@@ -166,3 +166,23 @@ def test_red_black_errors(gpu, gbx, problem):
     with pytest.raises(gbx.GbxError) as e:
         xe.upload_ref(np.zeros(64 * 2 * 24 // 2), 1)
     assert e.value.status == 8
+
+
+def test_batch_of_half_lattice_hops(gpu, gbx, problem):
+    """The persistent batch kernel takes red-black halves too (DhopEO and
+    DhopOE of two lattices in one launch) and matches the single launches."""
+    objs = []
+    for i, (dims, Ls) in enumerate((((4, 4, 4, 4), 4), ((8, 4, 2, 4), 3))):
+        p = problem(dims, Ls, gseed=700 + i, sseed=710 + i)
+        ge, go, Ue, Uo = halves(gbx, p, "f32")
+        pe, po = gbx.rb_split(p.dims, p.psi)
+        xe = gbx.SpinorField(ge, Ls, 1, "f32").upload(pe)
+        xo = gbx.SpinorField(go, Ls, 1, "f32").upload(po)
+        objs += [(Ue, gbx.SpinorField(ge, Ls, 1, "f32"), xo), (Uo, gbx.SpinorField(go, Ls, 1, "f32"), xe)]
+    b = gbx.Batch([o[0] for o in objs], [o[1] for o in objs], [o[2] for o in objs])
+    b.run()
+    got = [o[1].download() for o in objs]
+    for (U, y, x), g in zip(objs, got):
+        ref = gbx.SpinorField(y.geom, y.Ls, 1, "f32")
+        gbx.dhop(U, ref, x, cfg=gbx.kcfg(1, 64, 0, 0))
+        assert np.array_equal(ref.download(), g)
