@@ -20,7 +20,9 @@ Also reported, each with its own seeds:
              against FP64 on the same input.
   cg         32^4 x Ls16: defect-correction CG on M^dag M x = b,
              M = 1 - 0.12 Dhop, b Gaussian, to 1e-10 relative residual, inner
-             solves in FP32/BF16/FP16 (and FP64 only, for comparison).
+             solves in FP32/BF16/FP16 (and FP64 only, for comparison); and
+             M x = b by the even-odd Schur complement on a red-black pair
+             against CG on the normal equations over the whole lattice.
   batch      40 lattices L^4 x Ls x Nrhs (L in {8,12,16,20,24}, Ls in
              {8,12,16}, Nrhs in 1..8, drawn with seed 505), FP32, one
              persistent graph-captured launch; aggregate GFLOP/s and the
@@ -259,6 +261,61 @@ def cg_section(gbx, tune):
             "rhs_seed_device": 303, "rows": rows}
 
 
+def cg_eo_section(gbx, tune):
+    """32^4 x Ls16: M x = b by the even-odd Schur complement (defect-correction
+    CGNR on the even sites) against CG on the normal equations M^dag M x =
+    M^dag b over the whole lattice, same b, FP32 inner solves for the latter."""
+    dims, Ls = (32, 32, 32, 32), 16
+    dbl = gbx.gen_doubled_gauge(dims, 301)
+    b_lex = gbx.gen_spinor(dims, Ls, 1, 304)
+    be, bo = gbx.rb_split(dims, b_lex)
+    del b_lex
+    ge, go = gbx.Geometry.rb_pair(dims)
+    Ue = gbx.GaugeField(ge, "f64").upload(dbl)
+    Uo = gbx.GaugeField(go, "f64").upload(dbl)
+    b = (gbx.SpinorField(ge, Ls, 1, "f64").upload(be), gbx.SpinorField(go, Ls, 1, "f64").upload(bo))
+    x = (gbx.SpinorField(ge, Ls, 1, "f64"), gbx.SpinorField(go, Ls, 1, "f64"))
+    for U, xin, yout in ((Ue, x[1], x[0]), (Uo, x[0], x[1])):
+        tune(U, yout, xin)
+    rows = []
+    for lo in ("f32", "f16", "bf16", None):
+        low = None
+        if lo is not None:
+            low = (gbx.GaugeField(ge, lo).upload(dbl), gbx.GaugeField(go, lo).upload(dbl))
+            ts = (gbx.SpinorField(ge, Ls, 1, lo), gbx.SpinorField(go, Ls, 1, lo))
+            tune(low[0], ts[0], ts[1])
+            tune(low[1], ts[1], ts[0])
+        for f in x:
+            f.zero()
+        st = gbx.cg_solve_eo((Ue, Uo), low, x, b, KAPPA, tol=1e-10, inner_tol=1e-3, max_outer=60, max_inner=5000)
+        r = {"inner": lo or "f64", **st, "converged": st["true_resid"] <= 1e-10}
+        log(f"cg_eo {r}")
+        rows.append(r)
+    for f in (Ue, Uo, x[0], x[1], b[0], b[1]):
+        f.close()
+    # the same system on the whole lattice through the normal equations
+    g = gbx.Geometry(dims)
+    U = gbx.GaugeField(g, "f64").upload(dbl)
+    U32 = gbx.GaugeField(g, "f32").upload(dbl)
+    bf = gbx.SpinorField(g, Ls, 1, "f64").upload(gbx.rb_merge(dims, be, bo))
+    del be, bo
+    mdb = gbx.SpinorField(g, Ls, 1, "f64")
+    xf = gbx.SpinorField(g, Ls, 1, "f64").zero()
+    tune(U, mdb, bf)
+    gbx.dhop_axpby(U, mdb, bf, 1.0, -KAPPA, dag=True)
+    st = gbx.cg_solve(U, U32, xf, mdb, KAPPA, tol=1e-10, inner_tol=1e-3, max_outer=60, max_inner=5000)
+    # |b - M x| / |b| of that solution, like the even-odd solver reports
+    t = gbx.SpinorField(g, Ls, 1, "f64")
+    gbx.dhop_axpby(U, t, xf, 1.0, -KAPPA)
+    t.axpy(-1.0, bf)
+    full = {"inner": "f32", **st, "m_resid": float(f"{np.sqrt(t.norm2() / bf.norm2()):.3e}")}
+    log(f"cg normal equations {full}")
+    eo32 = rows[0]
+    return {"dims": list(dims), "Ls": Ls, "kappa": KAPPA, "tol": 1e-10, "gauge_seed": 301, "b_seed": 304,
+            "system": "M x = b, M = 1 - kappa Dhop", "even_odd": rows, "normal_equations_whole_lattice": full,
+            "speedup_even_odd_f32_vs_normal_f32": round(full["seconds"] / eo32["seconds"], 2)}
+
+
 def batch_section(gbx, nrep):
     """40 uneven lattices in one persistent graph-captured launch, FP32."""
     rng = np.random.default_rng(505)
@@ -374,6 +431,7 @@ def main():
         extra["mixed_precision"] = mixed_section(gbx, tune)
     if not args.no_cg:
         extra["cg"] = cg_section(gbx, tune)
+        extra["cg_even_odd"] = cg_eo_section(gbx, tune)
     if not args.no_batch:
         extra["batch"] = batch_section(gbx, 20)
 
