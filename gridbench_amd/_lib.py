@@ -51,6 +51,10 @@ class CGParams(C.Structure):
                 ("max_outer", C.c_int), ("max_inner", C.c_int)]
 
 
+class DwfParams(C.Structure):
+    _fields_ = [("M5", C.c_double), ("mass", C.c_double)]
+
+
 class CGStats(C.Structure):
     _fields_ = [("outer_iters", C.c_int), ("inner_iters", C.c_int), ("true_resid", C.c_double),
                 ("seconds", C.c_double), ("hi_applies", C.c_int), ("lo_applies", C.c_int)]
@@ -122,6 +126,10 @@ PROTOTYPES = {
     "gbx_cg_solve_f64": (i32, [vp, vp, vp, C.POINTER(CGParams), C.POINTER(CGStats)]),
     "gbx_cg_solve_eo": (i32, [C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(CGParams),
                               C.POINTER(CGStats)]),
+    "gbx_dwf_mooee": (i32, [vp, vp, C.POINTER(DwfParams), i32, i32]),
+    "gbx_dwf_apply": (i32, [vp, vp, vp, C.POINTER(DwfParams), i32]),
+    "gbx_dwf_solve_eo": (i32, [C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(DwfParams),
+                               C.POINTER(CGParams), C.POINTER(CGStats)]),
     "gbx_batch_create": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32, C.POINTER(vp)]),
     "gbx_batch_destroy": (i32, [vp]),
     "gbx_batch_run": (i32, [vp]),

@@ -215,19 +215,30 @@ dhop_cfg solver_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in)
 //   whole lattice:  A = M^dag M        (u = the lattice's gauge field, t)
 //   red-black pair: A = Mpc^dag Mpc     (ue / uo = even / odd halves' fields,
 //                   t on the even half, to on the odd half)
+enum { OP_WILSON = 0, OP_DWF = 1 };
+
 struct NormalOp {
   bool eo;
   const gbx_gauge_s *ue, *uo;
   double kappa;
   gbx_spinor_s *t, *to;
+  int kind = OP_WILSON;       // red-black only: M = 1 - kappa Dhop, or the DWF operator
+  gbx_dwf_params dwf{0, 0};
+  gbx_spinor_s *te = nullptr;  // DWF: a second even-half temporary
 };
 
-// y = Mpc x (dag = 0) or Mpc^dag x (dag = 1):
-//   to = DhopOE x;  y = x - kappa^2 DhopEO to      (dagger kernels for dag)
-// ((DhopEO DhopOE)^dag = (Dhop^dag)_EO (Dhop^dag)_OE.)
+// y = Mpc x (dag = 0) or Mpc^dag x (dag = 1), with dagger kernels for dag
+// ((DhopEO DhopOE)^dag = (Dhop^dag)_EO (Dhop^dag)_OE):
+//   Wilson:  to = DhopOE x;  y = x - kappa^2 DhopEO to
+//   DWF:     to = Mooee^-1 DhopOE x;  te = DhopEO to;  y = Mooee x - te / 4
 int apply_schur(const NormalOp &op, gbx_spinor_s *y, const gbx_spinor_s *x, int dag, hipStream_t st)
 {
   GBX_TRY(dhop_launch(op.uo, op.to, x, dag, solver_cfg(op.uo, x), 0.0, 1.0, nullptr, st));
+  if (op.kind == OP_DWF) {
+    GBX_TRY(dwf_s5_launch(op.to, op.to, nullptr, &op.dwf, 1, dag, 1.0, 0.0, st));
+    GBX_TRY(dhop_launch(op.ue, op.te, op.to, dag, solver_cfg(op.ue, op.to), 0.0, 1.0, nullptr, st));
+    return dwf_s5_launch(y, x, op.te, &op.dwf, 0, dag, 1.0, -0.25, st);
+  }
   GBX_TRY(dhop_launch(op.ue, y, op.to, dag, solver_cfg(op.ue, op.to), 1.0, -op.kappa * op.kappa, x, st));
   return GBX_OK;
 }
@@ -398,30 +409,38 @@ int xpay(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *c, const g
   return dhop_launch(u, out, in, dag, solver_cfg(u, in), alpha, beta, c, st);
 }
 
-int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, gbx_spinor_s *const x[2],
-             const gbx_spinor_s *const b[2], const gbx_cg_params *prm, gbx_cg_stats *stats)
+// Even-odd Schur-complement solve on a red-black pair (kind = OP_WILSON:
+// M = 1 - kappa Dhop; OP_DWF: D = Mooee - Dhop / 2).
+int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo,
+                gbx_spinor_s *const x[2], const gbx_spinor_s *const b[2], const gbx_cg_params *prm, gbx_cg_stats *stats)
 {
-  GBX_CHECK(u_hi && x && b && prm && u_hi[0] && u_hi[1] && x[0] && x[1] && b[0] && b[1], GBX_ENULL, "cg_eo: NULL argument");
+  const char *who = kind == OP_DWF ? "dwf_eo" : "cg_eo";
+  GBX_CHECK(u_hi && x && b && prm && u_hi[0] && u_hi[1] && x[0] && x[1] && b[0] && b[1], GBX_ENULL, "%s: NULL argument", who);
   const gbx_geom_s *ge = u_hi[0]->geom, *go = u_hi[1]->geom;
   GBX_CHECK(ge->parity == 0 && go->parity == 1 && ge->pair_id == go->pair_id, GBX_EMISMATCH,
-            "cg_eo: u_hi must be the even then the odd half of one red-black pair");
+            "%s: u_hi must be the even then the odd half of one red-black pair", who);
   for (int p = 0; p < 2; p++) {
     const gbx_geom_s *g = p ? go : ge;
-    GBX_CHECK(u_hi[p]->prec == GBX_F64, GBX_EINVAL, "cg_eo: the outer gauge fields must be FP64");
-    GBX_CHECK(x[p]->prec == GBX_F64 && b[p]->prec == GBX_F64, GBX_EINVAL, "cg_eo: x and b must be FP64");
-    GBX_CHECK(x[p]->geom == g && b[p]->geom == g, GBX_EMISMATCH, "cg_eo: x[%d] / b[%d] not on the %s half", p, p,
+    GBX_CHECK(u_hi[p]->prec == GBX_F64, GBX_EINVAL, "%s: the outer gauge fields must be FP64", who);
+    GBX_CHECK(x[p]->prec == GBX_F64 && b[p]->prec == GBX_F64, GBX_EINVAL, "%s: x and b must be FP64", who);
+    GBX_CHECK(x[p]->geom == g && b[p]->geom == g, GBX_EMISMATCH, "%s: x[%d] / b[%d] not on the %s half", who, p, p,
               p ? "odd" : "even");
     GBX_CHECK(x[p]->nrhs == 1 && b[p]->nrhs == 1 && x[p]->Ls == x[0]->Ls && b[p]->Ls == x[0]->Ls, GBX_EMISMATCH,
-              "cg_eo: one right-hand side, equal Ls");
-    GBX_CHECK(x[p]->d != b[p]->d, GBX_EALIAS, "cg_eo: x aliases b");
+              "%s: one right-hand side, equal Ls", who);
+    GBX_CHECK(x[p]->d != b[p]->d, GBX_EALIAS, "%s: x aliases b", who);
     if (u_lo) {
-      GBX_CHECK(u_lo[p] && u_lo[p]->geom == g, GBX_EMISMATCH, "cg_eo: inner gauge field %d not on the %s half", p,
+      GBX_CHECK(u_lo[p] && u_lo[p]->geom == g, GBX_EMISMATCH, "%s: inner gauge field %d not on the %s half", who, p,
                 p ? "odd" : "even");
       GBX_CHECK(u_lo[p]->prec != GBX_F64 && u_lo[p]->prec == u_lo[0]->prec, GBX_EINVAL,
-                "cg_eo: inner gauge fields must share one of FP32, FP16, BF16");
+                "%s: inner gauge fields must share one of FP32, FP16, BF16", who);
     }
   }
-  GBX_TRY(check_params(prm));
+  if (kind == OP_DWF) {
+    GBX_CHECK(prm->tol > 0 && prm->inner_tol > 0 && prm->inner_tol < 1 && prm->max_outer >= 1 && prm->max_inner >= 1,
+              GBX_EINVAL, "%s: bad tolerances or iteration limits", who);
+  } else {
+    GBX_TRY(check_params(prm));
+  }
   const auto t0 = std::chrono::steady_clock::now();
   gbx_cg_stats st{};
   gbx_geom_s *g = u_hi[0]->geom, *g1 = u_hi[1]->geom;
@@ -430,13 +449,14 @@ int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, g
   const double kappa = prm->kappa;
   const int plo = u_lo ? u_lo[0]->prec : GBX_F64;
   const int pe = u_lo ? GBX_F32 : GBX_F64;
-  Tmp bp, r, y, t, to, e, ri, p, Ap, tl, tol_, ro;
+  Tmp bp, r, y, t, te, to, e, ri, p, Ap, tl, tel, tol_, ro;
   Scratch sc;
   GBX_TRY(sc.init());
   GBX_TRY(make(g, Ls, GBX_F64, bp));
   GBX_TRY(make(g, Ls, GBX_F64, r));
   GBX_TRY(make(g, Ls, GBX_F64, y));
   GBX_TRY(make(g, Ls, GBX_F64, t));
+  GBX_TRY(make(g, Ls, GBX_F64, te));
   GBX_TRY(make(g1, Ls, GBX_F64, to));
   GBX_TRY(make(g1, Ls, GBX_F64, ro));
   GBX_TRY(make(g, Ls, pe, e));
@@ -444,18 +464,30 @@ int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, g
   GBX_TRY(make(g, Ls, plo, p));
   GBX_TRY(make(g, Ls, plo, Ap));
   GBX_TRY(make(g, Ls, plo, tl));
+  GBX_TRY(make(g, Ls, plo, tel));
   GBX_TRY(make(g1, Ls, plo, tol_));
-  const NormalOp op_hi{true, u_hi[0], u_hi[1], kappa, t.s, to.s};
-  const NormalOp op_lo = u_lo ? NormalOp{true, u_lo[0], u_lo[1], kappa, tl.s, tol_.s} : NormalOp{true, u_hi[0], u_hi[1], kappa,
-                                                                                                   tl.s, tol_.s};
+  NormalOp op_hi{true, u_hi[0], u_hi[1], kappa, t.s, to.s};
+  NormalOp op_lo = u_lo ? NormalOp{true, u_lo[0], u_lo[1], kappa, tl.s, tol_.s} : NormalOp{true, u_hi[0], u_hi[1], kappa, tl.s,
+                                                                                           tol_.s};
+  op_hi.kind = op_lo.kind = kind;
+  if (kind == OP_DWF) {
+    op_hi.dwf = op_lo.dwf = *dwf;
+    op_hi.te = te.s;
+    op_lo.te = tel.s;
+  }
   double bbe, bbo;
   GBX_TRY(spinor_norm2_on(b[0], &bbe, s));
   GBX_TRY(spinor_norm2_on(b[1], &bbo, s));
-  GBX_CHECK(bbe + bbo > 0.0, GBX_EINVAL, "cg_eo: b is zero");
+  GBX_CHECK(bbe + bbo > 0.0, GBX_EINVAL, "%s: b is zero", who);
   const double bnorm = std::sqrt(bbe + bbo);
   const double N = (double)g->vol * Ls * 24;
-  // b'_e = b_e + kappa DhopEO b_o
-  GBX_TRY(xpay(u_hi[0], bp.s, b[0], b[1], 0, 1.0, kappa, s));
+  // b'_e:  Wilson b_e + kappa DhopEO b_o;  DWF b_e + DhopEO Mooee^-1 b_o / 2
+  if (kind == OP_DWF) {
+    GBX_TRY(dwf_s5_launch(to.s, b[1], nullptr, dwf, 1, 0, 1.0, 0.0, s));
+    GBX_TRY(xpay(u_hi[0], bp.s, b[0], to.s, 0, 1.0, 0.5, s));
+  } else {
+    GBX_TRY(xpay(u_hi[0], bp.s, b[0], b[1], 0, 1.0, kappa, s));
+  }
   double rel = 1.0;
   for (;;) {
     // r = b'_e - Mpc x_e in FP64; |r| is |b - M x| once x_o is rebuilt
@@ -478,11 +510,21 @@ int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, g
     GBX_TRY(axpy_into_f64(1.0 / scale, e.s, x[0], s));
     st.outer_iters++;
   }
-  // x_o = b_o + kappa DhopOE x_e; then the whole-lattice residual b - M x
-  GBX_TRY(xpay(u_hi[1], x[1], b[1], x[0], 0, 1.0, kappa, s));
-  GBX_TRY(xpay(u_hi[0], y.s, x[0], x[1], 0, 1.0, -kappa, s));  // (M x)_e
+  if (kind == OP_DWF) {
+    // x_o = Mooee^-1 (b_o + DhopOE x_e / 2); then b - D x with D = Mooee - Dhop / 2
+    GBX_TRY(xpay(u_hi[1], to.s, b[1], x[0], 0, 1.0, 0.5, s));
+    GBX_TRY(dwf_s5_launch(x[1], to.s, nullptr, dwf, 1, 0, 1.0, 0.0, s));
+    GBX_TRY(dhop_launch(u_hi[0], te.s, x[1], 0, solver_cfg(u_hi[0], x[1]), 0.0, 1.0, nullptr, s));
+    GBX_TRY(dwf_s5_launch(y.s, x[0], te.s, dwf, 0, 0, 1.0, -0.5, s));  // (D x)_e
+    GBX_TRY(dhop_launch(u_hi[1], to.s, x[0], 0, solver_cfg(u_hi[1], x[0]), 0.0, 1.0, nullptr, s));
+    GBX_TRY(dwf_s5_launch(to.s, x[1], to.s, dwf, 0, 0, 1.0, -0.5, s));  // (D x)_o
+  } else {
+    // x_o = b_o + kappa DhopOE x_e; then b - M x
+    GBX_TRY(xpay(u_hi[1], x[1], b[1], x[0], 0, 1.0, kappa, s));
+    GBX_TRY(xpay(u_hi[0], y.s, x[0], x[1], 0, 1.0, -kappa, s));   // (M x)_e
+    GBX_TRY(xpay(u_hi[1], to.s, x[1], x[0], 0, 1.0, -kappa, s));  // (M x)_o
+  }
   GBX_TRY(axpby_f64(1.0, b[0], -1.0, y.s, r.s, s));
-  GBX_TRY(xpay(u_hi[1], to.s, x[1], x[0], 0, 1.0, -kappa, s));  // (M x)_o
   GBX_TRY(axpby_f64(1.0, b[1], -1.0, to.s, ro.s, s));
   double re2, ro2;
   GBX_TRY(spinor_norm2_on(r.s, &re2, s));
@@ -493,7 +535,7 @@ int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, g
   st.seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
   if (stats) *stats = st;
   if (rel > prm->tol)
-    return gbx_fail(GBX_ENOCONV, "cg_eo: relative residual %.3e after %d outer iterations", rel, st.outer_iters);
+    return gbx_fail(GBX_ENOCONV, "%s: relative residual %.3e after %d outer iterations", who, rel, st.outer_iters);
   return GBX_OK;
 }
 
@@ -502,7 +544,17 @@ int solve_eo(const gbx_gauge_s *const u_hi[2], const gbx_gauge_s *const *u_lo, g
 extern "C" int gbx_cg_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x[2], const gbx_spinor b[2],
                                const gbx_cg_params *p, gbx_cg_stats *st)
 {
-  return solve_eo(u_hi, u_lo, x, b, p, st);
+  return solve_schur(OP_WILSON, nullptr, u_hi, u_lo, x, b, p, st);
+}
+
+extern "C" int gbx_dwf_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x[2], const gbx_spinor b[2],
+                                const gbx_dwf_params *dwf, const gbx_cg_params *p, gbx_cg_stats *st)
+{
+  GBX_CHECK(dwf, GBX_ENULL, "dwf_eo: NULL DWF parameters");
+  GBX_CHECK(x && x[0], GBX_ENULL, "dwf_eo: NULL argument");
+  GBX_CHECK(5.0 - dwf->M5 > 0.0 && x[0]->Ls >= 2 && x[0]->Ls <= 256, GBX_EINVAL, "dwf_eo: M5 = %g, Ls = %d", dwf->M5,
+            x[0]->Ls);
+  return solve_schur(OP_DWF, dwf, u_hi, u_lo, x, b, p, st);
 }
 
 extern "C" int gbx_cg_solve(gbx_gauge u_hi, gbx_gauge u_lo, gbx_spinor x, gbx_spinor b, const gbx_cg_params *p,

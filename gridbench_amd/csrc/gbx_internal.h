@@ -146,6 +146,11 @@ void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c);
 bool dhop_lookup_cfg(const std::string &sig, gbx_kcfg *c);
 int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in);
 
+// out = alpha * S x + beta * z with S = Mooee / Mooee^-1 (inv) or their
+// adjoints (dag), site-local in s (dwf.hip).  z may be null when beta == 0.
+int dwf_s5_launch(gbx_spinor_s *out, const gbx_spinor_s *x, const gbx_spinor_s *z, const gbx_dwf_params *p, int inv, int dag,
+                  double alpha, double beta, hipStream_t st);
+
 // Vector algebra on a given stream (blas.hip).
 int spinor_convert_scaled(gbx_spinor_s *dst, const gbx_spinor_s *src, double scale, hipStream_t st);
 int spinor_norm2_on(const gbx_spinor_s *x, double *n2, hipStream_t st);

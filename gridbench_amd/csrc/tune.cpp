@@ -174,7 +174,7 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
   gbx_kcfg bk = def;
   auto search = [&](const std::vector<gbx_kcfg> &list) {
     for (const gbx_kcfg &k : list) {
-      double us;
+      double us = 0.0;
       if (time_cfg(u, out, in, dag, k, nrep, &us) != GBX_OK) {
         gbx_clear_error();
         continue;

@@ -12,7 +12,7 @@ import ctypes as C
 
 import numpy as np
 
-from ._lib import (lib, check, ptr, ivec, KCfg, CGParams, CGStats, GbxError, PREC_BY_NAME, PREC_NAMES,
+from ._lib import (lib, check, ptr, ivec, KCfg, CGParams, CGStats, DwfParams, GbxError, PREC_BY_NAME, PREC_NAMES,
                    F64, F32, F16, BF16)
 
 FLOPS_PER_SITE = 1320  # bench.cc:145, per 5D site and application
@@ -376,6 +376,33 @@ def cg_solve_eo(U_hi, U_lo, x, b, kappa, tol=1e-10, inner_tol=1e-3, max_outer=50
     return st.as_dict()
 
 
+# Domain-wall fermions: the reference's DomainWallFermion parameters
+# (TableGenerate.cc:47-50).
+DWF_M5, DWF_MASS = 1.0, 0.1
+
+
+def dwf_mooee(out, inp, M5=DWF_M5, mass=DWF_MASS, inv=False, dag=False):
+    """out = Mooee inp (or Mooee^-1, or their adjoints): the DWF fifth-dimension term."""
+    check(lib.gbx_dwf_mooee(out.h, inp.h, C.byref(DwfParams(M5, mass)), int(bool(inv)), int(bool(dag))),
+          "gbx_dwf_mooee")
+
+
+def dwf_apply(U, out, inp, M5=DWF_M5, mass=DWF_MASS, dag=False):
+    """out = D inp, D = Mooee - Dhop / 2 (whole lattice)."""
+    check(lib.gbx_dwf_apply(U.h, out.h, inp.h, C.byref(DwfParams(M5, mass)), int(bool(dag))), "gbx_dwf_apply")
+
+
+def dwf_solve_eo(U_hi, U_lo, x, b, M5=DWF_M5, mass=DWF_MASS, tol=1e-10, inner_tol=1e-3, max_outer=50,
+                 max_inner=5000):
+    """D x = b on a red-black pair by the even-odd Schur complement."""
+    p = CGParams(0.0, tol, inner_tol, max_outer, max_inner)
+    st = CGStats()
+    arr = lambda objs: (C.c_void_p * 2)(*[o.h for o in objs])
+    check(lib.gbx_dwf_solve_eo(arr(U_hi), arr(U_lo) if U_lo is not None else None, arr(x), arr(b),
+                               C.byref(DwfParams(M5, mass)), C.byref(p), C.byref(st)), "gbx_dwf_solve_eo")
+    return st.as_dict()
+
+
 def rb_split(dims, lex):
     """Whole-lattice field [..., V, ...] (site axis -1 of the leading shape is
     the lexicographic site) -> (even, odd) halves in checkerboarded
@@ -470,5 +497,5 @@ __all__ = [
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "dhop", "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
-    "cg_solve_eo", "rb_split", "rb_merge",
+    "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
 ]

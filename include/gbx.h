@@ -286,6 +286,38 @@ int gbx_cg_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x
                     const gbx_cg_params *p, gbx_cg_stats *st);
 
 /* ---------------------------------------------------------------------- */
+/* Domain-wall fermions: the fifth-dimension terms around the hop.  The    */
+/* reference builds a DomainWallFermion (mass 0.1, M5 1.0,                 */
+/* TableGenerate.cc:47-50) and benchmarks only its hopping term; these are */
+/* the Shamir terms of that operator, on fields whose Ls slices are s:     */
+/*   Mooee psi(s) = (5 - M5) psi(s) - P_- psi(s+1) - P_+ psi(s-1)          */
+/*   with the walls  -P_- psi(Ls) -> +mass P_- psi(0)                       */
+/*              and  -P_+ psi(-1) -> +mass P_+ psi(Ls-1),                   */
+/*   P_+- = (1 +- gamma5)/2, gamma5 = diag(1,1,-1,-1) (spins 0,1 are P_+);  */
+/*   D = Mooee - (1/2) Dhop.                                                */
+/* On a red-black pair Mee = Moo = Mooee, Meo = -(1/2) DhopEO and          */
+/* Moe = -(1/2) DhopOE.  Requires M5 < 5 and 2 <= Ls <= 256.               */
+/* ---------------------------------------------------------------------- */
+typedef struct {
+  double M5;
+  double mass;
+} gbx_dwf_params;
+/* out = Mooee in (inv = 0) or Mooee^-1 in (inv = 1, closed form); dag = 1 */
+/* for the adjoints.  Site-local: any geometry, any precision, out may be  */
+/* in itself.                                                              */
+int gbx_dwf_mooee(gbx_spinor out, gbx_spinor in, const gbx_dwf_params *p, int inv, int dag);
+/* out = D in (dag: D^dag = Mooee^dag - (1/2) Dhop^dag) on a whole lattice. */
+int gbx_dwf_apply(gbx_gauge u, gbx_spinor out, gbx_spinor in, const gbx_dwf_params *p, int dag);
+/* D x = b on a red-black pair by the Schur complement on the even sites:  */
+/*   b'_e = b_e + (1/2) DhopEO Mooee^-1 b_o                                 */
+/*   (Mooee - (1/4) DhopEO Mooee^-1 DhopOE) x_e = b'_e   (defect-correction */
+/*       CGNR as gbx_cg_solve_eo; p->kappa is not used)                     */
+/*   x_o = Mooee^-1 (b_o + (1/2) DhopOE x_e)                                */
+/* st->true_resid = |b - D x| / |b| over the whole lattice.                */
+int gbx_dwf_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x[2], const gbx_spinor b[2],
+                     const gbx_dwf_params *dwf, const gbx_cg_params *p, gbx_cg_stats *st);
+
+/* ---------------------------------------------------------------------- */
 /* Batched application over many independent lattices (each its own      */
 /* geometry, gauge field, Ls and nrhs; FP32/FP16/BF16 storage) in one      */
 /* persistent launch captured in a HIP graph.                              */
