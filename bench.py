@@ -23,6 +23,8 @@ Also reported, each with its own seeds:
              solves in FP32/BF16/FP16 (and FP64 only, for comparison); and
              M x = b by the even-odd Schur complement on a red-black pair
              against CG on the normal equations over the whole lattice.
+  dwf        24^4 x Ls16 domain-wall operator (mass 0.1, M5 1.0): D x = b
+             by the even-odd Schur complement, FP32 / FP16 inner solves.
   batch      40 lattices L^4 x Ls x Nrhs (L in {8,12,16,20,24}, Ls in
              {8,12,16}, Nrhs in 1..8, drawn with seed 505), FP32, one
              persistent graph-captured launch; aggregate GFLOP/s and the
@@ -316,6 +318,37 @@ def cg_eo_section(gbx, tune):
             "speedup_even_odd_f32_vs_normal_f32": round(full["seconds"] / eo32["seconds"], 2)}
 
 
+def dwf_section(gbx, tune):
+    """24^4 x Ls16 domain-wall operator D = Mooee - Dhop / 2 (the reference's
+    DomainWallFermion parameters mass 0.1, M5 1.0): D x = b by the even-odd
+    Schur complement, FP64 outer, FP32 / FP16 inner."""
+    Ls = LS
+    dbl = gbx.gen_doubled_gauge(DIMS, GAUGE_SEED)
+    b_lex = gbx.gen_spinor(DIMS, Ls, 1, 501)
+    be, bo = gbx.rb_split(DIMS, b_lex)
+    del b_lex
+    ge, go = gbx.Geometry.rb_pair(DIMS)
+    U = (gbx.GaugeField(ge, "f64").upload(dbl), gbx.GaugeField(go, "f64").upload(dbl))
+    b = (gbx.SpinorField(ge, Ls, 1, "f64").upload(be), gbx.SpinorField(go, Ls, 1, "f64").upload(bo))
+    x = (gbx.SpinorField(ge, Ls, 1, "f64"), gbx.SpinorField(go, Ls, 1, "f64"))
+    tune(U[0], x[0], x[1])
+    tune(U[1], x[1], x[0])
+    rows = []
+    for lo in ("f32", "f16"):
+        low = (gbx.GaugeField(ge, lo).upload(dbl), gbx.GaugeField(go, lo).upload(dbl))
+        ts = (gbx.SpinorField(ge, Ls, 1, lo), gbx.SpinorField(go, Ls, 1, lo))
+        tune(low[0], ts[0], ts[1])
+        tune(low[1], ts[1], ts[0])
+        for f in x:
+            f.zero()
+        st = gbx.dwf_solve_eo(U, low, x, b, tol=1e-10, max_outer=60)
+        r = {"inner": lo, **st, "converged": st["true_resid"] <= 1e-10}
+        log(f"dwf_eo {r}")
+        rows.append(r)
+    return {"dims": list(DIMS), "Ls": Ls, "M5": gbx.DWF_M5, "mass": gbx.DWF_MASS, "gauge_seed": GAUGE_SEED,
+            "b_seed": 501, "system": "D x = b, D = Mooee - Dhop / 2 (Shamir)", "even_odd": rows}
+
+
 def batch_section(gbx, nrep):
     """40 uneven lattices in one persistent graph-captured launch, FP32."""
     rng = np.random.default_rng(505)
@@ -432,6 +465,7 @@ def main():
     if not args.no_cg:
         extra["cg"] = cg_section(gbx, tune)
         extra["cg_even_odd"] = cg_eo_section(gbx, tune)
+        extra["dwf"] = dwf_section(gbx, tune)
     if not args.no_batch:
         extra["batch"] = batch_section(gbx, 20)
 
