@@ -127,6 +127,10 @@ PROTOTYPES = {
     "gbx_cg_solve_eo": (i32, [C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(CGParams),
                               C.POINTER(CGStats)]),
     "gbx_dwf_mooee": (i32, [vp, vp, C.POINTER(DwfParams), i32, i32]),
+    "gbx_static_data_write": (i32, [C.c_char_p, C.c_char_p, u64, i32, i32, pdbl, pdbl, pdbl, pu64, pu8]),
+    "gbx_static_data_read_header": (i32, [C.c_char_p, pu64, pi32]),
+    "gbx_static_data_sizes": (i32, [C.POINTER(C.c_char_p), i32, pu64]),
+    "gbx_static_data_read": (i32, [C.POINTER(C.c_char_p), i32, pdbl, pdbl, pdbl, pu64, pu8, pu64]),
     "gbx_dwf_apply": (i32, [vp, vp, vp, C.POINTER(DwfParams), i32]),
     "gbx_dwf_solve_eo": (i32, [C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), C.POINTER(DwfParams),
                                C.POINTER(CGParams), C.POINTER(CGStats)]),

@@ -403,6 +403,39 @@ def dwf_solve_eo(U_hi, U_lo, x, b, M5=DWF_M5, mass=DWF_MASS, tol=1e-10, inner_to
     return st.as_dict()
 
 
+# --------------------------------------------------- reference static_data
+def static_data_write(cc_path, h_path, nsite, Ls, nsimd, U, Phi, Psi_cpp, nbr, prm):
+    """Write static_data.cc / .h in the text format of TableGenerate.cc:93-138."""
+    a = [np.ascontiguousarray(v, dtype=np.float64) for v in (U, Phi, Psi_cpp)]
+    n = np.ascontiguousarray(nbr, dtype=np.uint64)
+    p = np.ascontiguousarray(prm, dtype=np.uint8)
+    check(lib.gbx_static_data_write(str(cc_path).encode(), str(h_path).encode(), int(nsite), int(Ls), int(nsimd),
+                                    ptr(a[0], C.c_double), ptr(a[1], C.c_double), ptr(a[2], C.c_double),
+                                    ptr(n, C.c_uint64), ptr(p, C.c_uint8)), "gbx_static_data_write")
+
+
+def static_data_read_header(h_path):
+    ns, ls = C.c_uint64(), C.c_int()
+    check(lib.gbx_static_data_read_header(str(h_path).encode(), C.byref(ns), C.byref(ls)),
+          "gbx_static_data_read_header")
+    return int(ns.value), int(ls.value)
+
+
+def static_data_read(paths):
+    """Arrays of a static_data blob (one .cc file or several): dict U, Phi, Psi_cpp, nbr, prm."""
+    if isinstance(paths, (str, bytes)) or hasattr(paths, "__fspath__"):
+        paths = [paths]
+    arr = (C.c_char_p * len(paths))(*[str(p).encode() for p in paths])
+    sizes = np.zeros(5, dtype=np.uint64)
+    check(lib.gbx_static_data_sizes(arr, len(paths), ptr(sizes, C.c_uint64)), "gbx_static_data_sizes")
+    out = {"U": np.empty(int(sizes[0])), "Phi": np.empty(int(sizes[1])), "Psi_cpp": np.empty(int(sizes[2])),
+           "nbr": np.empty(int(sizes[3]), dtype=np.uint64), "prm": np.empty(int(sizes[4]), dtype=np.uint8)}
+    check(lib.gbx_static_data_read(arr, len(paths), ptr(out["U"], C.c_double), ptr(out["Phi"], C.c_double),
+                                   ptr(out["Psi_cpp"], C.c_double), ptr(out["nbr"], C.c_uint64),
+                                   ptr(out["prm"], C.c_uint8), ptr(sizes, C.c_uint64)), "gbx_static_data_read")
+    return out
+
+
 def rb_split(dims, lex):
     """Whole-lattice field [..., V, ...] (site axis -1 of the leading shape is
     the lexicographic site) -> (even, odd) halves in checkerboarded
@@ -498,4 +531,5 @@ __all__ = [
     "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "dhop", "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
+    "static_data_write", "static_data_read_header", "static_data_read",
 ]

@@ -286,6 +286,28 @@ int gbx_cg_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor x
                     const gbx_cg_params *p, gbx_cg_stats *st);
 
 /* ---------------------------------------------------------------------- */
+/* The reference's static_data blobs (TableGenerate.cc:93-138 writes them; */
+/* bench.cc:25-56 includes the header and links the arrays; the blobs are  */
+/* not in the repository, .MISSING_LARGE_BLOBS).  U_static is the doubled  */
+/* gauge field, Phi_static the source, Psi_cpp_static Grid's Dhop of it,   */
+/* all in the reference host layout for nsimd lanes (bench.cc:105-138),    */
+/* plus nbr_static / prm_static in the table form of gbx_table_generate.   */
+/* ---------------------------------------------------------------------- */
+/* Write static_data.cc / static_data.h in TableGenerate's exact text      */
+/* format (%16.8le values, hex tables, a trailing 0 element per array).    */
+/* Arrays: U nsite*144*nsimd, Phi and Psi_cpp nsite*24*Ls*nsimd reals, nbr */
+/* and prm nsite*Ls*8 entries.                                             */
+int gbx_static_data_write(const char *cc_path, const char *h_path, uint64_t nsite, int Ls, int nsimd, const double *U,
+                          const double *Phi, const double *Psi_cpp, const uint64_t *nbr, const uint8_t *prm);
+int gbx_static_data_read_header(const char *h_path, uint64_t *nsite, int *Ls);
+/* Element counts {U, Phi, Psi_cpp, nbr, prm} of the arrays defined in the */
+/* given files (one static_data.cc, or a split gauge / fermion pair).      */
+int gbx_static_data_sizes(const char *const *paths, int npaths, uint64_t sizes[5]);
+/* Read the arrays into caller buffers of exactly those counts. */
+int gbx_static_data_read(const char *const *paths, int npaths, double *U, double *Phi, double *Psi_cpp, uint64_t *nbr,
+                         uint8_t *prm, const uint64_t sizes[5]);
+
+/* ---------------------------------------------------------------------- */
 /* Domain-wall fermions: the fifth-dimension terms around the hop.  The    */
 /* reference builds a DomainWallFermion (mass 0.1, M5 1.0,                 */
 /* TableGenerate.cc:47-50) and benchmarks only its hopping term; these are */
