@@ -76,6 +76,10 @@ PROTOTYPES = {
     "gbx_device_count": (i32, []),
     "gbx_dslash_kernel_f64": (i32, [i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
     "gbx_dslash_kernel_f32": (i32, [i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_kernel_rrii_f64": (i32, [i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_kernel_rrii_f32": (i32, [i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_kernel_expand_f64": (i32, [i32, i32, i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_kernel_expand_f32": (i32, [i32, i32, i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
     "gbx_dslash_last_path": (i32, []),
     "gbx_table_generate": (i32, [pi32, i32, u64, pu64, pu8]),
     "gbx_table_recognise": (i32, [i32, u64, u64, pu64, pu8, pi32]),

@@ -8,8 +8,12 @@
 //    repacked into the SoA layout and the fast kernel (dhop.hip) runs;
 //  * any other valid tables: a table-driven kernel on the reference layout
 //    honours nbr/prm exactly: thread = (5D outer site, lane), neighbour lane
-//    = lane ^ (Nsimd >> (ptype+1)) when prm is set (the reference GPU
+//    = lane ^ (DATA_SIMD >> (ptype+1)) when prm is set (the reference GPU
 //    kernel's gather form, WilsonKernelsHandGpu.h:33-41).
+// Host arrays may be RIRI (re,im per lane) or RRII (all real parts, then
+// all imaginary parts, per vector; bench.cc:105-109), and vectors may be
+// expanded: nsimd lanes holding nsimd/dsimd replicas of a dsimd-lane lattice
+// (bench.cc:1-3, 98-138), run as independent problems.
 // Timing follows the reference: nrep applications, the clock covering
 // applications 1..nrep-1 (WilsonKernelsHandCpu.h:407-450), measured here
 // with HIP events on the stream the kernels run on.
@@ -26,10 +30,22 @@ namespace {
 
 thread_local int g_last_path = 0;
 
-template <typename T, bool DAG>
+// Real ri of lane `lane` of complex element c of `width`-lane vectors: RIRI
+// [c][lane][ri] or RRII [c][ri][lane] (bench.cc:105-109).
+template <bool RRII>
+__device__ __forceinline__ uint64_t rpos(uint64_t c, int lane, int ri, int width)
+{
+  return c * 2 * (uint64_t)width + (RRII ? (uint64_t)ri * width + lane : (uint64_t)lane * 2 + ri);
+}
+
+// Thread = (5D outer site, lane) over `width`-lane vectors.  The permute
+// mask comes from the data SIMD width dsimd <= width, so an expanded vector
+// is width/dsimd independent copies of the dsimd-lane problem
+// (Simd_riri_vec.h:466, "keep the permutes fixed as SIMD expanded").
+template <typename T, bool DAG, bool RRII>
 __global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *__restrict__ out, const T *__restrict__ in,
                                                     const uint64_t *__restrict__ nbr, const uint8_t *__restrict__ prm,
-                                                    uint64_t nsite, uint64_t Ls, int nsimd)
+                                                    uint64_t nsite, uint64_t Ls, int nsimd, int dsimd)
 {
   const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
   if (t >= nsite * Ls * (uint64_t)nsimd) return;
@@ -41,20 +57,18 @@ __global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *
 #define GBX_TLEG(DIR, PTYPE, FIRST)                                                                                      \
   {                                                                                                                      \
     const uint64_t off = nbr[ss * 8 + DIR];                                                                              \
-    const int plane = prm[ss * 8 + DIR] ? (lane ^ (nsimd >> (PTYPE + 1))) : lane;                                        \
+    const int plane = prm[ss * 8 + DIR] ? (lane ^ (dsimd >> (PTYPE + 1))) : lane;                                        \
     T v[24];                                                                                                             \
     _Pragma("unroll") for (int sc = 0; sc < 12; sc++)                                                                    \
     {                                                                                                                    \
-      const T *p = in + ((off * 12 + sc) * (uint64_t)nsimd + plane) * 2;                                                 \
-      v[2 * sc] = p[0];                                                                                                  \
-      v[2 * sc + 1] = p[1];                                                                                              \
+      v[2 * sc] = in[rpos<RRII>(off * 12 + sc, plane, 0, nsimd)];                                                        \
+      v[2 * sc + 1] = in[rpos<RRII>(off * 12 + sc, plane, 1, nsimd)];                                                    \
     }                                                                                                                    \
     T Ul[18];                                                                                                            \
     _Pragma("unroll") for (int e = 0; e < 9; e++)                                                                        \
     {                                                                                                                    \
-      const T *p = U + (((sU * 8 + DIR) * 9 + e) * (uint64_t)nsimd + lane) * 2;                                          \
-      Ul[2 * e] = p[0];                                                                                                  \
-      Ul[2 * e + 1] = p[1];                                                                                              \
+      Ul[2 * e] = U[rpos<RRII>((sU * 8 + DIR) * 9 + e, lane, 0, nsimd)];                                                 \
+      Ul[2 * e + 1] = U[rpos<RRII>((sU * 8 + DIR) * 9 + e, lane, 1, nsimd)];                                             \
     }                                                                                                                    \
     T hr[6], hi[6], ur[6], ui[6];                                                                                        \
     dev::project<dev::LegProj<DAG, DIR>::value>(v, hr, hi);                                                              \
@@ -72,9 +86,8 @@ __global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *
 #undef GBX_TLEG
 #pragma unroll
   for (int sc = 0; sc < 12; sc++) {
-    T *p = out + ((ss * 12 + sc) * (uint64_t)nsimd + lane) * 2;
-    p[0] = rr[sc];
-    p[1] = ri[sc];
+    out[rpos<RRII>(ss * 12 + sc, lane, 0, nsimd)] = rr[sc];
+    out[rpos<RRII>(ss * 12 + sc, lane, 1, nsimd)] = ri[sc];
   }
 }
 
@@ -131,18 +144,12 @@ int fast_objects(const int dims[4], int Ls, int prec)
   return GBX_OK;
 }
 
-template <typename T> struct RefIO;
-template <> struct RefIO<double> {
+template <typename T> struct RefPrec;
+template <> struct RefPrec<double> {
   static constexpr int prec = GBX_F64;
-  static int up_g(gbx_gauge u, const double *p, int n) { return gbx_gauge_upload_ref_f64(u, p, n); }
-  static int up_s(gbx_spinor x, const double *p, int n) { return gbx_spinor_upload_ref_f64(x, p, n); }
-  static int down_s(gbx_spinor x, double *p, int n) { return gbx_spinor_download_ref_f64(x, p, n); }
 };
-template <> struct RefIO<float> {
+template <> struct RefPrec<float> {
   static constexpr int prec = GBX_F32;
-  static int up_g(gbx_gauge u, const float *p, int n) { return gbx_gauge_upload_ref_f32(u, p, n); }
-  static int up_s(gbx_spinor x, const float *p, int n) { return gbx_spinor_upload_ref_f32(x, p, n); }
-  static int down_s(gbx_spinor x, float *p, int n) { return gbx_spinor_download_ref_f32(x, p, n); }
 };
 
 // Time nrep launches of `launch` on stream st, reference convention.
@@ -167,16 +174,22 @@ template <typename F> int timed_reps(int nrep, hipStream_t st, F launch, double 
   return rc;
 }
 
+bool pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+// nsimd = lanes per host vector (EXPAND_SIMD), dsimd = lanes the tables and
+// permutes are for (DATA_SIMD, bench.cc:1-3); rrii = the RRII lane layout.
 template <typename T>
-int dslash_kernel_impl(int nrep, int nsimd, int dag, const T *U, T *out, const T *in, const uint64_t *nbr, uint64_t nsite,
-                       uint64_t Ls, const uint8_t *prm, double *usec)
+int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const T *U, T *out, const T *in,
+                       const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
   g_last_path = 0;
   GBX_CHECK(U && out && in && nbr && prm, GBX_ENULL, "dslash_kernel: NULL array");
   GBX_CHECK(nrep >= 1, GBX_EINVAL, "dslash_kernel: nrep=%d", nrep);
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dslash_kernel: dag=%d", dag);
-  GBX_CHECK(nsimd == 1 || nsimd == 2 || nsimd == 4 || nsimd == 8 || nsimd == 16, GBX_EINVAL,
-            "dslash_kernel: nsimd=%d not in {1,2,4,8,16}", nsimd);
+  GBX_CHECK(rrii == 0 || rrii == 1, GBX_EINVAL, "dslash_kernel: rrii=%d", rrii);
+  GBX_CHECK(pow2(dsimd) && dsimd <= 16, GBX_EINVAL, "dslash_kernel: data nsimd=%d not in {1,2,4,8,16}", dsimd);
+  GBX_CHECK(pow2(nsimd) && nsimd >= dsimd && nsimd <= 64, GBX_EINVAL,
+            "dslash_kernel: vector nsimd=%d must be a power of two in [%d, 64]", nsimd, dsimd);
   GBX_CHECK(nsite >= 1 && Ls >= 1 && Ls <= 4096 && nsite * Ls <= (1ull << 34), GBX_EINVAL,
             "dslash_kernel: nsite=%llu Ls=%llu", (unsigned long long)nsite, (unsigned long long)Ls);
   const uint64_t n5 = nsite * Ls;
@@ -192,27 +205,41 @@ int dslash_kernel_impl(int nrep, int nsimd, int dag, const T *U, T *out, const T
             (unsigned long long)n5);
   GBX_TRY(gbx_require_device());
   std::lock_guard<std::mutex> lk(g_cache.mu);
-  int dims[4];
-  const int canon = gbx_table_recognise(nsimd, nsite, Ls, nbr, prm, dims);
-  gbx_clear_error();
-  if (canon == GBX_OK) {
-    GBX_TRY(fast_objects(dims, (int)Ls, RefIO<T>::prec));
-    DropinCache &c = g_cache;
-    GBX_TRY(RefIO<T>::up_g(c.gauge, U, nsimd));
-    GBX_TRY(RefIO<T>::up_s(c.sin, in, nsimd));
-    gbx_kcfg k;
-    gbx_dhop_get_cfg(c.gauge, c.sin, &k);
-    hipStream_t st = c.geom->stream;
-    GBX_TRY(timed_reps(nrep, st, [&]() { return dhop_run(c.gauge, c.sout, c.sin, dag, k, st); }, usec));
-    GBX_TRY(RefIO<T>::down_s(c.sout, out, nsimd));
-    g_last_path = 1;
-    return GBX_OK;
-  }
-  // Generic table-driven path.
   DropinCache &c = g_cache;
   GBX_TRY(ensure(&c.dU, &c.nU, nu));
   GBX_TRY(ensure(&c.dIn, &c.nIn, nf));
   GBX_TRY(ensure(&c.dOut, &c.nOut, nf));
+  int dims[4];
+  const int canon = gbx_table_recognise(dsimd, nsite, Ls, nbr, prm, dims);
+  gbx_clear_error();
+  if (canon == GBX_OK) {
+    // The SoA kernel, once per replica of the dsimd-lane lattice.
+    GBX_TRY(fast_objects(dims, (int)Ls, RefPrec<T>::prec));
+    hipStream_t st = c.geom->stream;
+    GBX_HIP(hipMemcpyAsync(c.dU, U, nu, hipMemcpyHostToDevice, st));
+    GBX_HIP(hipMemcpyAsync(c.dIn, in, nf, hipMemcpyHostToDevice, st));
+    gbx_kcfg k;
+    gbx_dhop_get_cfg(c.gauge, c.sin, &k);
+    double total = 0.0;
+    for (int nn = 0; nn < nsimd / dsimd; nn++) {
+      gbx_ref_layout lay;
+      lay.width = nsimd;
+      lay.lane0 = nn * dsimd;
+      lay.rrii = rrii;
+      GBX_TRY(ref_pack_gauge_dev(c.gauge, (const T *)c.dU, dsimd, lay));
+      GBX_TRY(ref_pack_spinor_dev(c.sin, (const T *)c.dIn, dsimd, lay));
+      double us = 0.0;
+      GBX_TRY(timed_reps(nrep, st, [&]() { return dhop_run(c.gauge, c.sout, c.sin, dag, k, st); }, &us));
+      total += us;
+      GBX_TRY(ref_unpack_spinor_dev(c.sout, (T *)c.dOut, dsimd, lay));
+    }
+    GBX_HIP(hipMemcpyAsync(out, c.dOut, nf, hipMemcpyDeviceToHost, st));
+    GBX_HIP(hipStreamSynchronize(st));
+    if (usec) *usec = total;
+    g_last_path = 1;
+    return GBX_OK;
+  }
+  // Generic table-driven path.
   GBX_TRY(ensure(&c.dNbr, &c.nNbr, nt * 8));
   GBX_TRY(ensure(&c.dPrm, &c.nPrm, nt));
   hipStream_t st = nullptr;
@@ -223,12 +250,14 @@ int dslash_kernel_impl(int nrep, int nsimd, int dag, const T *U, T *out, const T
   const uint64_t nthr = n5 * (uint64_t)nsimd;
   const dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
   auto launch = [&]() -> int {
-    if (dag)
-      hipLaunchKernelGGL((k_dhop_table<T, true>), grid, block, 0, st, (const T *)c.dU, (T *)c.dOut, (const T *)c.dIn,
-                         (const uint64_t *)c.dNbr, (const uint8_t *)c.dPrm, nsite, Ls, nsimd);
-    else
-      hipLaunchKernelGGL((k_dhop_table<T, false>), grid, block, 0, st, (const T *)c.dU, (T *)c.dOut, (const T *)c.dIn,
-                         (const uint64_t *)c.dNbr, (const uint8_t *)c.dPrm, nsite, Ls, nsimd);
+#define GBX_TABLE(D, R)                                                                                                   \
+  hipLaunchKernelGGL((k_dhop_table<T, D, R>), grid, block, 0, st, (const T *)c.dU, (T *)c.dOut, (const T *)c.dIn,         \
+                     (const uint64_t *)c.dNbr, (const uint8_t *)c.dPrm, nsite, Ls, nsimd, dsimd)
+    if (dag && rrii) GBX_TABLE(true, true);
+    else if (dag) GBX_TABLE(true, false);
+    else if (rrii) GBX_TABLE(false, true);
+    else GBX_TABLE(false, false);
+#undef GBX_TABLE
     GBX_HIP(hipGetLastError());
     return GBX_OK;
   };
@@ -243,13 +272,39 @@ int dslash_kernel_impl(int nrep, int nsimd, int dag, const T *U, T *out, const T
 extern "C" int gbx_dslash_kernel_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
                                      const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<double>(nrep, nsimd, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return dslash_kernel_impl<double>(nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
                                      const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<float>(nrep, nsimd, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return dslash_kernel_impl<float>(nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_kernel_rrii_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
+                                          const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<double>(nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_kernel_rrii_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
+                                          const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<float>(nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_kernel_expand_f64(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const double *U,
+                                            double *out, const double *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
+                                            const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<double>(nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_kernel_expand_f32(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const float *U,
+                                            float *out, const float *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
+                                            const uint8_t *prm, double *usec)
+{
+  return dslash_kernel_impl<float>(nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_last_path(void) { return g_last_path; }

@@ -121,6 +121,21 @@ void geom_release(gbx_geom_s *g);
 // stream; freed with the library's cached allocations at exit).
 int staging_buffer(size_t bytes, void **ptr);
 
+// Lane placement of a reference-layout array: `width` lanes per vector (0 =
+// the lattice's own nsimd), the lattice's lanes starting at lane0, RIRI or
+// RRII order of the real and imaginary parts.
+struct gbx_ref_layout {
+  int width = 0, lane0 = 0, rrii = 0;
+};
+// Reference-layout arrays in device memory <-> device objects, enqueued on
+// the geometry's stream (whole lattices only).
+int ref_pack_gauge_dev(gbx_gauge u, const double *d, int nsimd, const gbx_ref_layout &l);
+int ref_pack_gauge_dev(gbx_gauge u, const float *d, int nsimd, const gbx_ref_layout &l);
+int ref_pack_spinor_dev(gbx_spinor x, const double *d, int nsimd, const gbx_ref_layout &l);
+int ref_pack_spinor_dev(gbx_spinor x, const float *d, int nsimd, const gbx_ref_layout &l);
+int ref_unpack_spinor_dev(gbx_spinor x, double *d, int nsimd, const gbx_ref_layout &l);
+int ref_unpack_spinor_dev(gbx_spinor x, float *d, int nsimd, const gbx_ref_layout &l);
+
 // Kernel launch configuration of the VALU hopping kernel.
 struct dhop_cfg {
   int spw;       // 4D sites per workgroup (links of these sites staged in LDS)

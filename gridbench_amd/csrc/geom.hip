@@ -77,11 +77,24 @@ __global__ void k_unpack_lex(const void *__restrict__ srcd, double *__restrict__
   }
 }
 
-// Reference virtual-node host layout [nsite*Ls][12][nsimd][2] <-> device.
+// Reference virtual-node host layout [nsite*Ls][12][width][2] <-> device.
+// The lattice fills nsimd lanes (the data SIMD width); a vector holds `width`
+// >= nsimd lanes, of which lanes lane0 .. lane0+nsimd-1 are this lattice's
+// (bench.cc:98-138 expands DATA_SIMD lanes to EXPAND_SIMD this way, replica
+// nn at lanes nn*DATA_SIMD + n).
 struct RefGeom {
   int dims[4], simd[4], rd[4];
   int nsimd;
+  int width, lane0;
+  int rrii;  // 0: RIRI [..][lane][re,im]; 1: RRII [..][re,im][lane] (bench.cc:105-109)
 };
+
+// Position of real `ri` of data lane `lane` of complex element `c`.
+__device__ __forceinline__ uint64_t ref_real(uint64_t c, uint32_t lane, uint32_t ri, const RefGeom &rg)
+{
+  const uint64_t w = (uint64_t)rg.width, l = (uint64_t)rg.lane0 + lane;
+  return c * 2 * w + (rg.rrii ? (uint64_t)ri * w + l : l * 2 + ri);
+}
 
 __device__ inline void ref_coords(const RefGeom &rg, uint32_t g, uint64_t &o, uint32_t &lane)
 {
@@ -118,7 +131,7 @@ __global__ void k_pack_ref(const T *__restrict__ ref, void *__restrict__ dst, co
 #pragma unroll
     for (int j = 0; j < R; j++) {
       const uint32_t ridx = k * R + j;
-      v[j] = (double)ref[(((o * Ls + s) * 12 + (ridx >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (ridx & 1)];
+      v[j] = (double)ref[ref_real((o * Ls + s) * 12 + (ridx >> 1), lane, ridx & 1, rg)];
     }
     dev::store_chunk<P>(dst, t, v, false);
   }
@@ -142,7 +155,7 @@ __global__ void k_unpack_ref(const void *__restrict__ srcd, T *__restrict__ ref,
 #pragma unroll
     for (int j = 0; j < R; j++) {
       const uint32_t ridx = k * R + j;
-      ref[(((o * Ls + s) * 12 + (ridx >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (ridx & 1)] = (T)v[j];
+      ref[ref_real((o * Ls + s) * 12 + (ridx >> 1), lane, ridx & 1, rg)] = (T)v[j];
     }
   }
 }
@@ -175,13 +188,18 @@ __global__ void k_pack_gauge_ref(const T *__restrict__ ref, void *__restrict__ d
     uint64_t o;
     uint32_t lane;
     ref_coords(rg, site_of_slot[q], o, lane);
-    dev::store_real<P>(dst, t, (double)ref[(((o * 8 + dir) * 9 + (e >> 1)) * (uint64_t)rg.nsimd + lane) * 2 + (e & 1)]);
+    dev::store_real<P>(dst, t, (double)ref[ref_real((o * 8 + dir) * 9 + (e >> 1), lane, e & 1, rg)]);
   }
 }
 
 // ------------------------------------------------------------- host helpers
-int ref_geom(const gbx_geom_s *g, int nsimd, RefGeom &rg)
+int ref_geom(const gbx_geom_s *g, int nsimd, RefGeom &rg, const gbx_ref_layout &lay = gbx_ref_layout())
 {
+  rg.rrii = lay.rrii;
+  rg.width = lay.width > 0 ? lay.width : nsimd;
+  rg.lane0 = lay.lane0;
+  GBX_CHECK(rg.width >= nsimd && rg.lane0 >= 0 && rg.lane0 + nsimd <= rg.width, GBX_EINVAL,
+            "reference layout: lanes %d..%d outside a %d-lane vector", rg.lane0, rg.lane0 + nsimd - 1, rg.width);
   GBX_CHECK(g->parity < 0, GBX_EUNSUPPORTED, "the reference's virtual-node layout describes a whole lattice, not a red-black half");
   rg.nsimd = nsimd;
   for (int d = 0; d < 4; d++) rg.simd[d] = 1;
@@ -776,3 +794,55 @@ extern "C" int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src)
   // convert point by point (all 24 reals of one (slot, m) per thread).
   return spinor_convert_scaled(dst, src, 1.0, g->stream);
 }
+
+// Reference-layout arrays already in device memory <-> device objects,
+// in any lane layout; enqueued on the geometry's stream, no synchronisation.
+template <typename T> static int gauge_pack_ref_dev(gbx_gauge u, const T *dref, int nsimd, const gbx_ref_layout &lay)
+{
+  GBX_CHECK(u && dref, GBX_ENULL, "gauge_pack_ref_dev: NULL argument");
+  gbx_geom_s *g = u->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg, lay));
+  const int Rl = u->comp12 ? 12 : 18;
+  const uint64_t total = g->vol * 8 * (uint64_t)Rl;
+  dispatch_prec<PackGaugeRefT<T>::template F>(u->prec, dref, u->d, (const uint32_t *)g->site_of_slot, total, Rl, rg, g->stream);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+template <typename T> static int spinor_pack_ref_dev(gbx_spinor x, const T *dref, int nsimd, const gbx_ref_layout &lay)
+{
+  GBX_TRY(spinor_ok(x));
+  GBX_CHECK(dref, GBX_ENULL, "spinor_pack_ref_dev: NULL array");
+  GBX_CHECK(x->nrhs == 1, GBX_EINVAL, "spinor_pack_ref_dev: reference layout holds one right-hand side");
+  gbx_geom_s *g = x->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg, lay));
+  const uint64_t total = g->vol * (uint64_t)spinor_nchunk(x->prec) * x->M;
+  dispatch_prec<PackRefT<T>::template F>(x->prec, dref, x->d, (const uint32_t *)g->site_of_slot, total, rg, (uint32_t)x->Ls,
+                                         g->stream);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+template <typename T> static int spinor_unpack_ref_dev(gbx_spinor x, T *dref, int nsimd, const gbx_ref_layout &lay)
+{
+  GBX_TRY(spinor_ok(x));
+  GBX_CHECK(dref, GBX_ENULL, "spinor_unpack_ref_dev: NULL array");
+  GBX_CHECK(x->nrhs == 1, GBX_EINVAL, "spinor_unpack_ref_dev: reference layout holds one right-hand side");
+  gbx_geom_s *g = x->geom;
+  RefGeom rg;
+  GBX_TRY(ref_geom(g, nsimd, rg, lay));
+  const uint64_t total = g->vol * (uint64_t)spinor_nchunk(x->prec) * x->M;
+  dispatch_prec<UnpackRefT<T>::template F>(x->prec, (const void *)x->d, dref, (const uint32_t *)g->site_of_slot, total, rg,
+                                           (uint32_t)x->Ls, g->stream);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+int ref_pack_gauge_dev(gbx_gauge u, const double *d, int nsimd, const gbx_ref_layout &l) { return gauge_pack_ref_dev(u, d, nsimd, l); }
+int ref_pack_gauge_dev(gbx_gauge u, const float *d, int nsimd, const gbx_ref_layout &l) { return gauge_pack_ref_dev(u, d, nsimd, l); }
+int ref_pack_spinor_dev(gbx_spinor x, const double *d, int nsimd, const gbx_ref_layout &l) { return spinor_pack_ref_dev(x, d, nsimd, l); }
+int ref_pack_spinor_dev(gbx_spinor x, const float *d, int nsimd, const gbx_ref_layout &l) { return spinor_pack_ref_dev(x, d, nsimd, l); }
+int ref_unpack_spinor_dev(gbx_spinor x, double *d, int nsimd, const gbx_ref_layout &l) { return spinor_unpack_ref_dev(x, d, nsimd, l); }
+int ref_unpack_spinor_dev(gbx_spinor x, float *d, int nsimd, const gbx_ref_layout &l) { return spinor_unpack_ref_dev(x, d, nsimd, l); }

@@ -83,16 +83,21 @@ def table_recognise(nsimd, nsite, Ls, nbr, prm):
 
 
 # ------------------------------------------------------------------- drop-in
-def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False):
+def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii=False, data_nsimd=None):
     """The reference entry dslash_kernel<Simd>(nrep, U, out, in, nbr, nsite, Ls, prm)
     (WilsonKernelsHand.h:17-29) on numpy arrays in the reference layout.
-    float64 arrays run the double kernel, float32 the single.  Returns the
-    microseconds of applications 1..nrep-1, like the reference."""
+    float64 arrays run the double kernel, float32 the single.  rrii=True
+    takes every array in the RRII lane layout (nsimd real parts, then nsimd
+    imaginary parts per vector, bench.cc:105-106) instead of RIRI.
+    data_nsimd < nsimd: vectors of nsimd lanes expanded from data_nsimd data
+    lanes, replica nn at lanes nn*data_nsimd + n, with the tables and
+    permutes of data_nsimd (bench.cc:1-3, 98-138).  Returns the microseconds
+    of applications 1..nrep-1, like the reference."""
     usec = C.c_double(0.0)
     if U.dtype == np.float64:
-        fn, ct = lib.gbx_dslash_kernel_f64, C.c_double
+        ct, sfx = C.c_double, "f64"
     elif U.dtype == np.float32:
-        fn, ct = lib.gbx_dslash_kernel_f32, C.c_float
+        ct, sfx = C.c_float, "f32"
     else:
         raise TypeError("U must be float64 or float32")
     for a in (out, in_):
@@ -100,8 +105,15 @@ def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False):
             raise TypeError("U, out, in must share a dtype")
     if nbr.dtype != np.uint64 or prm.dtype != np.uint8:
         raise TypeError("nbr must be uint64 and prm uint8")
-    check(fn(int(nrep), int(nsimd), int(bool(dag)), ptr(U, ct), ptr(out, ct), ptr(in_, ct), ptr(nbr, C.c_uint64),
-             int(nsite), int(Ls), ptr(prm, C.c_uint8), C.byref(usec)), "dslash_kernel")
+    args = (ptr(U, ct), ptr(out, ct), ptr(in_, ct), ptr(nbr, C.c_uint64), int(nsite), int(Ls), ptr(prm, C.c_uint8),
+            C.byref(usec))
+    if data_nsimd is not None and int(data_nsimd) != int(nsimd):
+        fn = getattr(lib, "gbx_dslash_kernel_expand_" + sfx)
+        rc = fn(int(nrep), int(nsimd), int(data_nsimd), int(bool(rrii)), int(bool(dag)), *args)
+    else:
+        fn = getattr(lib, ("gbx_dslash_kernel_rrii_" if rrii else "gbx_dslash_kernel_") + sfx)
+        rc = fn(int(nrep), int(nsimd), int(bool(dag)), *args)
+    check(rc, "dslash_kernel")
     return usec.value
 
 

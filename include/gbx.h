@@ -75,6 +75,26 @@ int gbx_dslash_kernel_f64(int nrep, int nsimd, int dag, const double *U, double 
                           const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
 int gbx_dslash_kernel_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
                           const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
+/* The same entry for host arrays in the reference's RRII layout: each     */
+/* vector of nsimd complex lanes stores its nsimd real parts, then its     */
+/* nsimd imaginary parts (bench.cc:105-106 under -DRRII; RIRI interleaves  */
+/* re,im per lane, bench.cc:108).  Tables and timing as above.             */
+int gbx_dslash_kernel_rrii_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
+                               const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
+int gbx_dslash_kernel_rrii_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
+                               const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec);
+/* The general form: host vectors of nsimd lanes (EXPAND_SIMD) holding     */
+/* nsimd/data_nsimd replicas of a data_nsimd-lane lattice (DATA_SIMD), the  */
+/* replica nn at lanes nn*data_nsimd + n (bench.cc:1-3, 98-138); tables and */
+/* permutes are those of data_nsimd, so each replica is an independent      */
+/* problem (Simd_riri_vec.h:466).  rrii selects the RRII lane layout.       */
+/* data_nsimd == nsimd is the plain entry above.                            */
+int gbx_dslash_kernel_expand_f64(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const double *U, double *out,
+                                 const double *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm,
+                                 double *usec);
+int gbx_dslash_kernel_expand_f32(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const float *U, float *out,
+                                 const float *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm,
+                                 double *usec);
 /* Which path the last gbx_dslash_kernel_* call of this thread took:
  * 1 = fast SoA kernel (tables recognised), 2 = table-driven kernel. */
 int gbx_dslash_last_path(void);

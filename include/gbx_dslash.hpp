@@ -16,6 +16,11 @@
 // 1..nrep-1 as the reference does; on failure returns a negative value and
 // leaves the library's message in gbx_last_error().  dslash_kernel_dag is
 // the dagger (the reference has none).
+//
+// The reference's compile-time layout switches are honoured as it defines
+// them: -DRRII selects the RRII lane layout (bench.cc:105-106), and
+// DATA_SIMD < Nsimd() means vectors expanded from DATA_SIMD data lanes
+// (bench.cc:1-3, 98-138) whose tables and permutes stay those of DATA_SIMD.
 #pragma once
 
 #include <cstdint>
@@ -42,14 +47,24 @@ inline double dslash_kernel_impl(int nrep, int dag, Simd *Up, Simd *outp, Simd *
   typedef typename Simd::scalar_type S;
   static_assert(sizeof(S) == 16 || sizeof(S) == 8, "scalar_type must be a double or float complex");
   const int nsimd = Simd::Nsimd();
+#ifdef DATA_SIMD
+  const int data_nsimd = (DATA_SIMD < nsimd) ? (int)(DATA_SIMD) : nsimd;
+#else
+  const int data_nsimd = nsimd;
+#endif
+#ifdef RRII
+  const int rrii = 1;
+#else
+  const int rrii = 0;
+#endif
   double usec = 0.0;
   int rc;
   if (sizeof(S) == 16)
-    rc = gbx_dslash_kernel_f64(nrep, nsimd, dag, (const double *)Up, (double *)outp, (const double *)inp, nbr, nsite, Ls,
-                               prm, &usec);
+    rc = gbx_dslash_kernel_expand_f64(nrep, nsimd, data_nsimd, rrii, dag, (const double *)Up, (double *)outp,
+                                      (const double *)inp, nbr, nsite, Ls, prm, &usec);
   else
-    rc = gbx_dslash_kernel_f32(nrep, nsimd, dag, (const float *)Up, (float *)outp, (const float *)inp, nbr, nsite, Ls, prm,
-                               &usec);
+    rc = gbx_dslash_kernel_expand_f32(nrep, nsimd, data_nsimd, rrii, dag, (const float *)Up, (float *)outp,
+                                      (const float *)inp, nbr, nsite, Ls, prm, &usec);
   if (rc != GBX_OK) {
     std::fprintf(stderr, "dslash_kernel: %s: %s\n", gbx_strerror(rc), gbx_last_error());
     return -(double)rc;

@@ -144,6 +144,31 @@ def gamma5(ref, nsimd):
     return aligned_copy(a.reshape(-1))
 
 
+def to_rrii(ref, width):
+    """RIRI -> RRII lane layout of a reference-layout array of `width`-lane
+    vectors: each vector's reals first, then its imaginary parts (bench.cc:105-108)."""
+    a = np.ascontiguousarray(ref).reshape(-1, width, 2)
+    return aligned_copy(a.transpose(0, 2, 1).reshape(-1))
+
+
+def from_rrii(ref, width):
+    a = np.ascontiguousarray(ref).reshape(-1, 2, width)
+    return aligned_copy(a.transpose(0, 2, 1).reshape(-1))
+
+
+def expand_lanes(replicas, data_nsimd):
+    """Vectors of len(replicas)*data_nsimd lanes from RIRI arrays of
+    data_nsimd lanes: replica nn at lanes nn*data_nsimd + n (bench.cc:110-138)."""
+    parts = [np.ascontiguousarray(r).reshape(-1, 1, data_nsimd, 2) for r in replicas]
+    return aligned_copy(np.concatenate(parts, axis=1).reshape(-1))
+
+
+def replica_lanes(ref, width, data_nsimd, nn):
+    """Replica nn of an expanded RIRI array, as a data_nsimd-lane array."""
+    a = np.ascontiguousarray(ref).reshape(-1, width // data_nsimd, data_nsimd, 2)
+    return aligned_copy(a[:, nn].reshape(-1))
+
+
 def sha256(a):
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
 

@@ -210,6 +210,89 @@ def test_dslash_kernel_dropin_arbitrary_tables(gpu, gbx, orc, nsimd, dtype):
     assert rel_l2(out, want) <= tol
 
 
+def relabel_outer_sites(U, inp, nbr, prm, nsite, Ls, seed=7):
+    """The same problem with its outer sites permuted: non-canonical tables."""
+    rng = np.random.default_rng(seed)
+    perm = rng.permutation(nsite)  # new index of old outer site
+    inv = np.argsort(perm)
+    Ur = U.reshape(nsite, -1)[inv].reshape(-1).copy()
+    inr = inp.reshape(nsite, Ls, -1)[inv].reshape(-1).copy()
+    nb = nbr.reshape(nsite, Ls, 8)[inv]
+    nb = (perm[nb // Ls] * Ls + nb % Ls).astype(np.uint64).reshape(-1).copy()
+    pr = prm.reshape(nsite, Ls, 8)[inv].reshape(-1).copy()
+    return Ur, inr, nb, pr
+
+
+@pytest.mark.parametrize("nsimd", [1, 2, 8, 16])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32])
+@pytest.mark.parametrize("tables", ["canonical", "relabelled"])
+def test_dslash_kernel_rrii_layout(gpu, gbx, orc, nsimd, dtype, tables):
+    """RRII host arrays (bench.cc:105-106): the result is the RIRI result in
+    RRII order, bit for bit, on both device paths, and matches the oracle."""
+    dims, Ls = (4, 4, 4, 8), 3
+    U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, nsimd, dtype)
+    if tables == "relabelled":
+        U, inp, nbr, prm = relabel_outer_sites(U, inp, nbr, prm, nsite, Ls)
+    for dag in (False, True):
+        out = np.zeros_like(inp)
+        gbx.dslash_kernel(2, U, out, inp, nbr, nsite, Ls, prm, nsimd, dag=dag)
+        path = gbx.dslash_last_path()
+        out_r = np.zeros_like(inp)
+        gbx.dslash_kernel(2, orc.to_rrii(U, nsimd), out_r, orc.to_rrii(inp, nsimd), nbr, nsite, Ls, prm, nsimd, dag=dag,
+                          rrii=True)
+        assert gbx.dslash_last_path() == path == (1 if tables == "canonical" else 2)
+        assert np.array_equal(out_r, orc.to_rrii(out, nsimd))
+        want = orc.hand(orc.aligned_copy(U), orc.aligned_copy(inp), orc.aligned_copy(nbr), orc.aligned_copy(prm), nsite,
+                        Ls, nsimd, dag=dag)
+        tol = 1e-14 if dtype == np.float64 else 1e-6
+        assert rel_l2(orc.from_rrii(out_r, nsimd), want) <= tol
+
+
+@pytest.mark.parametrize("data_nsimd,width", [(1, 4), (2, 8), (4, 16), (8, 8)])
+@pytest.mark.parametrize("rrii", [False, True])
+@pytest.mark.parametrize("tables", ["canonical", "relabelled"])
+def test_dslash_kernel_expanded_simd(gpu, gbx, orc, data_nsimd, width, rrii, tables):
+    """Vectors of `width` lanes holding width/data_nsimd replicas of a
+    data_nsimd-lane problem (bench.cc:1-3, 98-138); tables and permutes are
+    those of data_nsimd (Simd_riri_vec.h:466), so replica nn's output is the
+    oracle's Dhop of replica nn.  The replicas here hold different fields."""
+    dims, Ls = (4, 4, 4, 8), 2
+    R = width // data_nsimd
+    nsite = int(np.prod(dims)) // data_nsimd
+    nbr, prm = orc.tables(dims, data_nsimd, Ls)
+    Us, ins = [], []
+    for nn in range(R):
+        dbl = gbx.double_links(dims, gbx.gen_su3_links(dims, 600 + nn))
+        Us.append(orc.pack_gauge(dims, data_nsimd, dbl))
+        ins.append(orc.pack_spinor(dims, data_nsimd, Ls, gbx.gen_spinor(dims, Ls, 1, 700 + nn)[0]))
+    if tables == "relabelled":
+        rel = [relabel_outer_sites(u, i, nbr, prm, nsite, Ls) for u, i in zip(Us, ins)]
+        Us, ins = [r[0] for r in rel], [r[1] for r in rel]
+        nbr, prm = rel[0][2], rel[0][3]
+    U, inp = orc.expand_lanes(Us, data_nsimd), orc.expand_lanes(ins, data_nsimd)
+    if rrii:
+        U, inp = orc.to_rrii(U, width), orc.to_rrii(inp, width)
+    out = np.zeros_like(inp)
+    gbx.dslash_kernel(2, U, out, inp, nbr, nsite, Ls, prm, width, rrii=rrii, data_nsimd=data_nsimd)
+    assert gbx.dslash_last_path() == (1 if tables == "canonical" else 2)
+    if rrii:
+        out = orc.from_rrii(out, width)
+    for nn in range(R):
+        want = orc.hand(orc.aligned_copy(Us[nn]), orc.aligned_copy(ins[nn]), orc.aligned_copy(nbr),
+                        orc.aligned_copy(prm), nsite, Ls, data_nsimd)
+        assert rel_l2(orc.replica_lanes(out, width, data_nsimd, nn), want) <= 1e-14
+
+
+def test_dslash_kernel_layout_argument_errors(gpu, gbx, orc):
+    dims, Ls = (4, 4, 4, 4), 2
+    U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, 2, np.float64)
+    out = np.zeros_like(inp)
+    for width, data in ((2, 4), (6, 2), (128, 2)):  # data wider than vector, not a power of two, too wide
+        with pytest.raises(gbx.GbxError) as e:
+            gbx.dslash_kernel(1, U, out, inp, nbr, nsite, Ls, prm, width, data_nsimd=data)
+        assert e.value.status == 1
+
+
 def test_dslash_kernel_rejects_bad_tables(gpu, gbx, orc):
     dims, Ls, nsimd = (4, 4, 4, 4), 2, 1
     U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, nsimd, np.float64)

@@ -352,6 +352,32 @@ def test_lane_layout_matches_grid_virtual_nodes(orc):
             assert ref[o, 0, l, 0] == g
 
 
+@pytest.mark.parametrize("width", [1, 2, 8])
+def test_rrii_layout_is_planar_re_im(orc, width):
+    """RRII vector = its lanes' real parts, then their imaginary parts
+    (bench.cc:105-106), checked through numpy's complex view of RIRI."""
+    rng = np.random.default_rng(width)
+    riri = rng.standard_normal(5 * width * 2)
+    z = riri.view(np.complex128).reshape(5, width)
+    rrii = orc.to_rrii(riri, width).reshape(5, 2, width)
+    assert np.array_equal(rrii[:, 0], z.real) and np.array_equal(rrii[:, 1], z.imag)
+    assert np.array_equal(orc.from_rrii(orc.to_rrii(riri, width), width), riri)
+
+
+def test_expanded_lanes_hold_replicas(orc):
+    """Expanded vectors: replica nn at lanes nn*DATA_SIMD + n (bench.cc:110-122);
+    with identical replicas every group of DATA_SIMD lanes repeats the data."""
+    rng = np.random.default_rng(1)
+    reps = [rng.standard_normal(6 * 2 * 2) for _ in range(4)]
+    ex = orc.expand_lanes(reps, 2)
+    z = ex.view(np.complex128).reshape(6, 8)
+    for nn in range(4):
+        assert np.array_equal(z[:, 2 * nn:2 * nn + 2], reps[nn].view(np.complex128).reshape(6, 2))
+        assert np.array_equal(orc.replica_lanes(ex, 8, 2, nn), reps[nn])
+    same = orc.expand_lanes([reps[0]] * 4, 2).view(np.complex128).reshape(6, 8)
+    assert all(np.array_equal(same[:, 0:2], same[:, 2 * k:2 * k + 2]) for k in range(4))
+
+
 # ------------------------------------------------- generator
 def test_su3_links_are_special_unitary(gbx):
     U = gbx.gen_su3_links((4, 4, 2, 2), 7)
