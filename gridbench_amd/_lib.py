@@ -40,7 +40,7 @@ def check(status, where="libgbx"):
 class KCfg(C.Structure):
     _fields_ = [("spw", C.c_int), ("threads", C.c_int), ("xcd", C.c_int), ("nt", C.c_int), ("kind", C.c_int),
                 ("mc", C.c_int), ("pf", C.c_int), ("wgpc", C.c_int),
-                ("cblk", C.c_int)]
+                ("cblk", C.c_int), ("tile", C.c_int)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

@@ -53,7 +53,7 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
     for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
       const uint32_t sl = w / M, m = w - sl * M;
       A rr[12], ri[12];
-      dev::dhop_site<P, DAG, false>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+      dev::dhop_site<P, DAG, 0>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
       dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
     }
     __syncthreads();  // LDS and s_idx are reused by the next item

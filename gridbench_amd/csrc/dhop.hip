@@ -23,7 +23,7 @@ namespace {
 // and minor indices m0 .. m0+mc-1 (minor chunk L / nwg_sites): with mc < M
 // the lattice is swept once per minor chunk, so the 8-neighbour working set
 // is that of an mc-wide field (the links are re-read once per chunk).
-template <int P, bool DAG, bool C12, bool AXPBY, bool PF>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF>
 __global__ void __launch_bounds__(PF ? 256 : 1024)
     k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
            void *__restrict__ out, const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mc,
@@ -71,7 +71,7 @@ struct LaunchArgs {
   double alpha, beta;
 };
 
-template <int P, bool DAG, bool C12, bool AXPBY, bool PF>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF>
 void launch_t(const LaunchArgs &a)
 {
   typedef typename dev::Traits<P>::acc A;
@@ -81,15 +81,16 @@ void launch_t(const LaunchArgs &a)
 }
 
 template <int P, bool DAG, bool C12>
-void launch_dc(bool axpby, bool pf, const LaunchArgs &a)
+void launch_dc(bool axpby, int pf, const LaunchArgs &a)
 {
-  if (axpby) launch_t<P, DAG, C12, true, false>(a);
-  else if (pf) launch_t<P, DAG, C12, false, true>(a);
-  else launch_t<P, DAG, C12, false, false>(a);
+  if (axpby) launch_t<P, DAG, C12, true, 0>(a);
+  else if (pf == 2) launch_t<P, DAG, C12, false, 2>(a);
+  else if (pf == 1) launch_t<P, DAG, C12, false, 1>(a);
+  else launch_t<P, DAG, C12, false, 0>(a);
 }
 
 template <int P>
-void launch_p(bool dag, bool c12, bool axpby, bool pf, const LaunchArgs &a)
+void launch_p(bool dag, bool c12, bool axpby, int pf, const LaunchArgs &a)
 {
   if (!dag && !c12) launch_dc<P, false, false>(axpby, pf, a);
   else if (!dag && c12) launch_dc<P, false, true>(axpby, pf, a);
@@ -178,6 +179,7 @@ int dhop_cfg_check(const dhop_cfg &c, const gbx_gauge_s *u, const gbx_spinor_s *
 {
   (void)in;
   GBX_CHECK(c.spw >= 1 && c.spw <= 4096, GBX_EINVAL, "kernel config: spw=%d", c.spw);
+  GBX_CHECK(c.pf >= 0 && c.pf <= 2, GBX_EINVAL, "kernel config: pf=%d", c.pf);
   GBX_CHECK(c.threads >= 64 && c.threads <= (c.pf ? 256 : 1024) && c.threads % 64 == 0, GBX_EINVAL,
             "kernel config: threads=%d (pf=%d)", c.threads, c.pf);
   GBX_CHECK(c.mc >= 0, GBX_EINVAL, "kernel config: mc=%d", c.mc);
@@ -244,7 +246,7 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   a.nt = c.nt;
   a.alpha = alpha;
   a.beta = beta;
-  const bool pf = c.pf != 0 && !axpby;
+  const int pf = axpby ? 0 : c.pf;
   switch (u->prec) {
   case GBX_F64: launch_p<GBX_F64>(dag, u->comp12, axpby, pf, a); break;
   case GBX_F32: launch_p<GBX_F32>(dag, u->comp12, axpby, pf, a); break;
@@ -271,6 +273,7 @@ static gbx_kcfg to_kcfg(const dhop_cfg &c)
   k.pf = c.pf;
   k.wgpc = c.wgpc;
   k.cblk = c.cblk;
+  k.tile = 0;
   return k;
 }
 
@@ -292,6 +295,7 @@ int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, in
 {
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dhop: dag must be 0 or 1");
   if (k.kind == 1) return mfma_launch(u, out, in, dag, k, st);
+  if (k.kind == 2) return tile_launch(u, out, in, dag, k, st);
   GBX_CHECK(k.kind == 0, GBX_EINVAL, "dhop: kernel kind %d", k.kind);
   return dhop_launch(u, out, in, dag, from_kcfg(k), 0.0, 1.0, nullptr, st);
 }

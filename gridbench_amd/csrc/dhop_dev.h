@@ -216,17 +216,18 @@ __device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri
 }
 
 // Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
-// PF = false: gather and compute leg by leg.  PF = true: the gather of leg
-// d+1 is issued before leg d is computed (two gather buffers in registers),
-// which keeps a second set of loads in flight per wavefront.
-template <int P, bool DAG, bool PF>
+// PF = 0: gather and compute leg by leg.  PF = 1: the gather of leg d+1 is
+// issued before leg d is computed (two gather buffers in registers), which
+// keeps a second set of loads in flight per wavefront.  PF = 2: three
+// buffers, gathers two legs ahead.
+template <int P, bool DAG, int PF>
 __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
                                           const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
                                           typename Traits<P>::acc *ri)
 {
   typedef typename Traits<P>::acc A;
   constexpr bool CH = (P == GBX_F64 || P == GBX_F32);
-  if (!PF) {
+  if (PF == 0) {
     A v[24];
 #define GBX_LEG(D, FIRST)                                                  \
     load_nb<P>(in, nb[D], M, m, v);                                        \
@@ -240,6 +241,24 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
     GBX_LEG(6, false)
     GBX_LEG(7, false)
 #undef GBX_LEG
+  } else if (PF == 2) {
+    A v0[24], v1[24], v2[24];
+    load_nb<P>(in, nb[0], M, m, v0);
+    load_nb<P>(in, nb[1], M, m, v1);
+    load_nb<P>(in, nb[2], M, m, v2);
+    leg_compute<LegProj<DAG, 0>::value, true, CH>(v0, U + 0 * 18, rr, ri);
+    load_nb<P>(in, nb[3], M, m, v0);
+    leg_compute<LegProj<DAG, 1>::value, false, CH>(v1, U + 1 * 18, rr, ri);
+    load_nb<P>(in, nb[4], M, m, v1);
+    leg_compute<LegProj<DAG, 2>::value, false, CH>(v2, U + 2 * 18, rr, ri);
+    load_nb<P>(in, nb[5], M, m, v2);
+    leg_compute<LegProj<DAG, 3>::value, false, CH>(v0, U + 3 * 18, rr, ri);
+    load_nb<P>(in, nb[6], M, m, v0);
+    leg_compute<LegProj<DAG, 4>::value, false, CH>(v1, U + 4 * 18, rr, ri);
+    load_nb<P>(in, nb[7], M, m, v1);
+    leg_compute<LegProj<DAG, 5>::value, false, CH>(v2, U + 5 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 6>::value, false, CH>(v0, U + 6 * 18, rr, ri);
+    leg_compute<LegProj<DAG, 7>::value, false, CH>(v1, U + 7 * 18, rr, ri);
   } else {
     A v0[24], v1[24];
     load_nb<P>(in, nb[0], M, m, v0);

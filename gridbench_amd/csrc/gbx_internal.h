@@ -5,6 +5,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -86,6 +87,8 @@ struct gbx_geom_s {
   // pair reads across them, so it is ordered on one stream) and holds a
   // reference to it; nullptr when the geometry owns its stream.
   gbx_geom_s *stream_owner = nullptr;
+  // Tile tables of the LDS-tiled kernel, by packed tile shape (device arrays).
+  std::map<int, uint32_t *> tile_tabs;
 };
 
 // Full-lattice lexicographic site of each slot (the gauge upload's index).
@@ -154,6 +157,11 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
                 hipStream_t stream);
 dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in);
 int dhop_cfg_check(const dhop_cfg &cfg, const gbx_gauge_s *u, const gbx_spinor_s *in);
+// LDS-tiled kernel (kind 2): out = Dhop(in) on a whole lattice.
+int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *in);
+int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
+void tile_tables_free(gbx_geom_s *g);
+int pack_tile(const int e[4]);
 // Any kernel kind (VALU or MFMA) with an explicit configuration.
 int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
 std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in);

@@ -310,6 +310,7 @@ void geom_release(gbx_geom_s *g)
   (void)hipFree(g->slot_of_site);
   (void)hipFree(g->nbr);
   (void)hipFree(g->gsite_of_slot);
+  tile_tables_free(g);
   gbx_geom_s *owner = g->stream_owner;
   if (g->stream && !owner) (void)hipStreamDestroy(g->stream);
   delete g;

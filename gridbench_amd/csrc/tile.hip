@@ -1,0 +1,292 @@
+// LDS-tiled hopping kernel (kernel kind 2).
+//
+// The VALU kernel gathers each of a site's 8 neighbours from L2: every
+// spinor line is requested from L2 eight times per application, and those
+// requests, not HBM bandwidth, set its speed.  Here a workgroup owns a 4D
+// box of sites (the tile, extents tx x ty x tz x tt) and a chunk of mc minor
+// indices.  It loads the tile's own spinors once into LDS; a leg whose
+// neighbour lies inside the box reads LDS, only legs that leave the box
+// gather from L2.  For a 2^4 tile half the legs stay inside, so each spinor
+// line is requested 1 + 4 times instead of 8.
+//
+// Tiles are visited in the geometry's slot order (sorted by their smallest
+// slot), so consecutive workgroups -- and, with xcd = 1, each XCD's share --
+// cover a compact region whose halos the XCD's L2 holds.  A site's data is
+// contiguous in the SoA layout ([slot][chunk][minor]), so the tile's sites
+// need not be consecutive slots.
+#include <algorithm>
+#include <map>
+#include <mutex>
+
+#include "dhop_dev.h"
+#include "gbx_internal.h"
+
+namespace {
+
+std::mutex g_tile_mu;
+
+struct TileShape {
+  int e[4];
+  int T() const { return e[0] * e[1] * e[2] * e[3]; }
+};
+
+TileShape unpack_tile(int packed)
+{
+  TileShape s;
+  for (int d = 0; d < 4; d++) s.e[d] = (packed >> (4 * d)) & 15;
+  return s;
+}
+
+// Slots of every tile's sites, tiles ordered by smallest slot, sites in
+// tile-lexicographic order (x fastest).  Device array [ntiles][T].
+int tile_table(gbx_geom_s *g, int packed, const uint32_t **tab, uint32_t *ntiles)
+{
+  std::lock_guard<std::mutex> lk(g_tile_mu);
+  auto it = g->tile_tabs.find(packed);
+  const TileShape s = unpack_tile(packed);
+  uint64_t nt = g->vol / (uint64_t)s.T();
+  if (it != g->tile_tabs.end()) {
+    *tab = it->second;
+    *ntiles = (uint32_t)nt;
+    return GBX_OK;
+  }
+  const int *D = g->dims;
+  int nb[4];
+  for (int d = 0; d < 4; d++) nb[d] = D[d] / s.e[d];
+  const int T = s.T();
+  std::vector<std::pair<uint32_t, uint32_t>> order(nt);  // (min slot, tile)
+  std::vector<uint32_t> slots((size_t)nt * T);
+  for (uint64_t t = 0; t < nt; t++) {
+    uint64_t r = t;
+    int o[4];
+    for (int d = 0; d < 4; d++) {
+      o[d] = (int)(r % (uint64_t)nb[d]) * s.e[d];
+      r /= (uint64_t)nb[d];
+    }
+    uint32_t mn = UINT32_MAX;
+    for (int i = 0; i < T; i++) {
+      int c[4], q = i;
+      for (int d = 0; d < 4; d++) {
+        c[d] = o[d] + q % s.e[d];
+        q /= s.e[d];
+      }
+      const uint64_t site = (uint64_t)c[0] + (uint64_t)D[0] * (c[1] + (uint64_t)D[1] * (c[2] + (uint64_t)D[2] * c[3]));
+      const uint32_t slot = g->slot_of_site_h[site];
+      slots[t * T + i] = slot;
+      mn = std::min(mn, slot);
+    }
+    order[t] = {mn, (uint32_t)t};
+  }
+  std::sort(order.begin(), order.end());
+  std::vector<uint32_t> sorted((size_t)nt * T);
+  for (uint64_t t = 0; t < nt; t++)
+    std::copy(slots.begin() + (size_t)order[t].second * T, slots.begin() + (size_t)(order[t].second + 1) * T,
+              sorted.begin() + t * T);
+  uint32_t *d = nullptr;
+  GBX_HIP(hipMalloc(&d, sorted.size() * sizeof(uint32_t)));
+  if (hipMemcpy(d, sorted.data(), sorted.size() * sizeof(uint32_t), hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipFree(d);
+    return gbx_fail(GBX_EHIP, "tile table upload failed");
+  }
+  g->tile_tabs[packed] = d;
+  *tab = d;
+  *ntiles = (uint32_t)nt;
+  return GBX_OK;
+}
+
+struct TileArgs {
+  const void *gauge;
+  const uint32_t *nbr, *tslots;
+  const void *in;
+  void *out;
+  uint32_t ntiles, M, mc, nwg, per_xcd;
+  int ex, ey, ez, et, wrap;  // tile extents; bit d of wrap: extent d spans the lattice (periodic inside the tile)
+  int xcd, nt;
+};
+
+template <int P, bool DAG, bool C12>
+__global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArgs a)
+{
+  typedef typename dev::Traits<P>::acc A;
+  typedef typename dev::Traits<P>::vec V;
+  constexpr int NCH = dev::Traits<P>::NCH, RL = C12 ? 12 : 18;
+  // LDS holds a (site, minor) point's NCH chunks contiguously, SW chunks
+  // apart: one base address per leg, the chunks at immediate offsets.  FP64's
+  // 12 chunks are padded to 13 so that 8 consecutive lanes' 16-byte reads
+  // fall in distinct banks.
+  constexpr int SW = NCH == 12 ? 13 : NCH;
+  constexpr bool CH = (P == GBX_F64 || P == GBX_F32);
+  const int e[4] = {a.ex, a.ey, a.ez, a.et};
+  const int T = e[0] * e[1] * e[2] * e[3];
+  extern __shared__ __align__(16) unsigned char smem[];
+  V *sIn = (V *)smem;                                       // [T][mc][SW]
+  A *sU = (A *)(sIn + (size_t)T * SW * a.mc);               // [T][8][18]
+  uint32_t *sN = (uint32_t *)(sU + (size_t)T * 8 * 18);     // [T][8]
+  uint32_t *sQ = sN + (size_t)T * 8;                        // [T]
+  int *sL = (int *)(sQ + T);                                // [T][8] local neighbour or -1
+
+  const uint32_t L = a.xcd ? (blockIdx.x & 7u) * a.per_xcd + (blockIdx.x >> 3) : blockIdx.x;
+  if (L >= a.nwg) return;  // uniform over the workgroup, before any barrier
+  const uint32_t tile = L % a.ntiles, chunk = L / a.ntiles;
+  const uint32_t m0 = chunk * a.mc, mcc = min(a.mc, a.M - m0);
+  for (int i = threadIdx.x; i < T; i += blockDim.x) sQ[i] = a.tslots[(uint64_t)tile * T + i];
+  __syncthreads();
+  // The tile's spinors (minor chunk), links and neighbour slots into LDS.
+  const V *gin = (const V *)a.in;
+  const uint32_t nload = (uint32_t)T * NCH * mcc;
+  for (uint32_t w = threadIdx.x; w < nload; w += blockDim.x) {
+    const uint32_t ik = w / mcc, m = w - ik * mcc;
+    const uint32_t i = ik / NCH, k = ik - i * NCH;
+    sIn[(i * a.mc + m) * SW + k] = gin[((uint64_t)sQ[i] * NCH + k) * a.M + m0 + m];
+  }
+  typedef typename dev::Traits<P>::st S;
+  const S *gU = (const S *)a.gauge;
+  for (uint32_t w = threadIdx.x; w < (uint32_t)T * 8 * RL; w += blockDim.x) {
+    const uint32_t i = w / (8 * RL), r = w - i * (8 * RL);
+    const uint32_t link = r / RL, k = r - link * RL;
+    sU[(i * 8 + link) * 18 + k] = (A)gU[(uint64_t)sQ[i] * 8 * RL + r];
+  }
+  for (uint32_t w = threadIdx.x; w < (uint32_t)T * 8; w += blockDim.x) {
+    sN[w] = a.nbr[(uint64_t)sQ[w >> 3] * 8 + (w & 7)];
+    // Leg d's neighbour inside the box, wrapping only along an extent that
+    // spans the whole (periodic) lattice; -1: it leaves the box.
+    const int i = (int)(w >> 3), d = (int)(w & 7), mu = d & 3;
+    int stride = 1;
+    for (int k = 0; k < mu; k++) stride *= e[k];
+    const int c = (i / stride) % e[mu];
+    int c2 = c + (d < 4 ? 1 : -1);
+    bool inside = c2 >= 0 && c2 < e[mu];
+    if (!inside && ((a.wrap >> mu) & 1)) {
+      c2 = (c2 + e[mu]) % e[mu];
+      inside = true;
+    }
+    sL[w] = inside ? i + (c2 - c) * stride : -1;
+  }
+  if (C12) {
+    __syncthreads();
+    for (uint32_t l = threadIdx.x; l < (uint32_t)T * 8; l += blockDim.x) dev::su3_row2(sU + l * 18);
+  }
+  __syncthreads();
+
+  // A leg whose neighbour is inside the box reads LDS, one that leaves it
+  // gathers from L2; each path carries its own copy of the leg arithmetic
+  // (a shared tail after the two loads makes the compiler keep both sets of
+  // loaded values live across the legs).
+  const uint32_t items = (uint32_t)T * mcc;
+  for (uint32_t w = threadIdx.x; w < items; w += blockDim.x) {
+    const int i = (int)(w / mcc);
+    const uint32_t ml = w - (uint32_t)i * mcc, m = m0 + ml;
+    A rr[12], ri[12];
+    A v[24];
+#define GBX_TLEG(D, FIRST)                                                                                             \
+    {                                                                                                                  \
+      const int ln = sL[i * 8 + (D)];                                                                                  \
+      if (ln >= 0) {                                                                                                   \
+        const V *p = sIn + ((uint32_t)ln * a.mc + ml) * SW;                                                            \
+        _Pragma("unroll") for (int k = 0; k < NCH; k++) dev::load_chunk<P, A>(p, k, v + k * dev::Traits<P>::R);        \
+        dev::leg_compute<dev::LegProj<DAG, D>::value, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                    \
+      } else {                                                                                                         \
+        dev::load_nb<P>(a.in, sN[i * 8 + D], a.M, m, v);                                                               \
+        dev::leg_compute<dev::LegProj<DAG, D>::value, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                    \
+      }                                                                                                                \
+    }
+    GBX_TLEG(0, true)
+    GBX_TLEG(1, false)
+    GBX_TLEG(2, false)
+    GBX_TLEG(3, false)
+    GBX_TLEG(4, false)
+    GBX_TLEG(5, false)
+    GBX_TLEG(6, false)
+    GBX_TLEG(7, false)
+#undef GBX_TLEG
+    dev::store_site<P, false>(a.out, nullptr, sQ[i], a.M, m, rr, ri, (A)0, (A)1, a.nt != 0);
+  }
+}
+
+template <int P>
+void launch_p(bool dag, bool c12, dim3 grid, dim3 block, size_t lds, hipStream_t st, const TileArgs &a)
+{
+  if (!dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, false, false>), grid, block, lds, st, a);
+  else if (!dag && c12) hipLaunchKernelGGL((k_dhop_tile<P, false, true>), grid, block, lds, st, a);
+  else if (dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, true, false>), grid, block, lds, st, a);
+  else hipLaunchKernelGGL((k_dhop_tile<P, true, true>), grid, block, lds, st, a);
+}
+
+size_t tile_lds(int prec, int T, int mc)
+{
+  const size_t acc = prec == GBX_F64 ? 8 : 4;
+  const size_t nch = prec == GBX_F64 ? 12 : prec == GBX_F32 ? 6 : 3;
+  const size_t sw = nch == 12 ? 13 : nch;
+  return (size_t)T * sw * mc * 16 + (size_t)T * 8 * 18 * acc + (size_t)T * 8 * 4 + (size_t)T * 4 + (size_t)T * 8 * 4;
+}
+
+}  // namespace
+
+void tile_tables_free(gbx_geom_s *g)
+{
+  for (auto &kv : g->tile_tabs) (void)hipFree(kv.second);
+  g->tile_tabs.clear();
+}
+
+int pack_tile(const int e[4]) { return e[0] | (e[1] << 4) | (e[2] << 8) | (e[3] << 12); }
+
+// Configuration check for kind 2: the tile divides the lattice, its LDS fits.
+int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *in)
+{
+  const gbx_geom_s *g = u->geom;
+  GBX_CHECK(g->parity < 0, GBX_EUNSUPPORTED, "tile kernel: whole lattices only");
+  const TileShape s = unpack_tile(k.tile);
+  for (int d = 0; d < 4; d++)
+    GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,
+              s.e[d], g->dims[d]);
+  const uint32_t mc = (k.mc <= 0 || (uint32_t)k.mc > in->M) ? in->M : (uint32_t)k.mc;
+  const int tmax = u->prec == GBX_F64 ? 512 : 1024;  // the kernel's launch bound
+  GBX_CHECK(k.threads >= 64 && k.threads <= tmax && k.threads % 64 == 0, GBX_EINVAL, "tile kernel: threads=%d (max %d)",
+            k.threads, tmax);
+  GBX_CHECK(k.wgpc >= 0 && k.wgpc <= 64, GBX_EINVAL, "tile kernel: wgpc=%d", k.wgpc);
+  const size_t lds = tile_lds(u->prec, s.T(), (int)mc);
+  GBX_CHECK(lds <= 160 * 1024, GBX_EINVAL, "tile kernel: tile of %d sites x %u minor needs %zu B of LDS", s.T(), mc, lds);
+  return GBX_OK;
+}
+
+int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st)
+{
+  GBX_TRY(tile_cfg_check(k, u, in));
+  gbx_geom_s *g = u->geom;
+  const TileShape s = unpack_tile(k.tile);
+  TileArgs a;
+  GBX_TRY(tile_table(g, k.tile, &a.tslots, &a.ntiles));
+  a.gauge = u->d;
+  a.nbr = g->nbr;
+  a.in = in->d;
+  a.out = out->d;
+  a.M = in->M;
+  a.mc = (k.mc <= 0 || (uint32_t)k.mc > in->M) ? in->M : (uint32_t)k.mc;
+  const uint64_t nwg = (uint64_t)a.ntiles * ((a.M + a.mc - 1) / a.mc);
+  GBX_CHECK(nwg < (1ull << 31), GBX_EINVAL, "tile kernel: %llu workgroups", (unsigned long long)nwg);
+  a.nwg = (uint32_t)nwg;
+  a.per_xcd = (a.nwg + 7) / 8;
+  a.ex = s.e[0];
+  a.ey = s.e[1];
+  a.ez = s.e[2];
+  a.et = s.e[3];
+  a.wrap = 0;
+  for (int d = 0; d < 4; d++)
+    if (s.e[d] == g->dims[d]) a.wrap |= 1 << d;
+  a.xcd = k.xcd;
+  a.nt = k.nt;
+  size_t lds = tile_lds(u->prec, s.T(), (int)a.mc);
+  if (k.wgpc > 0) {
+    const size_t pad = ((size_t)160 * 1024 / (size_t)k.wgpc) & ~(size_t)511;
+    if (pad > lds) lds = pad;
+  }
+  const dim3 grid(k.xcd ? a.per_xcd * 8 : a.nwg), block((unsigned)k.threads);
+  switch (u->prec) {
+  case GBX_F64: launch_p<GBX_F64>(dag, u->comp12, grid, block, lds, st, a); break;
+  case GBX_F32: launch_p<GBX_F32>(dag, u->comp12, grid, block, lds, st, a); break;
+  case GBX_F16: launch_p<GBX_F16>(dag, u->comp12, grid, block, lds, st, a); break;
+  default: launch_p<GBX_BF16>(dag, u->comp12, grid, block, lds, st, a); break;
+  }
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}

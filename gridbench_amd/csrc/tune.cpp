@@ -68,7 +68,7 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
         if ((uint64_t)t < items && items % t == 0 && (uint64_t)t != items) thr.push_back(t);
       if (items < 1024 && items % 64 != 0) thr.push_back((int)((items + 63) / 64 * 64));
       for (int t : thr)
-        for (int pf = 0; pf <= 1; pf++) {
+        for (int pf = 0; pf <= 2; pf++) {
           dhop_cfg c{spw, t, 1, 1, mc, pf, 0, 0};
           if (dhop_cfg_check(c, u, in) != GBX_OK) continue;
           v.push_back(gbx_kcfg{spw, t, 1, 1, 0, mc, pf, 0, 0});
@@ -76,6 +76,24 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
     }
   }
   gbx_clear_error();
+  if (kinds & GBX_KIND_TILE) {
+    // LDS-tiled kernel: compact boxes of 8..64 sites, minor chunks, and
+    // workgroups of one or several items per thread.
+    static const int shapes[][4] = {{2, 2, 2, 2}, {4, 2, 2, 2}, {2, 2, 2, 4}, {2, 4, 2, 2}, {4, 4, 2, 2}, {2, 2, 4, 4},
+                                    {4, 2, 2, 4}, {2, 2, 2, 1}, {2, 2, 1, 2}, {3, 3, 3, 1}, {3, 3, 1, 3}, {3, 2, 2, 2}};
+    for (const auto &e : shapes)
+      for (int mc : minor_chunks(M)) {
+        const uint32_t mw = mc ? (uint32_t)mc : M;
+        const uint64_t items = (uint64_t)e[0] * e[1] * e[2] * e[3] * mw;
+        for (int t : {256, 512, 1024}) {
+          if ((uint64_t)t > (items + 63) / 64 * 64 && t != 256) continue;
+          gbx_kcfg k{0, t, 1, 1, 2, mc, 0, 0, 0, pack_tile(e)};
+          if (tile_cfg_check(k, u, in) != GBX_OK) continue;
+          v.push_back(k);
+        }
+      }
+    gbx_clear_error();
+  }
   if ((kinds & GBX_KIND_MFMA) && mfma_supported(u, in)) {
     // MFMA form: minor columns per wavefront unit (multiples of 16 dividing
     // M), sites per workgroup, wavefronts per workgroup, next-tile gather.
@@ -106,6 +124,10 @@ std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
         k.wgpc = wgpc;
         v.push_back(k);
       }
+  if (b.kind == 2) {  // tile kernel: XCD order, store kind, residency cap
+    for (gbx_kcfg &k : v) k.wgpc = k.wgpc > 4 ? 0 : k.wgpc;
+    return v;
+  }
   if (b.mc > 0 && b.kind == 0)  // chunk-inner visiting order: site groups per chunk cycle
     for (int cblk : {16, 64, 256, 1024}) {
       gbx_kcfg k = b;
@@ -118,12 +140,16 @@ std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
 bool parse_line(const std::string &line, std::string &sig, gbx_kcfg &k, double &us)
 {
   char s[200];
-  k = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  k = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   if (sscanf(line.c_str(),
              "{\"sig\": \"%199[^\"]\", \"spw\": %d, \"threads\": %d, \"xcd\": %d, \"nt\": %d, \"kind\": %d, \"mc\": %d, "
-             "\"pf\": %d, \"wgpc\": %d, \"cblk\": %d, \"usec\": %lf}",
-             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &k.mc, &k.pf, &k.wgpc, &k.cblk, &us) != 11)
+             "\"pf\": %d, \"wgpc\": %d, \"cblk\": %d, ",
+             s, &k.spw, &k.threads, &k.xcd, &k.nt, &k.kind, &k.mc, &k.pf, &k.wgpc, &k.cblk) != 10)
     return false;
+  const size_t t = line.find("\"tile\": ");
+  if (t != std::string::npos && sscanf(line.c_str() + t, "\"tile\": %d", &k.tile) != 1) return false;
+  const size_t u = line.find("\"usec\": ");
+  if (u == std::string::npos || sscanf(line.c_str() + u, "\"usec\": %lf", &us) != 1) return false;
   sig = s;
   return true;
 }
@@ -135,10 +161,16 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
 {
   GBX_CHECK(u && out && in && best && best_usec, GBX_ENULL, "gbx_tune: NULL argument");
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_tune: dag=%d", dag);
-  GBX_CHECK(kinds >= 1 && kinds <= 3, GBX_EINVAL, "gbx_tune: kinds=%d", kinds);
-  const bool all = kinds == (GBX_KIND_VALU | GBX_KIND_MFMA);
-  const std::string sig = dhop_signature(u, in) + (all ? "" : (kinds == GBX_KIND_VALU ? "#valu" : "#mfma"));
-  *best = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0};
+  GBX_CHECK(kinds >= 1 && kinds <= 7, GBX_EINVAL, "gbx_tune: kinds=%d", kinds);
+  const bool all = kinds == (GBX_KIND_VALU | GBX_KIND_MFMA | GBX_KIND_TILE);
+  std::string sig = dhop_signature(u, in);
+  if (!all) {
+    sig += "#";
+    if (kinds & GBX_KIND_VALU) sig += "valu";
+    if (kinds & GBX_KIND_MFMA) sig += "mfma";
+    if (kinds & GBX_KIND_TILE) sig += "tile";
+  }
+  *best = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   *best_usec = 0.0;
   if (cache_path) {
     std::ifstream f(cache_path);
@@ -150,7 +182,7 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
         *best = k;
         *best_usec = us;
       }
-    if (best->threads > 0 && best->spw > 0) {
+    if (best->threads > 0 && (best->spw > 0 || best->kind == 2)) {
       // validate against this build before trusting it
       double us2;
       if (time_cfg(u, out, in, dag, *best, 2, &us2) == GBX_OK) {
@@ -196,7 +228,8 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
     if (f)
       f << "{\"sig\": \"" << sig << "\", \"spw\": " << bk.spw << ", \"threads\": " << bk.threads << ", \"xcd\": " << bk.xcd
         << ", \"nt\": " << bk.nt << ", \"kind\": " << bk.kind << ", \"mc\": " << bk.mc << ", \"pf\": " << bk.pf
-        << ", \"wgpc\": " << bk.wgpc << ", \"cblk\": " << bk.cblk << ", \"usec\": " << bus << "}\n";
+        << ", \"wgpc\": " << bk.wgpc << ", \"cblk\": " << bk.cblk << ", \"tile\": " << bk.tile << ", \"usec\": " << bus
+        << "}\n";
   }
   return GBX_OK;
 }
@@ -204,5 +237,5 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
 extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
                         double *best_usec)
 {
-  return gbx_tune_kinds(u, out, in, dag, GBX_KIND_VALU | GBX_KIND_MFMA, cache_path, best, best_usec);
+  return gbx_tune_kinds(u, out, in, dag, GBX_KIND_VALU | GBX_KIND_MFMA | GBX_KIND_TILE, cache_path, best, best_usec);
 }

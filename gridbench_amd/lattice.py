@@ -311,8 +311,16 @@ class SpinorField:
             pass
 
 
-def kcfg(spw, threads, xcd=1, nt=1, kind=0, mc=0, pf=0, wgpc=0, cblk=0):
-    return KCfg(spw, threads, xcd, nt, kind, mc, pf, wgpc, cblk)
+def pack_tile(ext):
+    """4D box extents (each 1..15) in gbx_kcfg.tile's packed form."""
+    return int(ext[0]) | (int(ext[1]) << 4) | (int(ext[2]) << 8) | (int(ext[3]) << 12)
+
+
+def kcfg(spw, threads, xcd=1, nt=1, kind=0, mc=0, pf=0, wgpc=0, cblk=0, tile=0):
+    """Kernel configuration; kind 2 (LDS-tiled) takes tile = 4D box extents."""
+    if not isinstance(tile, int):
+        tile = pack_tile(tile)
+    return KCfg(spw, threads, xcd, nt, kind, mc, pf, wgpc, cblk, tile)
 
 
 def dhop(U, out, inp, dag=False, cfg=None):
@@ -350,15 +358,17 @@ def dhop_time(U, out, inp, dag=False, cfg=None, nrep=10):
 
 
 def tune(U, out, inp, dag=False, cache_path=None, kinds=None):
-    """Autotune; kinds=None searches both kernel forms and sets what gbx_dhop
-    uses, kinds="valu" / "mfma" restricts the search (result not adopted)."""
+    """Autotune; kinds=None searches every kernel form and sets what gbx_dhop
+    uses; kinds = "valu" / "mfma" / "tile" (or a list of them) restricts the
+    search (result not adopted)."""
     best = KCfg()
     us = C.c_double()
     cp = cache_path.encode() if cache_path else None
     if kinds is None:
         check(lib.gbx_tune(U.h, out.h, inp.h, int(bool(dag)), cp, C.byref(best), C.byref(us)), "gbx_tune")
     else:
-        k = {"valu": 1, "mfma": 2}[kinds]
+        names = [kinds] if isinstance(kinds, str) else list(kinds)
+        k = sum({"valu": 1, "mfma": 2, "tile": 4}[n] for n in names)
         check(lib.gbx_tune_kinds(U.h, out.h, inp.h, int(bool(dag)), k, cp, C.byref(best), C.byref(us)),
               "gbx_tune_kinds")
     return best, us.value
@@ -540,7 +550,8 @@ def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
 __all__ = [
     "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
-    "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "dhop", "dhop_axpby",
+    "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile", "dhop",
+    "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read",

@@ -219,15 +219,20 @@ typedef struct {
   int threads; /* workgroup size, multiple of 64, <= 1024               */
   int xcd;     /* 1: give each XCD a contiguous range of sites           */
   int nt;      /* 1: non-temporal output stores                          */
-  int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS)    */
+  int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS),   */
+               /* 2: LDS-tiled VALU kernel (whole lattices)              */
   int mc;      /* minor indices (s, rhs) per workgroup; 0 = all Ls*nrhs  */
-  int pf;      /* 1: gather of leg d+1 issued before leg d is computed   */
+  int pf;      /* 1 (2): gather of leg d+1 (d+2) issued before leg d is  */
+               /* computed; VALU kernel, workgroups of <= 256 threads    */
   int wgpc;    /* cap on resident workgroups per CU (0 = none): the LDS  */
                /* request is padded to 160 KiB / wgpc, which shrinks the */
                /* set of sites in flight per XCD to what its L2 holds    */
   int cblk;    /* with mc < Ls*nrhs: workgroup site-groups visited for   */
                /* every minor chunk before moving on (0 = one sweep of   */
                /* the lattice per chunk); keeps a block's links in L2    */
+  int tile;    /* kind 2: 4D box a workgroup owns, extents packed as     */
+               /* x | y << 4 | z << 8 | t << 12 (each 1..15, dividing    */
+               /* the lattice); legs inside the box read LDS             */
 } gbx_kcfg;
 int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg);
 /* Configuration gbx_dhop would use now (autotuned if gbx_tune ran). */
@@ -245,11 +250,12 @@ int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx
 int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
              double *best_usec);
 /* The same search restricted to kernel kinds: bit 0 = VALU (kind 0), bit 1 */
-/* = MFMA (kind 1); kinds = 3 is gbx_tune.  A restricted result is cached  */
+/* = MFMA (kind 1), bit 2 = LDS-tiled (kind 2); kinds = 7 is gbx_tune.  A restricted result is cached  */
 /* under its own key and does not change what gbx_dhop uses.  For          */
 /* comparing the two forms on one problem.                                 */
 #define GBX_KIND_VALU 1
 #define GBX_KIND_MFMA 2
+#define GBX_KIND_TILE 4
 int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kinds, const char *cache_path,
                    gbx_kcfg *best, double *best_usec);
 

@@ -90,6 +90,10 @@ def test_launch_configs_identical(gpu, gbx, problem, spw, threads, xcd, nt):
     base = run_dhop(gbx, p, "f64")
     got = run_dhop(gbx, p, "f64", cfg=gbx.kcfg(spw, threads, xcd, nt))
     assert np.array_equal(base, got)
+    if threads <= 256:  # the gather pipelines (launch bound 256)
+        for pf in (1, 2):
+            got = run_dhop(gbx, p, "f64", cfg=gbx.kcfg(spw, threads, xcd, nt, pf=pf))
+            assert np.array_equal(base, got), pf
 
 
 def test_axpby(gpu, gbx, problem):
@@ -364,7 +368,7 @@ def test_multilevel_site_orders(gpu, gbx, problem, levels):
     assert rel_l2(y.download()[0], p.oracle_lex()) <= 1e-14
 
 
-@pytest.mark.parametrize("mc,cblk,pf", [(2, 0, 0), (4, 3, 1), (8, 7, 0), (4, 1000, 1)])
+@pytest.mark.parametrize("mc,cblk,pf", [(2, 0, 0), (4, 3, 1), (8, 7, 0), (4, 1000, 1), (8, 0, 2), (2, 5, 2)])
 def test_minor_chunk_configs(gpu, gbx, problem, mc, cblk, pf):
     p = problem((4, 4, 4, 6), 4, nrhs=2)
     geom = gbx.Geometry(p.dims)
