@@ -14,8 +14,9 @@ Also reported, each with its own seeds:
              Ls12 x 16 RHS in FP32 and BF16 (gauge seed 401, spinors 402).
              Throughput per right-hand side relative to FP64 1-RHS (per 5D
              site: the Ls12 forms are scaled by 12/16), error against the
-             FP64 kernel on the same input; for 16-bit storage the MFMA form
-             and the best VALU configuration are both reported.
+             FP64 kernel on the same input; for the multi-RHS rows each
+             kernel form (VALU, MFMA, LDS-tiled) is also autotuned and timed
+             on its own ("forms").
   mixed      32^4 x Ls16: Dhop error of FP32/BF16/FP16/FP16+12-real storage
              against FP64 on the same input.
   cg         32^4 x Ls16: defect-correction CG on M^dag M x = b,
@@ -158,9 +159,9 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             for f in (x64, w):
                 f.close()
         forms = None
-        if compare and compare_forms:  # the matrix-core form against the best VALU configuration, both autotuned
+        if compare and compare_forms:  # each kernel form (VALU, matrix-core, LDS-tiled), autotuned on its own
             forms = {}
-            for kind in ("valu", "mfma"):
+            for kind in ("valu", "mfma", "tile"):
                 try:
                     kc, _ = tune(U, y, x, dag=dag, kinds=kind)
                     forms[kind] = {"us_per_launch": round(gbx.dhop_time(U, y, x, dag=dag, cfg=kc, nrep=20), 2),
@@ -184,7 +185,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
 
     variant("f64_1rhs_dag", "f64", 1, dag=True)
     variant("f32_1rhs", "f32", 1)
-    variant("f32_16rhs", "f32", 16)
+    variant("f32_16rhs", "f32", 16, compare=True)
     variant("f16_16rhs", "f16", 16, compare=True)
     variant("bf16_16rhs", "bf16", 16, compare=True)
     variant("f16_16rhs_comp12", "f16", 16, comp12=True)
