@@ -7,8 +7,10 @@
 // workgroups (a few per CU) pulls items from an atomic counter until the list
 // is exhausted -- every workgroup reaches that exit.  Per item the workgroup
 // stages the item's links into LDS and computes all (site, m) points, exactly
-// like the single-lattice kernel.  The counter is cleared by a one-thread
-// kernel node (no memset nodes in the graph).
+// like the single-lattice kernel (with its gather pipeline PF).  The next
+// item's index is claimed while the current item is computed, so the
+// atomic's round trip overlaps the work.  The counter is cleared by a
+// one-thread kernel node (no memset nodes in the graph).
 #include <algorithm>
 #include <vector>
 
@@ -29,21 +31,22 @@ constexpr int kThreads = 256;
 
 __global__ void k_reset(uint32_t *c) { c[0] = 0u; }
 
-template <int P, bool DAG>
+template <int P, bool DAG, int PF>
 __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
                                                          uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw,
                                                          int nt)
 {
   typedef typename dev::Traits<P>::acc A;
   extern __shared__ __align__(16) unsigned char smem[];
-  __shared__ uint32_t s_idx;
+  __shared__ uint32_t s_next;
   A *sU = (A *)smem;
   uint32_t *sN = (uint32_t *)(sU + (size_t)max_spw * 8 * 18);
-  for (;;) {
-    if (threadIdx.x == 0) s_idx = atomicAdd(counter, 1u);
-    __syncthreads();
-    const uint32_t idx = s_idx;
-    if (idx >= nitems) break;  // same value in every thread of the workgroup
+  if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+  __syncthreads();
+  uint32_t idx = s_next;
+  while (idx < nitems) {  // idx is the same in every thread of the workgroup
+    __syncthreads();      // every thread has read s_next
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);  // claimed now, read after this item
     const uint4 it = items[idx];
     const LatDesc L = lat[it.x];
     const uint32_t q0 = it.y, nsl = it.z;
@@ -53,10 +56,11 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
     for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
       const uint32_t sl = w / M, m = w - sl * M;
       A rr[12], ri[12];
-      dev::dhop_site<P, DAG, 0>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+      dev::dhop_site<P, DAG, PF>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
       dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
     }
-    __syncthreads();  // LDS and s_idx are reused by the next item
+    __syncthreads();  // LDS is reused by the next item; s_next is written
+    idx = s_next;
   }
 }
 
@@ -96,10 +100,10 @@ void launch(gbx_batch_s *b, hipStream_t st, int nt)
   const size_t lds = lds_for(P, b->max_spw);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, b->d_counter);
   if (b->dag)
-    hipLaunchKernelGGL((k_dhop_batch<P, true>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
+    hipLaunchKernelGGL((k_dhop_batch<P, true, 1>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
                        (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
   else
-    hipLaunchKernelGGL((k_dhop_batch<P, false>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
+    hipLaunchKernelGGL((k_dhop_batch<P, false, 1>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
                        (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
 }
 
