@@ -46,6 +46,10 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
+# The CPU baseline's OpenMP threads, one per core, packed (read when the
+# OpenMP runtime starts, i.e. before the libraries below are loaded).
+os.environ.setdefault("OMP_PLACES", "cores")
+os.environ.setdefault("OMP_PROC_BIND", "close")
 
 DIMS = (24, 24, 24, 24)
 LS = 16
@@ -94,7 +98,23 @@ def cpu_baseline(dbl, psi, threads, budget_s=12.0):
     gflops = 1320.0 * V * LS / dt / 1e9
     return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
             "sample": f"24^4 x Ls16 FP64 1-RHS, {n} full applications after 1 untimed (oracle, OpenMP "
-                      f"{threads} threads, -O3 -march=x86-64-v3)", "ms_per_application": round(dt * 1e3, 2)}
+                      f"{threads} threads, -O3 -march=x86-64-v3)", "ms_per_application": round(dt * 1e3, 2),
+            "host": host_info()}
+
+
+def host_info():
+    """The host the CPU baseline ran on."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(),
+            "omp": {k: os.environ.get(k) for k in ("OMP_NUM_THREADS", "OMP_PLACES", "OMP_PROC_BIND")}}
 
 
 def rel_err_device(gbx, geom, got, want64):
@@ -391,7 +411,7 @@ def batch_section(gbx, nrep):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--steps", type=int, default=1000, help="timed applications (the reference's nrep)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-variants", action="store_true", help="only the FP64 1-RHS headline")

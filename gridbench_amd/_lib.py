@@ -81,6 +81,11 @@ PROTOTYPES = {
     "gbx_dslash_kernel_expand_f64": (i32, [i32, i32, i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
     "gbx_dslash_kernel_expand_f32": (i32, [i32, i32, i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
     "gbx_dslash_last_path": (i32, []),
+    "gbx_dslash_ctx_create": (i32, [C.POINTER(C.c_void_p)]),
+    "gbx_dslash_ctx_destroy": (i32, [C.c_void_p]),
+    "gbx_dslash_ctx_kernel_f64": (i32, [C.c_void_p, i32, i32, i32, i32, i32, pdbl, pdbl, pdbl, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_ctx_kernel_f32": (i32, [C.c_void_p, i32, i32, i32, i32, i32, pflt, pflt, pflt, pu64, u64, u64, pu8, pdbl]),
+    "gbx_dslash_ctx_last_path": (i32, [C.c_void_p]),
     "gbx_table_generate": (i32, [pi32, i32, u64, pu64, pu8]),
     "gbx_table_recognise": (i32, [i32, u64, u64, pu64, pu8, pi32]),
     "gbx_gen_gaussian": (i32, [u64, u64, u64, u64, pdbl]),

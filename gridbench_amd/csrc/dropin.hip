@@ -14,6 +14,9 @@
 // all imaginary parts, per vector; bench.cc:105-109), and vectors may be
 // expanded: nsimd lanes holding nsimd/dsimd replicas of a dsimd-lane lattice
 // (bench.cc:1-3, 98-138), run as independent problems.
+// Device buffers and objects live in a context (gbx_dslash_ctx): one per
+// caller that wants its own stream, or the process-wide one behind the
+// context-free entries.
 // Timing follows the reference: nrep applications, the clock covering
 // applications 1..nrep-1 (WilsonKernelsHandCpu.h:407-450), measured here
 // with HIP events on the stream the kernels run on.
@@ -97,10 +100,14 @@ bool overlaps(const void *a, size_t na, const void *b, size_t nb)
   return a0 < b0 + nb && b0 < a0 + na;
 }
 
-// Device buffers of the table path and SoA objects of the fast path, cached
-// across calls with the same shape (the reference benchmark calls once with
-// nrep = 1000; tests call many times with small shapes).
-struct DropinCache {
+}  // namespace
+
+// A drop-in context: the device buffers of the table path and the SoA
+// objects of the fast path, cached across calls with the same shape (the
+// reference benchmark calls once with nrep = 1000; tests call many times with
+// small shapes), and the stream its work runs on.  Contexts share nothing,
+// so calls on different contexts may run from different host threads.
+struct gbx_dslash_ctx_s {
   std::mutex mu;
   void *dU = nullptr, *dIn = nullptr, *dOut = nullptr, *dNbr = nullptr, *dPrm = nullptr;
   size_t nU = 0, nIn = 0, nOut = 0, nNbr = 0, nPrm = 0;
@@ -108,8 +115,11 @@ struct DropinCache {
   gbx_gauge gauge = nullptr;
   gbx_spinor sin = nullptr, sout = nullptr;
   int dims[4] = {0, 0, 0, 0}, Ls = 0, prec = -1;
+  hipStream_t st = nullptr;  // table path; the fast path runs on its geometry's stream
+  int last_path = 0;
 };
-DropinCache g_cache;
+
+namespace {
 
 int ensure(void **p, size_t *have, size_t need)
 {
@@ -122,9 +132,8 @@ int ensure(void **p, size_t *have, size_t need)
   return GBX_OK;
 }
 
-int fast_objects(const int dims[4], int Ls, int prec)
+int fast_objects(gbx_dslash_ctx_s &c, const int dims[4], int Ls, int prec)
 {
-  DropinCache &c = g_cache;
   if (c.geom && c.prec == prec && c.Ls == Ls && !memcmp(c.dims, dims, sizeof c.dims)) return GBX_OK;
   gbx_spinor_destroy(c.sin);
   gbx_spinor_destroy(c.sout);
@@ -179,10 +188,9 @@ bool pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
 // nsimd = lanes per host vector (EXPAND_SIMD), dsimd = lanes the tables and
 // permutes are for (DATA_SIMD, bench.cc:1-3); rrii = the RRII lane layout.
 template <typename T>
-int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const T *U, T *out, const T *in,
-                       const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+int dslash_kernel_impl(gbx_dslash_ctx_s &c, int nrep, int nsimd, int dsimd, int rrii, int dag, const T *U, T *out,
+                       const T *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  g_last_path = 0;
   GBX_CHECK(U && out && in && nbr && prm, GBX_ENULL, "dslash_kernel: NULL array");
   GBX_CHECK(nrep >= 1, GBX_EINVAL, "dslash_kernel: nrep=%d", nrep);
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dslash_kernel: dag=%d", dag);
@@ -204,8 +212,7 @@ int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const 
             (long long)bad, (unsigned long long)nbr[bad < 0 ? 0 : bad], (unsigned)prm[bad < 0 ? 0 : bad],
             (unsigned long long)n5);
   GBX_TRY(gbx_require_device());
-  std::lock_guard<std::mutex> lk(g_cache.mu);
-  DropinCache &c = g_cache;
+  if (!c.st) GBX_HIP(hipStreamCreate(&c.st));
   GBX_TRY(ensure(&c.dU, &c.nU, nu));
   GBX_TRY(ensure(&c.dIn, &c.nIn, nf));
   GBX_TRY(ensure(&c.dOut, &c.nOut, nf));
@@ -214,7 +221,7 @@ int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const 
   gbx_clear_error();
   if (canon == GBX_OK) {
     // The SoA kernel, once per replica of the dsimd-lane lattice.
-    GBX_TRY(fast_objects(dims, (int)Ls, RefPrec<T>::prec));
+    GBX_TRY(fast_objects(c, dims, (int)Ls, RefPrec<T>::prec));
     hipStream_t st = c.geom->stream;
     GBX_HIP(hipMemcpyAsync(c.dU, U, nu, hipMemcpyHostToDevice, st));
     GBX_HIP(hipMemcpyAsync(c.dIn, in, nf, hipMemcpyHostToDevice, st));
@@ -236,17 +243,17 @@ int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const 
     GBX_HIP(hipMemcpyAsync(out, c.dOut, nf, hipMemcpyDeviceToHost, st));
     GBX_HIP(hipStreamSynchronize(st));
     if (usec) *usec = total;
-    g_last_path = 1;
+    c.last_path = 1;
     return GBX_OK;
   }
   // Generic table-driven path.
   GBX_TRY(ensure(&c.dNbr, &c.nNbr, nt * 8));
   GBX_TRY(ensure(&c.dPrm, &c.nPrm, nt));
-  hipStream_t st = nullptr;
-  GBX_HIP(hipMemcpy(c.dU, U, nu, hipMemcpyHostToDevice));
-  GBX_HIP(hipMemcpy(c.dIn, in, nf, hipMemcpyHostToDevice));
-  GBX_HIP(hipMemcpy(c.dNbr, nbr, nt * 8, hipMemcpyHostToDevice));
-  GBX_HIP(hipMemcpy(c.dPrm, prm, nt, hipMemcpyHostToDevice));
+  hipStream_t st = c.st;
+  GBX_HIP(hipMemcpyAsync(c.dU, U, nu, hipMemcpyHostToDevice, st));
+  GBX_HIP(hipMemcpyAsync(c.dIn, in, nf, hipMemcpyHostToDevice, st));
+  GBX_HIP(hipMemcpyAsync(c.dNbr, nbr, nt * 8, hipMemcpyHostToDevice, st));
+  GBX_HIP(hipMemcpyAsync(c.dPrm, prm, nt, hipMemcpyHostToDevice, st));
   const uint64_t nthr = n5 * (uint64_t)nsimd;
   const dim3 grid((unsigned)((nthr + 255) / 256)), block(256);
   auto launch = [&]() -> int {
@@ -262,9 +269,51 @@ int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const 
     return GBX_OK;
   };
   GBX_TRY(timed_reps(nrep, st, launch, usec));
-  GBX_HIP(hipMemcpy(out, c.dOut, nf, hipMemcpyDeviceToHost));
-  g_last_path = 2;
+  GBX_HIP(hipMemcpyAsync(out, c.dOut, nf, hipMemcpyDeviceToHost, st));
+  GBX_HIP(hipStreamSynchronize(st));
+  c.last_path = 2;
   return GBX_OK;
+}
+
+void ctx_release(gbx_dslash_ctx_s &c)
+{
+  if (c.st) (void)hipStreamSynchronize(c.st);
+  for (void *p : {c.dU, c.dIn, c.dOut, c.dNbr, c.dPrm}) (void)hipFree(p);
+  c.dU = c.dIn = c.dOut = c.dNbr = c.dPrm = nullptr;
+  c.nU = c.nIn = c.nOut = c.nNbr = c.nPrm = 0;
+  gbx_spinor_destroy(c.sin);
+  gbx_spinor_destroy(c.sout);
+  gbx_gauge_destroy(c.gauge);
+  gbx_geom_destroy(c.geom);
+  c.sin = c.sout = nullptr;
+  c.gauge = nullptr;
+  c.geom = nullptr;
+  c.prec = -1;
+  if (c.st) (void)hipStreamDestroy(c.st);
+  c.st = nullptr;
+}
+
+// One call on a context: serialised on the context's lock; the path taken is
+// kept on the context and in this thread's gbx_dslash_last_path().
+template <typename T>
+int ctx_call(gbx_dslash_ctx_s *c, int nrep, int nsimd, int dsimd, int rrii, int dag, const T *U, T *out, const T *in,
+             const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  g_last_path = 0;
+  GBX_CHECK(c, GBX_ENULL, "dslash_kernel: NULL context");
+  std::lock_guard<std::mutex> lk(c->mu);
+  c->last_path = 0;
+  const int rc = dslash_kernel_impl<T>(*c, nrep, nsimd, dsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  g_last_path = c->last_path;
+  return rc;
+}
+
+// The context behind the context-free entries (created on first use, kept
+// for the life of the process).
+gbx_dslash_ctx_s *default_ctx()
+{
+  static gbx_dslash_ctx_s *c = new gbx_dslash_ctx_s();
+  return c;
 }
 
 }  // namespace
@@ -272,39 +321,78 @@ int dslash_kernel_impl(int nrep, int nsimd, int dsimd, int rrii, int dag, const 
 extern "C" int gbx_dslash_kernel_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
                                      const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<double>(nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<double>(default_ctx(), nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
                                      const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<float>(nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<float>(default_ctx(), nrep, nsimd, nsimd, 0, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_rrii_f64(int nrep, int nsimd, int dag, const double *U, double *out, const double *in,
                                           const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<double>(nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<double>(default_ctx(), nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_rrii_f32(int nrep, int nsimd, int dag, const float *U, float *out, const float *in,
                                           const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<float>(nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<float>(default_ctx(), nrep, nsimd, nsimd, 1, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_expand_f64(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const double *U,
                                             double *out, const double *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
                                             const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<double>(nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<double>(default_ctx(), nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_kernel_expand_f32(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const float *U,
                                             float *out, const float *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
                                             const uint8_t *prm, double *usec)
 {
-  return dslash_kernel_impl<float>(nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+  return ctx_call<float>(default_ctx(), nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
 }
 
 extern "C" int gbx_dslash_last_path(void) { return g_last_path; }
+
+extern "C" int gbx_dslash_ctx_create(gbx_dslash_ctx *ctx)
+{
+  GBX_CHECK(ctx, GBX_ENULL, "gbx_dslash_ctx_create: NULL argument");
+  *ctx = new gbx_dslash_ctx_s();
+  return GBX_OK;
+}
+
+extern "C" int gbx_dslash_ctx_destroy(gbx_dslash_ctx ctx)
+{
+  if (!ctx) return GBX_OK;
+  {
+    std::lock_guard<std::mutex> lk(ctx->mu);
+    ctx_release(*ctx);
+  }
+  delete ctx;
+  return GBX_OK;
+}
+
+extern "C" int gbx_dslash_ctx_kernel_f64(gbx_dslash_ctx ctx, int nrep, int nsimd, int data_nsimd, int rrii, int dag,
+                                         const double *U, double *out, const double *in, const uint64_t *nbr, uint64_t nsite,
+                                         uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return ctx_call<double>(ctx, nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_ctx_kernel_f32(gbx_dslash_ctx ctx, int nrep, int nsimd, int data_nsimd, int rrii, int dag,
+                                         const float *U, float *out, const float *in, const uint64_t *nbr, uint64_t nsite,
+                                         uint64_t Ls, const uint8_t *prm, double *usec)
+{
+  return ctx_call<float>(ctx, nrep, nsimd, data_nsimd, rrii, dag, U, out, in, nbr, nsite, Ls, prm, usec);
+}
+
+extern "C" int gbx_dslash_ctx_last_path(gbx_dslash_ctx ctx)
+{
+  if (!ctx) return 0;
+  std::lock_guard<std::mutex> lk(ctx->mu);
+  return ctx->last_path;
+}

@@ -83,16 +83,7 @@ def table_recognise(nsimd, nsite, Ls, nbr, prm):
 
 
 # ------------------------------------------------------------------- drop-in
-def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii=False, data_nsimd=None):
-    """The reference entry dslash_kernel<Simd>(nrep, U, out, in, nbr, nsite, Ls, prm)
-    (WilsonKernelsHand.h:17-29) on numpy arrays in the reference layout.
-    float64 arrays run the double kernel, float32 the single.  rrii=True
-    takes every array in the RRII lane layout (nsimd real parts, then nsimd
-    imaginary parts per vector, bench.cc:105-106) instead of RIRI.
-    data_nsimd < nsimd: vectors of nsimd lanes expanded from data_nsimd data
-    lanes, replica nn at lanes nn*data_nsimd + n, with the tables and
-    permutes of data_nsimd (bench.cc:1-3, 98-138).  Returns the microseconds
-    of applications 1..nrep-1, like the reference."""
+def _dslash_args(U, out, in_, nbr, nsite, Ls, prm):
     usec = C.c_double(0.0)
     if U.dtype == np.float64:
         ct, sfx = C.c_double, "f64"
@@ -107,6 +98,20 @@ def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii
         raise TypeError("nbr must be uint64 and prm uint8")
     args = (ptr(U, ct), ptr(out, ct), ptr(in_, ct), ptr(nbr, C.c_uint64), int(nsite), int(Ls), ptr(prm, C.c_uint8),
             C.byref(usec))
+    return sfx, args, usec
+
+
+def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii=False, data_nsimd=None):
+    """The reference entry dslash_kernel<Simd>(nrep, U, out, in, nbr, nsite, Ls, prm)
+    (WilsonKernelsHand.h:17-29) on numpy arrays in the reference layout.
+    float64 arrays run the double kernel, float32 the single.  rrii=True
+    takes every array in the RRII lane layout (nsimd real parts, then nsimd
+    imaginary parts per vector, bench.cc:105-106) instead of RIRI.
+    data_nsimd < nsimd: vectors of nsimd lanes expanded from data_nsimd data
+    lanes, replica nn at lanes nn*data_nsimd + n, with the tables and
+    permutes of data_nsimd (bench.cc:1-3, 98-138).  Returns the microseconds
+    of applications 1..nrep-1, like the reference."""
+    sfx, args, usec = _dslash_args(U, out, in_, nbr, nsite, Ls, prm)
     if data_nsimd is not None and int(data_nsimd) != int(nsimd):
         fn = getattr(lib, "gbx_dslash_kernel_expand_" + sfx)
         rc = fn(int(nrep), int(nsimd), int(data_nsimd), int(bool(rrii)), int(bool(dag)), *args)
@@ -120,6 +125,44 @@ def dslash_kernel(nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii
 def dslash_last_path():
     """1 = the last dslash_kernel call ran the SoA kernel, 2 = the table-driven kernel."""
     return lib.gbx_dslash_last_path()
+
+
+class DslashContext:
+    """A drop-in context (gbx_dslash_ctx): its own cached device buffers and
+    stream.  Calls on different contexts may come from different threads."""
+
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib.gbx_dslash_ctx_create(C.byref(h)), "gbx_dslash_ctx_create")
+        self.h = h
+
+    def kernel(self, nrep, U, out, in_, nbr, nsite, Ls, prm, nsimd, dag=False, rrii=False, data_nsimd=None):
+        """As dslash_kernel(), on this context."""
+        sfx, args, usec = _dslash_args(U, out, in_, nbr, nsite, Ls, prm)
+        fn = getattr(lib, "gbx_dslash_ctx_kernel_" + sfx)
+        dn = int(nsimd if data_nsimd is None else data_nsimd)
+        check(fn(self.h, int(nrep), int(nsimd), dn, int(bool(rrii)), int(bool(dag)), *args), "gbx_dslash_ctx_kernel")
+        return usec.value
+
+    def last_path(self):
+        return lib.gbx_dslash_ctx_last_path(self.h)
+
+    def close(self):
+        if self.h:
+            lib.gbx_dslash_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ----------------------------------------------------------- device objects
@@ -550,8 +593,8 @@ def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
 __all__ = [
     "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
-    "dslash_kernel", "dslash_last_path", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile", "dhop",
-    "dhop_axpby",
+    "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
+    "dhop", "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read",

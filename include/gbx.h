@@ -95,6 +95,25 @@ int gbx_dslash_kernel_expand_f64(int nrep, int nsimd, int data_nsimd, int rrii, 
 int gbx_dslash_kernel_expand_f32(int nrep, int nsimd, int data_nsimd, int rrii, int dag, const float *U, float *out,
                                  const float *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls, const uint8_t *prm,
                                  double *usec);
+/* Drop-in contexts.  A context owns the device buffers, the device-layout */
+/* objects and the stream of its calls, cached across calls of the same    */
+/* shape; the caller keeps owning every host array and no pointer to one   */
+/* is kept after a call returns.  Calls on one context are serialised;     */
+/* calls on different contexts share no state and may come from different */
+/* host threads.  The context-free entries above use one process-wide      */
+/* context.  Same arguments and results as gbx_dslash_kernel_expand_*      */
+/* (WilsonKernelsHand.h:17-29 with Nsimd, lane layout and dagger explicit).*/
+typedef struct gbx_dslash_ctx_s *gbx_dslash_ctx;
+int gbx_dslash_ctx_create(gbx_dslash_ctx *ctx);
+int gbx_dslash_ctx_destroy(gbx_dslash_ctx ctx);
+int gbx_dslash_ctx_kernel_f64(gbx_dslash_ctx ctx, int nrep, int nsimd, int data_nsimd, int rrii, int dag, const double *U,
+                              double *out, const double *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
+                              const uint8_t *prm, double *usec);
+int gbx_dslash_ctx_kernel_f32(gbx_dslash_ctx ctx, int nrep, int nsimd, int data_nsimd, int rrii, int dag, const float *U,
+                              float *out, const float *in, const uint64_t *nbr, uint64_t nsite, uint64_t Ls,
+                              const uint8_t *prm, double *usec);
+/* Path the context's last call took (1 SoA kernel, 2 table-driven, 0 none). */
+int gbx_dslash_ctx_last_path(gbx_dslash_ctx ctx);
 /* Which path the last gbx_dslash_kernel_* call of this thread took:
  * 1 = fast SoA kernel (tables recognised), 2 = table-driven kernel. */
 int gbx_dslash_last_path(void);

@@ -77,6 +77,33 @@ def test_dslash_kernel_argument_errors(gbx):
     assert b"table entry 7" in lib.gbx_last_error()
 
 
+def test_dslash_context_lifecycle_and_argument_errors(gbx):
+    """Contexts are created and destroyed without a device; a call checks its
+    arguments before it needs one."""
+    from gridbench_amd._lib import lib, ptr
+    U, out, inp, nbr, nsite, Ls, prm, nsimd = drop_in_args(gbx)
+    d, u64, u8 = C.c_double, C.c_uint64, C.c_uint8
+    us = C.c_double()
+    h = C.c_void_p()
+    assert lib.gbx_dslash_ctx_create(C.byref(h)) == 0 and h.value
+    assert lib.gbx_dslash_ctx_create(None) == 2
+
+    def call(ctx=h, ns=nsimd, dn=nsimd, rrii=0, o=out):
+        return lib.gbx_dslash_ctx_kernel_f64(ctx, 1, ns, dn, rrii, 0, ptr(U, d), ptr(o, d), ptr(inp, d), ptr(nbr, u64),
+                                             nsite, Ls, ptr(prm, u8), C.byref(us))
+    assert call(ctx=None) == 2               # NULL context
+    assert call(ns=3) == 1                   # nsimd
+    assert call(dn=2 * nsimd) == 1           # data lanes wider than the vector
+    assert call(rrii=2) == 1
+    assert call(o=inp) == 3                  # out aliases in
+    assert lib.gbx_dslash_ctx_last_path(h) == 0
+    assert lib.gbx_dslash_ctx_destroy(h) == 0
+    assert lib.gbx_dslash_ctx_destroy(None) == 0
+    with gbx.DslashContext() as ctx:
+        with pytest.raises(gbx.GbxError):
+            ctx.kernel(1, U, inp, inp, nbr, nsite, Ls, prm, nsimd)
+
+
 def test_table_and_generator_argument_errors(gbx):
     from gridbench_amd._lib import lib
     nbr = np.zeros(4096, np.uint64)

@@ -287,6 +287,50 @@ def test_dslash_kernel_expanded_simd(gpu, gbx, orc, data_nsimd, width, rrii, tab
         assert rel_l2(orc.replica_lanes(out, width, data_nsimd, nn), want) <= 1e-14
 
 
+def test_dslash_contexts_from_threads(gpu, gbx, orc):
+    """Two contexts driven from two host threads at once (one on the SoA
+    path, one on the table path, each with its own buffers and stream) give
+    the same bits as the process-wide context, and keep their own last path."""
+    import threading
+    dims, Ls = (4, 4, 4, 8), 3
+    A = ref_case(gbx, orc, dims, Ls, 4, np.float64)
+    B = ref_case(gbx, orc, dims, Ls, 2, np.float32)
+    Br = relabel_outer_sites(*B[:4], B[4], Ls)
+    jobs = [("A", 4, A[0], A[1], A[2], A[3], A[4], 1), ("B", 2, Br[0], Br[1], Br[2], Br[3], B[4], 2)]
+    want = {}
+    for name, ns, U, inp, nbr, prm, nsite, path in jobs:
+        out = np.zeros_like(inp)
+        gbx.dslash_kernel(2, U, out, inp, nbr, nsite, Ls, prm, ns, dag=True)
+        want[name] = out
+    got, errors = {}, []
+
+    def work(ctx, job):
+        name, ns, U, inp, nbr, prm, nsite, path = job
+        try:
+            for rep in range(5):
+                out = np.zeros_like(inp)
+                ctx.kernel(2, U, out, inp, nbr, nsite, Ls, prm, ns, dag=True)
+                assert np.array_equal(out, want[name]), (name, rep)
+                assert ctx.last_path() == path
+            got[name] = True
+        except Exception as e:  # reported below, in the main thread
+            errors.append(e)
+
+    ctxs = [gbx.DslashContext(), gbx.DslashContext()]
+    threads = [threading.Thread(target=work, args=(c, j)) for c, j in zip(ctxs, jobs)]
+    for t in threads:
+        t.start()
+    for t in threads:
+        t.join(timeout=120)
+    assert not errors, errors
+    assert got == {"A": True, "B": True}
+    # a context reused with another shape and precision
+    ctxs[0].kernel(2, Br[0], np.zeros_like(Br[1]), Br[1], Br[2], B[4], Ls, Br[3], 2)
+    assert ctxs[0].last_path() == 2
+    for c in ctxs:
+        c.close()
+
+
 def test_dslash_kernel_layout_argument_errors(gpu, gbx, orc):
     dims, Ls = (4, 4, 4, 4), 2
     U, inp, nbr, prm, nsite = ref_case(gbx, orc, dims, Ls, 2, np.float64)
