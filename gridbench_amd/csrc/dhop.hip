@@ -23,7 +23,7 @@ namespace {
 // and minor indices m0 .. m0+mc-1 (minor chunk L / nwg_sites): with mc < M
 // the lattice is swept once per minor chunk, so the 8-neighbour working set
 // is that of an mc-wide field (the links are re-read once per chunk).
-template <int P, bool DAG, bool C12, bool AXPBY, int PF>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false>
 __global__ void __launch_bounds__(PF ? 256 : 1024)
     k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
            void *__restrict__ out, const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mc,
@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(PF ? 256 : 1024)
   for (uint32_t w = threadIdx.x; w < items; w += blockDim.x) {
     const uint32_t sl = w / mcc, m = m0 + (w - sl * mcc);
     A rr[12], ri[12];
-    dev::dhop_site<P, DAG, PF>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    dev::dhop_site<P, DAG, PF, EXACT>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
     dev::store_site<P, AXPBY>(out, center, q0 + sl, M, m, rr, ri, alpha, beta, nt != 0);
   }
 }
@@ -71,11 +71,11 @@ struct LaunchArgs {
   double alpha, beta;
 };
 
-template <int P, bool DAG, bool C12, bool AXPBY, int PF>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false>
 void launch_t(const LaunchArgs &a)
 {
   typedef typename dev::Traits<P>::acc A;
-  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out, a.center,
+  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF, EXACT>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out, a.center,
                      a.nslot, a.M, a.spw, a.mc, a.nwg_sites, a.cblk, a.nchunk, a.nwg, a.per_xcd, a.xcd, a.nt, (A)a.alpha,
                      (A)a.beta);
 }
@@ -96,6 +96,21 @@ void launch_p(bool dag, bool c12, bool axpby, int pf, const LaunchArgs &a)
   else if (!dag && c12) launch_dc<P, false, true>(axpby, pf, a);
   else if (dag && !c12) launch_dc<P, true, false>(axpby, pf, a);
   else launch_dc<P, true, true>(axpby, pf, a);
+}
+
+// The reference's rounding (FP64 / FP32 storage, 18-real links, no axpby).
+template <int P>
+void launch_exact(bool dag, int pf, const LaunchArgs &a)
+{
+  if (dag) {
+    if (pf == 2) launch_t<P, true, false, false, 2, true>(a);
+    else if (pf == 1) launch_t<P, true, false, false, 1, true>(a);
+    else launch_t<P, true, false, false, 0, true>(a);
+  } else {
+    if (pf == 2) launch_t<P, false, false, false, 2, true>(a);
+    else if (pf == 1) launch_t<P, false, false, false, 1, true>(a);
+    else launch_t<P, false, false, false, 0, true>(a);
+  }
 }
 
 size_t lds_bytes(int prec, int spw) { return (size_t)spw * 8 * 18 * (prec == GBX_F64 ? 8 : 4) + (size_t)spw * 8 * 4; }
@@ -205,11 +220,13 @@ static int check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const g
 }
 
 int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const dhop_cfg &c, double alpha,
-                double beta, const gbx_spinor_s *center, hipStream_t st)
+                double beta, const gbx_spinor_s *center, hipStream_t st, bool exact)
 {
   GBX_TRY(check_operands(u, out, in));
   GBX_TRY(dhop_cfg_check(c, u, in));
   const bool axpby = !(alpha == 0.0 && beta == 1.0);
+  GBX_CHECK(!exact || ((u->prec == GBX_F64 || u->prec == GBX_F32) && !u->comp12 && !axpby), GBX_EUNSUPPORTED,
+            "dhop: the reference-rounding kernel takes FP64 / FP32 fields with 18-real links");
   if (axpby) {
     GBX_CHECK(center && center->d, GBX_ENULL, "dhop_axpby: NULL center field");
     GBX_CHECK(center->d != out->d, GBX_EALIAS, "dhop_axpby: out aliases the center field");
@@ -247,6 +264,12 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   a.alpha = alpha;
   a.beta = beta;
   const int pf = axpby ? 0 : c.pf;
+  if (exact) {
+    if (u->prec == GBX_F64) launch_exact<GBX_F64>(dag, pf, a);
+    else launch_exact<GBX_F32>(dag, pf, a);
+    GBX_HIP(hipGetLastError());
+    return GBX_OK;
+  }
   switch (u->prec) {
   case GBX_F64: launch_p<GBX_F64>(dag, u->comp12, axpby, pf, a); break;
   case GBX_F32: launch_p<GBX_F32>(dag, u->comp12, axpby, pf, a); break;
@@ -289,6 +312,14 @@ static dhop_cfg from_kcfg(const gbx_kcfg &k)
   c.wgpc = k.wgpc;
   c.cblk = k.cblk;
   return c;
+}
+
+int dhop_run_exact(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, hipStream_t st)
+{
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dhop: dag must be 0 or 1");
+  gbx_kcfg k;
+  const dhop_cfg c = dhop_lookup_cfg(dhop_signature(u, in), &k) && k.kind == 0 ? from_kcfg(k) : dhop_default_cfg(u, in);
+  return dhop_launch(u, out, in, dag, c, 0.0, 1.0, nullptr, st, true);
 }
 
 int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st)

@@ -59,16 +59,48 @@ __device__ __forceinline__ void project(const A *v, A *hr, A *hi)
 #undef PI
 }
 
+// The reference's arithmetic for U x chi: every complex product
+// (ar*br - ai*bi, ar*bi + ai*br) rounded term by term, summed over the
+// columns left to right, t = U_i0 chi_0; t += U_i1 chi_1; t += U_i2 chi_2
+// (WilsonKernelsHandCpu.h:62-90, arch/generic/Simd_generic.h:153-155).
+// No multiply-add is fused, so the result is the reference's to the bit.
+template <typename A>
+__device__ __forceinline__ void su3_mul_exact(const A *U, const A *hr, const A *hi, A *ur, A *ui)
+{
+#pragma clang fp contract(off)
+#pragma unroll
+  for (int sp = 0; sp < 2; sp++)
+#pragma unroll
+    for (int i = 0; i < 3; i++) {
+      A re = U[(i * 3) * 2] * hr[sp * 3] - U[(i * 3) * 2 + 1] * hi[sp * 3];
+      A im = U[(i * 3) * 2] * hi[sp * 3] + U[(i * 3) * 2 + 1] * hr[sp * 3];
+#pragma unroll
+      for (int j = 1; j < 3; j++) {
+        const A pr = U[(i * 3 + j) * 2] * hr[sp * 3 + j] - U[(i * 3 + j) * 2 + 1] * hi[sp * 3 + j];
+        const A pi = U[(i * 3 + j) * 2] * hi[sp * 3 + j] + U[(i * 3 + j) * 2 + 1] * hr[sp * 3 + j];
+        re = re + pr;
+        im = im + pi;
+      }
+      ur[sp * 3 + i] = re;
+      ui[sp * 3 + i] = im;
+    }
+}
+
 // u[sp*3+i] = sum_j U[i][j] h[sp*3+j]; U row-major complex, 18 reals.
-// CHAIN (FP64 and FP32 storage): one multiply + five fused multiply-adds per
-// real output (72 ops per leg, 840 VALU ops per 5D site instead of 1032 for
-// the mul/fma/add form the compiler derives from the plain expression).
-// !CHAIN (16-bit storage): the plain expression, whose register schedule
-// keeps those kernels free of spills.
-template <bool CHAIN, typename A>
+// MUL_CHAIN (FP64 and FP32 storage): one multiply + five fused multiply-adds
+// per real output (72 ops per leg, 840 VALU ops per 5D site instead of 1032
+// for the mul/fma/add form the compiler derives from the plain expression).
+// MUL_PLAIN (16-bit storage): the plain expression, whose register schedule
+// keeps those kernels free of spills.  MUL_EXACT: the reference's rounding.
+enum { MUL_PLAIN = 0, MUL_CHAIN = 1, MUL_EXACT = 2 };
+template <int MUL, typename A>
 __device__ __forceinline__ void su3_mul(const A *U, const A *hr, const A *hi, A *ur, A *ui)
 {
-  if constexpr (!CHAIN) {
+  if constexpr (MUL == MUL_EXACT) {
+    su3_mul_exact(U, hr, hi, ur, ui);
+    return;
+  }
+  if constexpr (MUL == MUL_PLAIN) {
 #pragma unroll
     for (int sp = 0; sp < 2; sp++)
 #pragma unroll
@@ -206,12 +238,12 @@ __device__ __forceinline__ void load_nb(const void *__restrict__ in, uint32_t nb
 }
 
 // project, multiply by U (18 reals in LDS), reconstruct into r
-template <int K, bool FIRST, bool CHAIN, typename A>
+template <int K, bool FIRST, int MUL, typename A>
 __device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri)
 {
   A hr[6], hi[6], ur[6], ui[6];
   project<K>(v, hr, hi);
-  su3_mul<CHAIN>(U, hr, hi, ur, ui);
+  su3_mul<MUL>(U, hr, hi, ur, ui);
   recon<K, FIRST>(ur, ui, rr, ri);
 }
 
@@ -219,14 +251,15 @@ __device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri
 // PF = 0: gather and compute leg by leg.  PF = 1: the gather of leg d+1 is
 // issued before leg d is computed (two gather buffers in registers), which
 // keeps a second set of loads in flight per wavefront.  PF = 2: three
-// buffers, gathers two legs ahead.
-template <int P, bool DAG, int PF>
+// buffers, gathers two legs ahead.  EXACT: the reference's rounding in the
+// SU(3) products (FP64 / FP32).
+template <int P, bool DAG, int PF, bool EXACT = false>
 __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
                                           const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
                                           typename Traits<P>::acc *ri)
 {
   typedef typename Traits<P>::acc A;
-  constexpr bool CH = (P == GBX_F64 || P == GBX_F32);
+  constexpr int CH = EXACT ? MUL_EXACT : (P == GBX_F64 || P == GBX_F32) ? MUL_CHAIN : MUL_PLAIN;
   if (PF == 0) {
     A v[24];
 #define GBX_LEG(D, FIRST)                                                  \

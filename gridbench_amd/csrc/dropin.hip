@@ -3,9 +3,11 @@
 //       Simd *in, uint64_t *nbr, uint64_t nsite, uint64_t Ls, uint8_t *prm)
 // (WilsonKernelsHand.h:17-29), on host arrays in the reference layout.
 //
-// Two device paths:
+// Both device paths compute with the reference's rounding (no fused
+// multiply-adds, its order of operations), so the result is the reference
+// CPU kernel's to the bit:
 //  * canonical tables (recognised by gbx_table_recognise): the fields are
-//    repacked into the SoA layout and the fast kernel (dhop.hip) runs;
+//    repacked into the SoA layout and the VALU kernel (dhop.hip) runs;
 //  * any other valid tables: a table-driven kernel on the reference layout
 //    honours nbr/prm exactly: thread = (5D outer site, lane), neighbour lane
 //    = lane ^ (DATA_SIMD >> (ptype+1)) when prm is set (the reference GPU
@@ -26,7 +28,6 @@
 #include "dhop_dev.h"
 #include "gbx_internal.h"
 
-int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
 int64_t table_first_invalid(const uint64_t *nbr, const uint8_t *prm, uint64_t n, uint64_t n5);
 
 namespace {
@@ -75,7 +76,7 @@ __global__ void __launch_bounds__(256) k_dhop_table(const T *__restrict__ U, T *
     }                                                                                                                    \
     T hr[6], hi[6], ur[6], ui[6];                                                                                        \
     dev::project<dev::LegProj<DAG, DIR>::value>(v, hr, hi);                                                              \
-    dev::su3_mul<true>(Ul, hr, hi, ur, ui);                                                                                    \
+    dev::su3_mul<dev::MUL_EXACT>(Ul, hr, hi, ur, ui);                                                                        \
     dev::recon<dev::LegProj<DAG, DIR>::value, FIRST>(ur, ui, rr, ri);                                                    \
   }
   GBX_TLEG(0, 3, true)
@@ -225,8 +226,6 @@ int dslash_kernel_impl(gbx_dslash_ctx_s &c, int nrep, int nsimd, int dsimd, int 
     hipStream_t st = c.geom->stream;
     GBX_HIP(hipMemcpyAsync(c.dU, U, nu, hipMemcpyHostToDevice, st));
     GBX_HIP(hipMemcpyAsync(c.dIn, in, nf, hipMemcpyHostToDevice, st));
-    gbx_kcfg k;
-    gbx_dhop_get_cfg(c.gauge, c.sin, &k);
     double total = 0.0;
     for (int nn = 0; nn < nsimd / dsimd; nn++) {
       gbx_ref_layout lay;
@@ -236,7 +235,7 @@ int dslash_kernel_impl(gbx_dslash_ctx_s &c, int nrep, int nsimd, int dsimd, int 
       GBX_TRY(ref_pack_gauge_dev(c.gauge, (const T *)c.dU, dsimd, lay));
       GBX_TRY(ref_pack_spinor_dev(c.sin, (const T *)c.dIn, dsimd, lay));
       double us = 0.0;
-      GBX_TRY(timed_reps(nrep, st, [&]() { return dhop_run(c.gauge, c.sout, c.sin, dag, k, st); }, &us));
+      GBX_TRY(timed_reps(nrep, st, [&]() { return dhop_run_exact(c.gauge, c.sout, c.sin, dag, st); }, &us));
       total += us;
       GBX_TRY(ref_unpack_spinor_dev(c.sout, (T *)c.dOut, dsimd, lay));
     }

@@ -152,9 +152,13 @@ struct dhop_cfg {
 };
 
 // out = alpha*in_c + beta*Dhop(in) where in_c is `center` (may be null when alpha == 0).
+// exact: the reference's rounding (no fused multiply-adds; FP64 / FP32 fields
+// with 18-real links, no axpby).
 int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag,
                 const dhop_cfg &cfg, double alpha, double beta, const gbx_spinor_s *center,
-                hipStream_t stream);
+                hipStream_t stream, bool exact = false);
+// Dhop with the reference's rounding, on the problem's tuned VALU shape.
+int dhop_run_exact(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, hipStream_t st);
 dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in);
 int dhop_cfg_check(const dhop_cfg &cfg, const gbx_gauge_s *u, const gbx_spinor_s *in);
 // LDS-tiled kernel (kind 2): out = Dhop(in) on a whole lattice.

@@ -115,7 +115,7 @@ __global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArg
   // 12 chunks are padded to 13 so that 8 consecutive lanes' 16-byte reads
   // fall in distinct banks.
   constexpr int SW = NCH == 12 ? 13 : NCH;
-  constexpr bool CH = (P == GBX_F64 || P == GBX_F32);
+  constexpr int CH = (P == GBX_F64 || P == GBX_F32) ? dev::MUL_CHAIN : dev::MUL_PLAIN;
   const int e[4] = {a.ex, a.ey, a.ez, a.et};
   const int T = e[0] * e[1] * e[2] * e[3];
   extern __shared__ __align__(16) unsigned char smem[];
