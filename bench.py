@@ -102,6 +102,30 @@ def cpu_baseline(dbl, psi, threads, budget_s=12.0):
             "host": host_info()}
 
 
+def cpu_reference_build(dbl, psi, budget_s=10.0):
+    """The reference's own CPU Hand kernel (dslash_kernel_cpu compiled from its
+    sources into oracle/_ref, generic backend, Nsimd 1, one thread: the build
+    has no OpenMP) on the same inputs, timed by its own clock; None when that
+    library was not built in this tree."""
+    from oracle import oracle as orc
+    V = int(np.prod(DIMS))
+    nbr, prm = orc.tables(DIMS, 1, LS)
+    U = orc.pack_gauge(DIMS, 1, dbl)
+    inp = orc.pack_spinor(DIMS, 1, LS, psi)
+    t0 = time.perf_counter()
+    us1 = orc.ref_time(U, inp, nbr, prm, V, LS, 1, 2)  # rep 0 untimed, rep 1 timed
+    if us1 is None:
+        return None
+    per = max(time.perf_counter() - t0, 1e-3) / 2
+    nrep = max(2, min(20, int(budget_s / per)))
+    us = orc.ref_time(U, inp, nbr, prm, V, LS, 1, nrep)
+    dt = us * 1e-6 / (nrep - 1)
+    return {"value": round(1320.0 * V * LS / dt / 1e9, 3), "unit": "GFLOP/s", "cores": 1, "kind": "reference",
+            "sample": f"24^4 x Ls16 FP64 1-RHS, the reference's dslash_kernel_cpu (GEN backend, Nsimd 1, "
+                      f"-O2 -ffp-contract=off), nrep={nrep}, its timer over reps 1..{nrep - 1}",
+            "ms_per_application": round(dt * 1e3, 2)}
+
+
 def host_info():
     """The host the CPU baseline ran on."""
     model = None
@@ -490,12 +514,14 @@ def main():
     if not args.no_batch:
         extra["batch"] = batch_section(gbx, 20)
 
-    cpu = None
+    cpu = ref_cpu = None
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, os.cpu_count() or 1)
         log(f"CPU baseline with {threads} threads")
         cpu = cpu_baseline(dbl, psi, threads)
         log(f"CPU baseline: {cpu}")
+        ref_cpu = cpu_reference_build(dbl, psi)
+        log(f"reference build on the CPU: {ref_cpu}")
 
     result = {
         "metric": "DWF Dhop GFLOP/s + HBM GB/s @24^4xLs16: FP64 1-RHS vs FP32/FP16 16-RHS",
@@ -519,6 +545,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profiles(),
                      "algorithmic_bytes_per_launch": bytes_alg, "us_per_launch_events": round(us_ev, 2)},
         "cpu_baseline": cpu,
+        "cpu_reference_build": ref_cpu,
         **extra,
     }
     print(json.dumps(result), flush=True)

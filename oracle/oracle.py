@@ -225,6 +225,21 @@ def ref_hand(U, inp, nbr, prm, nsite, Ls, nsimd, nrep=1):
     return out
 
 
+def ref_time(U, inp, nbr, prm, nsite, Ls, nsimd, nrep):
+    """Microseconds the reference's own dslash_kernel_cpu reports for nrep
+    applications (its timer covers reps 1..nrep-1), or None if not built."""
+    L = ref_lib(ref_width_for(nsimd, U.dtype))
+    if L is None:
+        return None
+    out = aligned_empty(inp.size, inp.dtype)
+    U, inp = aligned_copy(U), aligned_copy(inp)
+    if U.dtype == np.float64:
+        return L.gbxref_dslash_d(int(nrep), _p(U, C.c_double), _p(out, C.c_double), _p(inp, C.c_double),
+                                 _p(nbr, C.c_uint64), nsite, Ls, _p(prm, C.c_uint8))
+    return L.gbxref_dslash_f(int(nrep), _p(U, C.c_float), _p(out, C.c_float), _p(inp, C.c_float), _p(nbr, C.c_uint64),
+                             nsite, Ls, _p(prm, C.c_uint8))
+
+
 def ref_simple(U, inp, nbr, prm, nsite, Ls):
     """The reference's scalar dslash_simple.h kernel (Nsimd = 1, FP64), or None."""
     L = ref_lib(16)
