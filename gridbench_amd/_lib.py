@@ -121,6 +121,7 @@ PROTOTYPES = {
     "gbx_dhop": (i32, [vp, vp, vp, i32]),
     "gbx_dhop_axpby": (i32, [vp, vp, vp, i32, dbl, dbl]),
     "gbx_dhop_xpay": (i32, [vp, vp, vp, vp, i32, dbl, dbl]),
+    "gbx_dhop_xpay_cfg": (i32, [vp, vp, vp, vp, i32, dbl, dbl, C.POINTER(KCfg)]),
     "gbx_dhop_cfg": (i32, [vp, vp, vp, i32, C.POINTER(KCfg)]),
     "gbx_dhop_get_cfg": (i32, [vp, vp, C.POINTER(KCfg)]),
     "gbx_dhop_time": (i32, [vp, vp, vp, i32, C.POINTER(KCfg), i32, pdbl]),

@@ -83,8 +83,11 @@ void launch_t(const LaunchArgs &a)
 template <int P, bool DAG, bool C12>
 void launch_dc(bool axpby, int pf, const LaunchArgs &a)
 {
-  if (axpby) launch_t<P, DAG, C12, true, 0>(a);
-  else if (pf == 2) launch_t<P, DAG, C12, false, 2>(a);
+  if (axpby) {
+    if (pf == 2) launch_t<P, DAG, C12, true, 2>(a);
+    else if (pf == 1) launch_t<P, DAG, C12, true, 1>(a);
+    else launch_t<P, DAG, C12, true, 0>(a);
+  } else if (pf == 2) launch_t<P, DAG, C12, false, 2>(a);
   else if (pf == 1) launch_t<P, DAG, C12, false, 1>(a);
   else launch_t<P, DAG, C12, false, 0>(a);
 }
@@ -263,7 +266,7 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   a.nt = c.nt;
   a.alpha = alpha;
   a.beta = beta;
-  const int pf = axpby ? 0 : c.pf;
+  const int pf = (axpby && !c.axpby_pf) ? 0 : c.pf;
   if (exact) {
     if (u->prec == GBX_F64) launch_exact<GBX_F64>(dag, pf, a);
     else launch_exact<GBX_F32>(dag, pf, a);
@@ -362,6 +365,19 @@ extern "C" int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spin
   gbx_dhop_get_cfg(u, in, &k);
   if (k.kind != 0) k = to_kcfg(dhop_default_cfg(u, in));
   return dhop_launch(u, out, in, dag, from_kcfg(k), alpha, beta, x, u->geom->stream);
+}
+
+extern "C" int gbx_dhop_xpay_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha,
+                                 double beta, const gbx_kcfg *cfg)
+{
+  GBX_CHECK(cfg, GBX_ENULL, "gbx_dhop_xpay_cfg: NULL config");
+  GBX_TRY(check_operands(u, out, in));
+  GBX_CHECK(x, GBX_ENULL, "gbx_dhop_xpay_cfg: NULL field x");
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_dhop_xpay_cfg: dag must be 0 or 1");
+  GBX_CHECK(cfg->kind == 0, GBX_EUNSUPPORTED, "gbx_dhop_xpay_cfg: the fused form runs on the VALU kernel (kind 0)");
+  dhop_cfg c = from_kcfg(*cfg);
+  c.axpby_pf = true;
+  return dhop_launch(u, out, in, dag, c, alpha, beta, x, u->geom->stream);
 }
 
 extern "C" int gbx_dhop_axpby(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, double alpha, double beta)

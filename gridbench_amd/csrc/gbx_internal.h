@@ -149,6 +149,10 @@ struct dhop_cfg {
   int pf;        // 1: two-buffer gather pipeline (workgroups of <= 256 threads)
   int wgpc;      // cap on resident workgroups per CU via padded LDS (0 = none)
   int cblk;      // site groups per minor-chunk cycle (0 = whole lattice per chunk)
+  // The fused axpby form runs without the gather pipeline (its tuned pf is
+  // that of the plain kernel, and the extra centre loads make the pipeline
+  // slower there) unless the caller asked for a pf explicitly.
+  bool axpby_pf = false;
 };
 
 // out = alpha*in_c + beta*Dhop(in) where in_c is `center` (may be null when alpha == 0).

@@ -375,9 +375,14 @@ def dhop(U, out, inp, dag=False, cfg=None):
     return out
 
 
-def dhop_xpay(U, out, x, inp, alpha, beta, dag=False):
-    """out = alpha*x + beta*Dhop(inp), x on out's geometry (red-black halves too)."""
-    check(lib.gbx_dhop_xpay(U.h, out.h, x.h, inp.h, int(bool(dag)), float(alpha), float(beta)), "gbx_dhop_xpay")
+def dhop_xpay(U, out, x, inp, alpha, beta, dag=False, cfg=None):
+    """out = alpha*x + beta*Dhop(inp), x on out's geometry (red-black halves too);
+    cfg: an explicit VALU-kernel configuration."""
+    if cfg is None:
+        check(lib.gbx_dhop_xpay(U.h, out.h, x.h, inp.h, int(bool(dag)), float(alpha), float(beta)), "gbx_dhop_xpay")
+    else:
+        check(lib.gbx_dhop_xpay_cfg(U.h, out.h, x.h, inp.h, int(bool(dag)), float(alpha), float(beta), C.byref(cfg)),
+              "gbx_dhop_xpay_cfg")
 
 
 def dhop_axpby(U, out, inp, alpha, beta, dag=False):

@@ -254,6 +254,9 @@ typedef struct {
                /* the lattice); legs inside the box read LDS             */
 } gbx_kcfg;
 int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg);
+/* gbx_dhop_xpay with an explicit VALU-kernel configuration (kind 0).     */
+int gbx_dhop_xpay_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta,
+                      const gbx_kcfg *cfg);
 /* Configuration gbx_dhop would use now (autotuned if gbx_tune ran). */
 int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg);
 /* Average device time per application over nrep applications after one  */

@@ -106,7 +106,15 @@ def test_axpby(gpu, gbx, problem):
     for dag in (False, True):
         gbx.dhop_axpby(U, dst, src, 1.0, -0.12, dag=dag)
         want = p.psi[0] - 0.12 * p.oracle_lex(dag=dag)
-        assert rel_l2(dst.download()[0], want) <= 1e-14
+        got = dst.download()[0]
+        assert rel_l2(got, want) <= 1e-14
+        # the fused form on every gather pipeline depth: the same bits
+        for pf in (0, 1, 2):
+            gbx.dhop_xpay(U, dst, src, src, 1.0, -0.12, dag=dag, cfg=gbx.kcfg(4, 64, 1, 0, pf=pf))
+            assert np.array_equal(dst.download()[0], got), pf
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.dhop_xpay(U, dst, src, src, 1.0, -0.12, cfg=gbx.kcfg(4, 64, kind=2, tile=(2, 2, 2, 2)))
+    assert e.value.status == 8
 
 
 def test_errors_on_device_objects(gpu, gbx, problem):
