@@ -30,6 +30,12 @@ Also reported, each with its own seeds:
              {8,12,16}, Nrhs in 1..8, drawn with seed 505), FP32, one
              persistent graph-captured launch; aggregate GFLOP/s and the
              per-lattice error against FP64.
+  halo       24^4 x Ls16 FP64 cut along T into 2 and 4 slabs (domain parts
+             with ghost slots): the assembled result against the whole
+             lattice's, and for one slab the Dhop time on its share, on its
+             interior and on its faces, the face pack / unpack times and the
+             face bytes; the xGMI transfer time is modelled (bytes / link
+             rate), not measured.
 
 Launch configurations come from the autotuner; its results for this machine
 are cached in gridbench_amd/tuned_mi355x.jsonl (re-validated on every run).
@@ -432,6 +438,86 @@ def batch_section(gbx, nrep):
     return r
 
 
+XGMI_GBPS = 153.6  # one MI355X xGMI link, AMD-published per-link figure (direction split not stated)
+
+
+def host_timed(fn, geom, nrep):
+    """Microseconds per call from the host clock around nrep enqueued calls
+    (every call's device work is far longer than its launch)."""
+    fn()
+    geom.sync()
+    t0 = time.perf_counter()
+    for _ in range(nrep):
+        fn()
+    geom.sync()
+    return (time.perf_counter() - t0) * 1e6 / nrep
+
+
+def halo_section(gbx, tune, dbl, psi, ref64, nrep):
+    """One GPU's share of a domain-decomposed 24^4 x Ls16 FP64 Dhop: the
+    lattice cut along T into 2 and 4 slabs.  Every slab is built and its
+    faces exchanged on this GPU (checks the assembled result against the
+    whole-lattice application); part 0 is timed: Dhop on its whole share,
+    on the interior slots alone, on the face slots alone, and the face
+    pack / unpack kernels.  The xGMI transfer itself needs two GPUs: its
+    time is the face bytes at one link's published rate (a model, labelled
+    as such)."""
+    rows = []
+    for nparts in (2, 4):
+        geoms = [gbx.Geometry.part_of(DIMS, 3, nparts, k) for k in range(nparts)]
+        loc = gbx.part_split(DIMS, 3, nparts, psi[None])
+        U = [gbx.GaugeField(g, "f64").upload(dbl) for g in geoms]
+        x = [gbx.SpinorField(g, LS, 1, "f64").upload(a) for g, a in zip(geoms, loc)]
+        y = [gbx.SpinorField(g, LS, 1, "f64") for g in geoms]
+        del loc
+        best, _ = tune(U[0], y[0], x[0])
+        nb = x[0].halo_bytes(proj=True)
+        bufs = [(gbx.DeviceBuffer(nb), gbx.DeviceBuffer(nb)) for _ in range(nparts)]
+        gbx.halo_exchange(x, proj=True, bufs=bufs)
+        for u_, y_, x_ in zip(U, y, x):
+            gbx.dhop(u_, y_, x_)
+        got = gbx.part_merge(DIMS, 3, [y_.download() for y_ in y])[0]
+        err = float(np.linalg.norm(got - ref64) / np.linalg.norm(ref64))
+        del got
+        g0, u0, x0, y0 = geoms[0], U[0], x[0], y[0]
+        us_part = gbx.dhop_time(u0, y0, x0, nrep=nrep)
+        nint, vol = g0.ninterior, g0.volume
+        us_int = host_timed(lambda: gbx.dhop_range(u0, y0, x0, 0, nint), g0, nrep)
+        us_bnd = host_timed(lambda: gbx.dhop_range(u0, y0, x0, nint, vol), g0, nrep)
+        us_pack = host_timed(lambda: x0.halo_pack(1, x[1].halo_ghost(0)), g0, nrep)
+        us_pack_proj = host_timed(lambda: x0.halo_pack(1, bufs[0][1].ptr, proj=True), g0, nrep)
+        us_unpack_proj = host_timed(lambda: x0.halo_unpack(0, bufs[1][1].ptr, proj=True), g0, nrep)
+        fb, fbp = x0.halo_bytes(), nb
+        # per application a part sends two faces and receives two: with two
+        # parts both go to the one neighbour (one link), with more parts to
+        # two neighbours over two links at once
+        faces_per_link = 2 if nparts == 2 else 1
+        xfer_us = faces_per_link * fbp / (XGMI_GBPS * 1e9) * 1e6
+        comm_us = 2 * us_pack_proj + xfer_us + 2 * us_unpack_proj
+        overlapped = max(us_int, comm_us) + us_bnd
+        r = {"nparts": nparts, "split_dim": "T", "local_dims": list(g0.dims), "local_sites": vol,
+             "interior_sites": nint, "face_sites": g0.face, "kernel_cfg": best.as_dict(),
+             "rel_l2_assembled_vs_whole_lattice": float(f"{err:.3e}"),
+             "us_dhop_local_share_events": round(us_part, 2), "us_dhop_interior_host": round(us_int, 2),
+             "us_dhop_faces_host": round(us_bnd, 2), "face_bytes_full": fb, "face_bytes_projected": fbp,
+             "us_pack_full_face": round(us_pack, 2), "us_pack_projected_face": round(us_pack_proj, 2),
+             "us_unpack_projected_face": round(us_unpack_proj, 2),
+             "model_xgmi_gbps_per_link": XGMI_GBPS, "model_projected_faces_per_link": faces_per_link,
+             "model_us_xgmi": round(xfer_us, 1),
+             "model_us_per_application_overlapped": round(overlapped, 1),
+             "gflops_local_share": round(1320.0 * vol * LS / (us_part * 1e-6) / 1e9, 1)}
+        log(f"halo {r}")
+        rows.append(r)
+        for f in [*bufs]:
+            f[0].close()
+            f[1].close()
+        for f in (*y, *x, *U):
+            f.close()
+    return {"dims": list(DIMS), "Ls": LS, "prec": "f64",
+            "note": "one GPU holds every slab; the xGMI time is the model bytes / link rate, not a measurement",
+            "splits": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -442,7 +528,9 @@ def main():
     ap.add_argument("--no-mixed", action="store_true", help="skip the 32^4 mixed-precision errors")
     ap.add_argument("--no-cg", action="store_true", help="skip the defect-correction CG")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
-    ap.add_argument("--headline-only", action="store_true", help="= --no-variants --no-mixed --no-cg --no-batch")
+    ap.add_argument("--no-halo", action="store_true", help="skip the domain-part halo section")
+    ap.add_argument("--headline-only", action="store_true",
+                    help="= --no-variants --no-mixed --no-cg --no-batch --no-halo")
     ap.add_argument("--only", default="", help="comma-separated variant names to run (profiling)")
     ap.add_argument("--no-compare", action="store_true", help="skip the MFMA-vs-VALU form comparison (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -451,11 +539,11 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the hopping term on one "
-              "GPU's lattice is the whole measured path; the multi-GPU halo-exchange design is described in "
-              "DESIGN.md but not built).", file=sys.stderr)
+              "GPU's lattice is the whole measured path; one GPU's slab of a domain decomposition, with its "
+              "face pack / unpack, is the 'halo' section; transfers between GPUs are not run).", file=sys.stderr)
         sys.exit(2)
     if args.headline_only:
-        args.no_variants = args.no_mixed = args.no_cg = args.no_batch = True
+        args.no_variants = args.no_mixed = args.no_cg = args.no_batch = args.no_halo = True
 
     import gridbench_amd as gbx
     if gbx.device_count() < 1:
@@ -513,6 +601,8 @@ def main():
         extra["dwf"] = dwf_section(gbx, tune)
     if not args.no_batch:
         extra["batch"] = batch_section(gbx, 20)
+    if not args.no_halo:
+        extra["halo"] = halo_section(gbx, tune, dbl, psi, ref64, 100)
 
     cpu = ref_cpu = None
     if not args.no_cpu:

@@ -343,7 +343,8 @@ int check_args(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, const gbx_spino
     GBX_CHECK(u_lo->geom == u_hi->geom, GBX_EMISMATCH, "cg: inner gauge field on another geometry");
     GBX_CHECK(u_lo->prec != GBX_F64, GBX_EINVAL, "cg: the inner gauge field must be FP32, FP16 or BF16");
   }
-  GBX_CHECK(u_hi->geom->parity < 0, GBX_EINVAL, "cg: gbx_cg_solve works on a whole lattice; use gbx_cg_solve_eo on a red-black pair");
+  GBX_CHECK(whole_lattice(u_hi->geom), GBX_EINVAL,
+            "cg: gbx_cg_solve works on a whole lattice; use gbx_cg_solve_eo on a red-black pair");
   return check_params(p);
 }
 

@@ -132,6 +132,7 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
            g->dims[2], g->dims[3]);
   std::string s = buf;
   if (g->parity >= 0) s += g->parity ? "_odd" : "_even";
+  if (g->nghost) s += "_part" + std::to_string(g->split_dim);  // ghost slots: no tiled kernel
   for (int l = 0; l < g->nlevels; l++) {
     snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
              g->levels[l * 4 + 3]);
@@ -223,7 +224,7 @@ static int check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const g
 }
 
 int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const dhop_cfg &c, double alpha,
-                double beta, const gbx_spinor_s *center, hipStream_t st, bool exact)
+                double beta, const gbx_spinor_s *center, hipStream_t st, bool exact, SlotRange range)
 {
   GBX_TRY(check_operands(u, out, in));
   GBX_TRY(dhop_cfg_check(c, u, in));
@@ -236,8 +237,11 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
     GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == out->geom, GBX_EMISMATCH,
               "dhop_axpby: the added field must have the output's geometry, precision and shape");
   }
+  const uint64_t q0 = range.begin, q1 = std::min<uint64_t>(range.end, u->geom->vol);
+  GBX_CHECK(q0 <= q1, GBX_EINVAL, "dhop: slot range %llu .. %llu", (unsigned long long)q0, (unsigned long long)q1);
+  if (q0 == q1) return GBX_OK;
   LaunchArgs a;
-  a.nslot = (uint32_t)u->geom->vol;
+  a.nslot = (uint32_t)(q1 - q0);
   a.M = in->M;
   a.spw = (uint32_t)c.spw;
   a.mc = (c.mc <= 0 || (uint32_t)c.mc > in->M) ? in->M : (uint32_t)c.mc;
@@ -257,11 +261,12 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
     if (pad > a.lds) a.lds = pad;
   }
   a.st = st;
-  a.gauge = u->d;
-  a.nbr = u->geom->nbr;
+  const uint64_t pb = (uint64_t)prec_bytes(u->prec), slot_bytes = (uint64_t)in->M * 24 * pb;
+  a.gauge = (const char *)u->d + q0 * 8 * (u->comp12 ? 12 : 18) * pb;
+  a.nbr = u->geom->nbr + q0 * 8;
   a.in = in->d;
-  a.out = out->d;
-  a.center = axpby ? center->d : nullptr;
+  a.out = (char *)out->d + q0 * slot_bytes;
+  a.center = axpby ? (const char *)center->d + q0 * slot_bytes : nullptr;
   a.xcd = c.xcd;
   a.nt = c.nt;
   a.alpha = alpha;
@@ -285,7 +290,8 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
 
 // ------------------------------------------------------------------- C-ABI
 int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in);
-int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st);
+int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st,
+                SlotRange range);
 
 static gbx_kcfg to_kcfg(const dhop_cfg &c)
 {
@@ -325,13 +331,18 @@ int dhop_run_exact(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *
   return dhop_launch(u, out, in, dag, c, 0.0, 1.0, nullptr, st, true);
 }
 
-int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st)
+int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st,
+             SlotRange range)
 {
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "dhop: dag must be 0 or 1");
-  if (k.kind == 1) return mfma_launch(u, out, in, dag, k, st);
-  if (k.kind == 2) return tile_launch(u, out, in, dag, k, st);
+  const bool whole = range.begin == 0 && range.end >= u->geom->vol;
+  if (k.kind == 1) return mfma_launch(u, out, in, dag, k, st, range);
+  if (k.kind == 2) {
+    GBX_CHECK(whole, GBX_EUNSUPPORTED, "dhop: the tiled kernel (kind 2) runs on the whole lattice only");
+    return tile_launch(u, out, in, dag, k, st);
+  }
   GBX_CHECK(k.kind == 0, GBX_EINVAL, "dhop: kernel kind %d", k.kind);
-  return dhop_launch(u, out, in, dag, from_kcfg(k), 0.0, 1.0, nullptr, st);
+  return dhop_launch(u, out, in, dag, from_kcfg(k), 0.0, 1.0, nullptr, st, false, range);
 }
 
 extern "C" int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg)
@@ -354,6 +365,20 @@ extern "C" int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag)
   gbx_kcfg k;
   gbx_dhop_get_cfg(u, in, &k);
   return dhop_run(u, out, in, dag, k, u->geom->stream);
+}
+
+extern "C" int gbx_dhop_range(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, uint64_t slot_begin, uint64_t slot_end)
+{
+  GBX_TRY(check_operands(u, out, in));
+  GBX_CHECK(slot_begin <= slot_end && slot_end <= u->geom->vol, GBX_EINVAL, "gbx_dhop_range: slots %llu .. %llu of %llu",
+            (unsigned long long)slot_begin, (unsigned long long)slot_end, (unsigned long long)u->geom->vol);
+  gbx_kcfg k;
+  gbx_dhop_get_cfg(u, in, &k);
+  if (k.kind == 2) k = to_kcfg(dhop_default_cfg(u, in));
+  SlotRange r;
+  r.begin = slot_begin;
+  r.end = slot_end;
+  return dhop_run(u, out, in, dag, k, u->geom->stream, r);
 }
 
 extern "C" int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta)

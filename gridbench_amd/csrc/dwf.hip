@@ -183,7 +183,7 @@ extern "C" int gbx_dwf_mooee(gbx_spinor out, gbx_spinor in, const gbx_dwf_params
 extern "C" int gbx_dwf_apply(gbx_gauge u, gbx_spinor out, gbx_spinor in, const gbx_dwf_params *p, int dag)
 {
   GBX_CHECK(u && out && in, GBX_ENULL, "gbx_dwf_apply: NULL argument");
-  GBX_CHECK(u->geom->parity < 0, GBX_EINVAL, "gbx_dwf_apply: whole lattices only (red-black halves: gbx_dwf_solve_eo)");
+  GBX_CHECK(whole_lattice(u->geom), GBX_EINVAL, "gbx_dwf_apply: whole lattices only (red-black halves: gbx_dwf_solve_eo)");
   GBX_TRY(check_params(p, in->Ls));
   GBX_TRY(gbx_dhop(u, out, in, dag));
   return dwf_s5_launch(out, in, out, p, 0, dag, 1.0, -0.5, u->geom->stream);

@@ -89,10 +89,29 @@ struct gbx_geom_s {
   gbx_geom_s *stream_owner = nullptr;
   // Tile tables of the LDS-tiled kernel, by packed tile shape (device arrays).
   std::map<int, uint32_t *> tile_tabs;
+  // Sites of the host gauge array an upload reads: the whole lattice (2*vol
+  // for a red-black half, the global volume for a domain part).
+  uint64_t gauge_vol = 0;
+  // Domain part (gbx_geom_create_part): this geometry is part `part` of
+  // `nparts` slabs of the global lattice `gdims` along `split_dim`.  Fields
+  // carry nghost = 2*face ghost slots after the vol local ones: slots vol ..
+  // vol+face-1 hold the previous part's top face, vol+face .. vol+2*face-1
+  // the next part's bottom face; nbr points into them across the cut.
+  // face_slots[0][f] / [1][f]: the local slots of the bottom (coordinate 0)
+  // and top (coordinate dims[split_dim]-1) faces, f = lexicographic index
+  // over the other three dimensions.
+  int split_dim = -1, nparts = 1, part = 0;
+  int gdims[4] = {0, 0, 0, 0};
+  // Slots 0 .. ninterior-1 are the interior sites (no leg reaches a ghost),
+  // slots ninterior .. vol-1 the two faces.
+  uint64_t face = 0, nghost = 0, ninterior = 0;
+  uint32_t *face_slots = nullptr;
 };
 
 // Full-lattice lexicographic site of each slot (the gauge upload's index).
 inline const uint32_t *gauge_sites(const gbx_geom_s *g) { return g->gsite_of_slot ? g->gsite_of_slot : g->site_of_slot; }
+// A whole periodic lattice (not a red-black half, not a domain part).
+inline bool whole_lattice(const gbx_geom_s *g) { return g->parity < 0 && g->nghost == 0; }
 // True when `in` holds the sites Dhop on `out_geom` reads: the same geometry
 // for a whole lattice, the partner half for a red-black half.
 inline bool dhop_input_geom_ok(const gbx_geom_s *out_geom, const gbx_geom_s *in_geom)
@@ -155,12 +174,19 @@ struct dhop_cfg {
   bool axpby_pf = false;
 };
 
+// Output slots begin .. end-1 (end clipped to the volume).  The VALU and MFMA
+// kernels run on a slot range by offsetting their per-slot pointers (links,
+// neighbour table, output): neighbour slots stay absolute.
+struct SlotRange {
+  uint64_t begin = 0, end = UINT64_MAX;
+};
+
 // out = alpha*in_c + beta*Dhop(in) where in_c is `center` (may be null when alpha == 0).
 // exact: the reference's rounding (no fused multiply-adds; FP64 / FP32 fields
 // with 18-real links, no axpby).
 int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag,
                 const dhop_cfg &cfg, double alpha, double beta, const gbx_spinor_s *center,
-                hipStream_t stream, bool exact = false);
+                hipStream_t stream, bool exact = false, SlotRange range = SlotRange());
 // Dhop with the reference's rounding, on the problem's tuned VALU shape.
 int dhop_run_exact(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, hipStream_t st);
 dhop_cfg dhop_default_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in);
@@ -171,7 +197,8 @@ int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
 void tile_tables_free(gbx_geom_s *g);
 int pack_tile(const int e[4]);
 // Any kernel kind (VALU or MFMA) with an explicit configuration.
-int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
+int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st,
+             SlotRange range = SlotRange());
 std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in);
 void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c);
 bool dhop_lookup_cfg(const std::string &sig, gbx_kcfg *c);

@@ -8,6 +8,7 @@
 // neighbours of a workgroup's sites were touched recently and are still in
 // L2 / the memory-side cache.  nbr[slot][mu] holds neighbour slots in the
 // reference's direction order Xp Yp Zp Tp Xm Ym Zm Tm (WilsonKernelsHand.h:5-12).
+#include <algorithm>
 #include <mutex>
 
 #include "gbx_internal.h"
@@ -200,7 +201,8 @@ int ref_geom(const gbx_geom_s *g, int nsimd, RefGeom &rg, const gbx_ref_layout &
   rg.lane0 = lay.lane0;
   GBX_CHECK(rg.width >= nsimd && rg.lane0 >= 0 && rg.lane0 + nsimd <= rg.width, GBX_EINVAL,
             "reference layout: lanes %d..%d outside a %d-lane vector", rg.lane0, rg.lane0 + nsimd - 1, rg.width);
-  GBX_CHECK(g->parity < 0, GBX_EUNSUPPORTED, "the reference's virtual-node layout describes a whole lattice, not a red-black half");
+  GBX_CHECK(whole_lattice(g), GBX_EUNSUPPORTED,
+            "the reference's virtual-node layout describes a whole lattice, not a red-black half or a domain part");
   rg.nsimd = nsimd;
   for (int d = 0; d < 4; d++) rg.simd[d] = 1;
   GBX_CHECK(nsimd == 1 || nsimd == 2 || nsimd == 4 || nsimd == 8 || nsimd == 16, GBX_EINVAL,
@@ -310,6 +312,7 @@ void geom_release(gbx_geom_s *g)
   (void)hipFree(g->slot_of_site);
   (void)hipFree(g->nbr);
   (void)hipFree(g->gsite_of_slot);
+  (void)hipFree(g->face_slots);
   tile_tables_free(g);
   gbx_geom_s *owner = g->stream_owner;
   if (g->stream && !owner) (void)hipStreamDestroy(g->stream);
@@ -422,6 +425,7 @@ gbx_geom_s *geom_new(const int dims[4], int nlevels, const int *blocks, uint64_t
   }
   for (int l = 0; l < nlevels * 4; l++) g->levels[l] = blocks[l];
   g->vol = vol;
+  g->gauge_vol = vol;
   g->device = device;
   return g;
 }
@@ -504,6 +508,7 @@ extern "C" int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geo
   std::vector<uint32_t> gsite[2];
   for (int p = 0; p < 2; p++) {
     h[p] = geom_new(dims, 1, blk, half, device);
+    h[p]->gauge_vol = vol;
     h[p]->parity = p;
     h[p]->pair_id = id;
     h[p]->site_of_slot_h.reserve(half);
@@ -539,6 +544,165 @@ extern "C" int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geo
   }
   *even = h[0];
   *odd = h[1];
+  return GBX_OK;
+}
+
+// Part `part` of `nparts` slabs along `split_dim`, with ghost slots for the
+// two cut faces (the local share of a domain-decomposed lattice; the faces
+// are exchanged with gbx_halo_pack / gbx_halo_unpack).
+namespace {
+
+struct PartTables {
+  int ld[4], blk[4];
+  uint64_t vol = 0, gvol = 0, face = 0, ninterior = 0;
+  std::vector<uint32_t> slot_of_site, site_of_slot, gsite, nbr, fslots;
+};
+
+int part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4], PartTables &t)
+{
+  GBX_CHECK(dims, GBX_ENULL, "domain part: NULL dims");
+  GBX_CHECK(split_dim >= 0 && split_dim < 4, GBX_EINVAL, "domain part: split_dim=%d", split_dim);
+  GBX_CHECK(nparts >= 2 && part >= 0 && part < nparts, GBX_EINVAL, "domain part: part %d of %d", part, nparts);
+  for (int d = 0; d < 4; d++) GBX_CHECK(dims[d] >= 1 && dims[d] <= 1024, GBX_EINVAL, "domain part: dims[%d]=%d", d, dims[d]);
+  GBX_CHECK(dims[split_dim] % nparts == 0, GBX_EINVAL, "domain part: %d parts do not divide dims[%d]=%d", nparts, split_dim,
+            dims[split_dim]);
+  int *ld = t.ld, *blk = t.blk;
+  for (int d = 0; d < 4; d++) ld[d] = d == split_dim ? dims[d] / nparts : dims[d];
+  for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : largest_divisor_le(ld[d], 6);
+  uint64_t vol;
+  GBX_TRY(check_dims_levels(ld, 1, blk, &vol));
+  t.vol = vol;
+  t.gvol = 1;
+  for (int d = 0; d < 4; d++) t.gvol *= (uint64_t)dims[d];
+  GBX_CHECK(t.gvol < (1ull << 32), GBX_EINVAL, "domain part: global volume %llu", (unsigned long long)t.gvol);
+  const uint64_t face = vol / (uint64_t)ld[split_dim];
+  t.face = face;
+  auto coords = [&](uint64_t site, int c[4]) {
+    for (int d = 0; d < 4; d++) {
+      c[d] = (int)(site % (uint64_t)ld[d]);
+      site /= (uint64_t)ld[d];
+    }
+  };
+  // lexicographic index over the three dimensions other than split_dim
+  auto face_index = [&](const int c[4]) {
+    uint64_t f = 0, stride = 1;
+    for (int d = 0; d < 4; d++) {
+      if (d == split_dim) continue;
+      f += (uint64_t)c[d] * stride;
+      stride *= (uint64_t)ld[d];
+    }
+    return f;
+  };
+  // Blocked order, then stably partitioned: the interior sites (no leg
+  // crosses the cut) take slots 0 .. ninterior-1, the two faces the rest,
+  // so the interior and the boundary are each one slot range.
+  const std::vector<uint32_t> bslot = blocked_slots(ld, 1, blk, vol);
+  std::vector<uint32_t> border(vol);
+  for (uint64_t site = 0; site < vol; site++) border[bslot[site]] = (uint32_t)site;
+  t.slot_of_site.assign(vol, 0);
+  t.site_of_slot.assign(vol, 0);
+  uint64_t n = 0;
+  for (int pass = 0; pass < 2; pass++)
+    for (uint64_t b = 0; b < vol; b++) {
+      const uint32_t site = border[b];
+      int c[4];
+      coords(site, c);
+      const bool bnd = c[split_dim] == 0 || c[split_dim] == ld[split_dim] - 1;
+      if (bnd != (pass == 1)) continue;
+      t.slot_of_site[site] = (uint32_t)n;
+      t.site_of_slot[n] = site;
+      n++;
+    }
+  t.ninterior = ld[split_dim] > 2 ? vol - 2 * face : 0;
+  // Neighbours: a leg that crosses the cut reads ghost slot vol + f (the
+  // previous part's top face) or vol + face + f (the next part's bottom face).
+  t.gsite.assign(vol, 0);
+  t.nbr.assign(vol * 8, 0);
+  t.fslots.assign(2 * face, 0);
+  const int off = part * ld[split_dim];
+  for (uint64_t q = 0; q < vol; q++) {
+    const uint64_t site = t.site_of_slot[q];
+    int c[4];
+    coords(site, c);
+    uint64_t gs = 0, stride = 1;
+    for (int d = 0; d < 4; d++) {
+      gs += (uint64_t)(d == split_dim ? c[d] + off : c[d]) * stride;
+      stride *= (uint64_t)dims[d];
+    }
+    t.gsite[q] = (uint32_t)gs;
+    const uint64_t f = face_index(c);
+    if (c[split_dim] == 0) t.fslots[f] = (uint32_t)q;
+    if (c[split_dim] == ld[split_dim] - 1) t.fslots[face + f] = (uint32_t)q;
+    for (int mu = 0; mu < 8; mu++) {
+      const int d = mu & 3;
+      if (d == split_dim && mu < 4 && c[d] == ld[d] - 1) t.nbr[q * 8 + mu] = (uint32_t)(vol + face + f);
+      else if (d == split_dim && mu >= 4 && c[d] == 0) t.nbr[q * 8 + mu] = (uint32_t)(vol + f);
+      else t.nbr[q * 8 + mu] = t.slot_of_site[nbr_site(site, mu, ld)];
+    }
+  }
+  return GBX_OK;
+}
+
+}  // namespace
+
+// The tables of a domain part, on the host (no device needed): site_of_slot
+// and gsite [V_local], nbr [V_local][8], face_slots [2][face]; sizes from
+// V_local = prod(dims) / nparts and face = V_local / (dims[split_dim] / nparts).
+extern "C" int gbx_part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4],
+                               uint32_t *site_of_slot, uint32_t *gsite, uint32_t *nbr, uint32_t *face_slots,
+                               uint64_t *ninterior)
+{
+  GBX_CHECK(site_of_slot && gsite && nbr && face_slots && ninterior, GBX_ENULL, "gbx_part_tables: NULL argument");
+  PartTables t;
+  GBX_TRY(part_tables(dims, split_dim, nparts, part, block, t));
+  std::copy(t.site_of_slot.begin(), t.site_of_slot.end(), site_of_slot);
+  std::copy(t.gsite.begin(), t.gsite.end(), gsite);
+  std::copy(t.nbr.begin(), t.nbr.end(), nbr);
+  std::copy(t.fslots.begin(), t.fslots.end(), face_slots);
+  *ninterior = t.ninterior;
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_create_part(const int dims[4], int split_dim, int nparts, int part, const int block[4],
+                                    gbx_geom *out)
+{
+  GBX_CHECK(dims && out, GBX_ENULL, "gbx_geom_create_part: NULL argument");
+  *out = nullptr;
+  PartTables t;
+  GBX_TRY(part_tables(dims, split_dim, nparts, part, block, t));
+  GBX_TRY(gbx_require_device());
+  int device;
+  GBX_HIP(hipGetDevice(&device));
+  gbx_geom_s *g = geom_new(t.ld, 1, t.blk, t.vol, device);
+  g->gauge_vol = t.gvol;
+  g->split_dim = split_dim;
+  g->nparts = nparts;
+  g->part = part;
+  for (int d = 0; d < 4; d++) g->gdims[d] = dims[d];
+  g->face = t.face;
+  g->nghost = 2 * t.face;
+  g->ninterior = t.ninterior;
+  g->slot_of_site_h = std::move(t.slot_of_site);
+  g->site_of_slot_h = std::move(t.site_of_slot);
+  GBX_TRY(geom_finish(g, t.nbr, &t.gsite));  // releases g on failure
+  if (hipMalloc(&g->face_slots, t.fslots.size() * 4) != hipSuccess ||
+      hipMemcpy(g->face_slots, t.fslots.data(), t.fslots.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+    (void)hipGetLastError();
+    geom_release(g);
+    return gbx_fail(GBX_ENOMEM, "gbx_geom_create_part: face table");
+  }
+  *out = g;
+  return GBX_OK;
+}
+
+extern "C" int gbx_geom_part_info(gbx_geom g, int *split_dim, int *nparts, int *part, uint64_t *face, uint64_t *ninterior)
+{
+  GBX_CHECK(g && split_dim && nparts && part && face && ninterior, GBX_ENULL, "gbx_geom_part_info: NULL argument");
+  *split_dim = g->split_dim;
+  *nparts = g->nparts;
+  *part = g->part;
+  *face = g->face;
+  *ninterior = g->nghost ? g->ninterior : g->vol;
   return GBX_OK;
 }
 
@@ -614,8 +778,7 @@ extern "C" int gbx_gauge_upload(gbx_gauge u, const double *doubled)
   gbx_geom_s *g = u->geom;
   std::lock_guard<std::mutex> lk(g_staging_mu);
   void *stage;
-  const uint64_t full_vol = g->parity < 0 ? g->vol : 2 * g->vol;  // a red-black half reads the whole lattice
-  const size_t in_bytes = full_vol * 8 * 18 * sizeof(double);
+  const size_t in_bytes = g->gauge_vol * 8 * 18 * sizeof(double);  // a half or a part reads the whole lattice
   GBX_TRY(staging_buffer(in_bytes, &stage));
   GBX_HIP(hipMemcpyAsync(stage, doubled, in_bytes, hipMemcpyHostToDevice, g->stream));
   const int Rl = u->comp12 ? 12 : 18;
@@ -663,7 +826,7 @@ extern "C" int gbx_spinor_create(gbx_geom g, int Ls, int nrhs, int prec, gbx_spi
   x->nrhs = nrhs;
   x->prec = prec;
   x->M = (uint32_t)(Ls * nrhs);
-  x->bytes = g->vol * (size_t)x->M * 24 * (size_t)prec_bytes(prec);
+  x->bytes = (g->vol + g->nghost) * (size_t)x->M * 24 * (size_t)prec_bytes(prec);
   if (hipMalloc(&x->d, x->bytes) != hipSuccess) {
     const size_t b = x->bytes;
     delete x;

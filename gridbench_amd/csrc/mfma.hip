@@ -232,7 +232,8 @@ int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in)
 // (one wavefront per 64), mc = minor columns per wavefront unit (a multiple
 // of 16 dividing M; 0 = all of M), pf = gather the next 16 columns before
 // the current MFMAs, xcd / nt / wgpc as for the VALU kernel (cblk unused).
-int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st)
+int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st,
+                SlotRange range)
 {
   GBX_CHECK(u->prec == in->prec && out->prec == in->prec, GBX_EMISMATCH, "dhop (MFMA): precisions differ");
   GBX_CHECK(mfma_supported(u, in), GBX_EUNSUPPORTED,
@@ -251,17 +252,25 @@ int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
     const size_t pad = ((size_t)160 * 1024 / (size_t)c.wgpc) & ~(size_t)511;
     if (pad > lds) lds = pad;
   }
-  const uint32_t nslot = (uint32_t)u->geom->vol;
+  const uint64_t q0 = range.begin, q1 = std::min<uint64_t>(range.end, u->geom->vol);
+  GBX_CHECK(q0 <= q1, GBX_EINVAL, "dhop (MFMA): slot range %llu .. %llu", (unsigned long long)q0, (unsigned long long)q1);
+  if (q0 == q1) return GBX_OK;
+  const uint32_t nslot = (uint32_t)(q1 - q0);
   const uint32_t nwg = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
   const uint32_t per_xcd = (nwg + 7) / 8;
   const dim3 grid(c.xcd ? per_xcd * 8 : nwg), block((unsigned)c.threads);
   const bool c12 = u->comp12 != 0, pf = c.pf != 0;
+  // a slot range offsets the per-slot arrays; neighbour slots stay absolute
+  const uint64_t pb = (uint64_t)prec_bytes(u->prec);
+  const void *gauge = (const char *)u->d + q0 * 8 * (c12 ? 12 : 18) * pb;
+  const uint32_t *nbr = u->geom->nbr + q0 * 8;
+  void *dout = (char *)out->d + q0 * (uint64_t)in->M * 24 * pb;
   if (in->prec == GBX_F16)
-    launch_prec<GBX_F16>(dag != 0, c12, pf, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, nslot, in->M,
-                         (uint32_t)c.spw, mw, nwg, per_xcd, c.xcd, c.nt);
+    launch_prec<GBX_F16>(dag != 0, c12, pf, grid, block, lds, st, gauge, nbr, in->d, dout, nslot, in->M, (uint32_t)c.spw,
+                         mw, nwg, per_xcd, c.xcd, c.nt);
   else
-    launch_prec<GBX_BF16>(dag != 0, c12, pf, grid, block, lds, st, u->d, u->geom->nbr, in->d, out->d, nslot, in->M,
-                          (uint32_t)c.spw, mw, nwg, per_xcd, c.xcd, c.nt);
+    launch_prec<GBX_BF16>(dag != 0, c12, pf, grid, block, lds, st, gauge, nbr, in->d, dout, nslot, in->M, (uint32_t)c.spw,
+                          mw, nwg, per_xcd, c.xcd, c.nt);
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }

@@ -234,7 +234,7 @@ int pack_tile(const int e[4]) { return e[0] | (e[1] << 4) | (e[2] << 8) | (e[3] 
 int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *in)
 {
   const gbx_geom_s *g = u->geom;
-  GBX_CHECK(g->parity < 0, GBX_EUNSUPPORTED, "tile kernel: whole lattices only");
+  GBX_CHECK(whole_lattice(g), GBX_EUNSUPPORTED, "tile kernel: whole lattices only");
   const TileShape s = unpack_tile(k.tile);
   for (int d = 0; d < 4; d++)
     GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,

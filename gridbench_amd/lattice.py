@@ -170,10 +170,13 @@ class Geometry:
     """Periodic 4D lattice on the GPU.  `block` is the traversal block; or
     `levels` = [innermost block, ..., outermost block] for a multi-level
     (blocks-of-blocks) site order.  Geometry.rb_pair(dims) gives the even and
-    odd halves of a red-black pair (parity -1 / 0 / 1; volume = sites held)."""
+    odd halves of a red-black pair (parity -1 / 0 / 1; volume = sites held).
+    Geometry.part(dims, split_dim, nparts, part) gives one slab of a domain
+    decomposition (dims = its local dims, gauge_dims = the whole lattice's)."""
 
     def __init__(self, dims, block=None, levels=None, _handle=None):
         self.dims = tuple(int(d) for d in dims)
+        self.gauge_dims = self.dims
         h = C.c_void_p()
         if _handle is not None:
             h = _handle
@@ -191,6 +194,23 @@ class Geometry:
         par = C.c_int()
         check(lib.gbx_geom_parity(h, C.byref(par)), "gbx_geom_parity")
         self.parity = int(par.value)
+        sd, npart, pt, face, nint = C.c_int(), C.c_int(), C.c_int(), C.c_uint64(), C.c_uint64()
+        check(lib.gbx_geom_part_info(h, C.byref(sd), C.byref(npart), C.byref(pt), C.byref(face), C.byref(nint)),
+              "gbx_geom_part_info")
+        self.split_dim, self.nparts, self.part = int(sd.value), int(npart.value), int(pt.value)
+        self.face, self.ninterior = int(face.value), int(nint.value)
+
+    @classmethod
+    def part_of(cls, dims, split_dim, nparts, part, block=None):
+        """Slab `part` of `nparts` along `split_dim` of the lattice `dims`."""
+        dims = tuple(int(d) for d in dims)
+        h = C.c_void_p()
+        check(lib.gbx_geom_create_part(ivec(dims), int(split_dim), int(nparts), int(part),
+                                       ivec(block) if block is not None else None, C.byref(h)), "gbx_geom_create_part")
+        local = tuple(d // nparts if i == split_dim else d for i, d in enumerate(dims))
+        g = cls(local, _handle=h)
+        g.gauge_dims = dims
+        return g
 
     @classmethod
     def rb_pair(cls, dims, block=None):
@@ -229,7 +249,7 @@ class GaugeField:
         """Doubled field of the WHOLE lattice, lexicographic float64
         [V][8][3][3][2] (a red-black half keeps its own sites' links)."""
         a = np.ascontiguousarray(doubled, dtype=np.float64)
-        assert a.size == int(np.prod(self.geom.dims)) * 144
+        assert a.size == int(np.prod(self.geom.gauge_dims)) * 144
         check(lib.gbx_gauge_upload(self.h, ptr(a, C.c_double)), "gbx_gauge_upload")
         return self
 
@@ -342,6 +362,29 @@ class SpinorField:
         check(lib.gbx_spinor_bytes(self.h, C.byref(b)), "gbx_spinor_bytes")
         return b.value
 
+    # ---- halo of a domain part: device pointers are plain integers
+    def halo_bytes(self, proj=False):
+        b = C.c_uint64()
+        check(lib.gbx_halo_bytes(self.h, int(bool(proj)), C.byref(b)), "gbx_halo_bytes")
+        return b.value
+
+    def halo_ghost(self, side):
+        p = C.c_void_p()
+        check(lib.gbx_halo_ghost(self.h, int(side), C.byref(p)), "gbx_halo_ghost")
+        return p.value
+
+    def halo_pack(self, face, dptr, proj=False, dag=False):
+        if proj:
+            check(lib.gbx_halo_pack_proj(self.h, int(face), int(bool(dag)), C.c_void_p(dptr)), "gbx_halo_pack_proj")
+        else:
+            check(lib.gbx_halo_pack(self.h, int(face), C.c_void_p(dptr)), "gbx_halo_pack")
+        return self
+
+    def halo_unpack(self, side, dptr, proj=False):
+        fn = lib.gbx_halo_unpack_proj if proj else lib.gbx_halo_unpack
+        check(fn(self.h, int(side), C.c_void_p(dptr)), "gbx_halo_unpack_proj" if proj else "gbx_halo_unpack")
+        return self
+
     def close(self):
         if self.h:
             lib.gbx_spinor_destroy(self.h)
@@ -373,6 +416,68 @@ def dhop(U, out, inp, dag=False, cfg=None):
     else:
         check(lib.gbx_dhop_cfg(U.h, out.h, inp.h, int(bool(dag)), C.byref(cfg)), "gbx_dhop_cfg")
     return out
+
+
+class DeviceBuffer:
+    """Device memory for halo faces (`ptr` is the device address)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        p = C.c_void_p()
+        check(lib.gbx_buffer_create(self.nbytes, C.byref(p)), "gbx_buffer_create")
+        self.ptr = p.value
+
+    def download(self, dtype=np.uint8):
+        out = np.empty(self.nbytes // np.dtype(dtype).itemsize, dtype=dtype)
+        check(lib.gbx_buffer_download(C.c_void_p(self.ptr), self.nbytes, out.ctypes.data_as(C.c_void_p)),
+              "gbx_buffer_download")
+        return out
+
+    def close(self):
+        if self.ptr:
+            lib.gbx_buffer_destroy(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dhop_range(U, out, inp, slot_begin, slot_end, dag=False):
+    """Dhop on output slots slot_begin .. slot_end-1 (device order)."""
+    check(lib.gbx_dhop_range(U.h, out.h, inp.h, int(bool(dag)), int(slot_begin), int(slot_end)), "gbx_dhop_range")
+    return out
+
+
+def halo_exchange(fields, proj=False, dag=False, bufs=None):
+    """Exchange the faces of the parts `fields` (part p = fields[p], all
+    split the same way), as the ranks of a domain decomposition would:
+    part p's top face into part p+1's minus ghosts, its bottom face into
+    part p-1's plus ghosts.  Whole spinors are packed straight into the
+    receiving ghost slots; spin-projected faces (proj=True) go through the
+    device buffers `bufs` (bufs[p] = (bottom, top) DeviceBuffers of
+    halo_bytes(proj=True) bytes each).  Synchronises every part's stream
+    before returning."""
+    n = len(fields)
+    for p, x in enumerate(fields):
+        up, down = fields[(p + 1) % n], fields[(p - 1) % n]
+        if proj:
+            x.halo_pack(1, bufs[p][1].ptr, proj=True, dag=dag)
+            x.halo_pack(0, bufs[p][0].ptr, proj=True, dag=dag)
+        else:
+            x.halo_pack(1, up.halo_ghost(0))
+            x.halo_pack(0, down.halo_ghost(1))
+    for x in fields:
+        x.geom.sync()
+    if proj:
+        for p, x in enumerate(fields):
+            x.halo_unpack(0, bufs[(p - 1) % n][1].ptr, proj=True)
+            x.halo_unpack(1, bufs[(p + 1) % n][0].ptr, proj=True)
+        for x in fields:
+            x.geom.sync()
+    return fields
 
 
 def dhop_xpay(U, out, x, inp, alpha, beta, dag=False, cfg=None):
@@ -506,6 +611,51 @@ def static_data_read(paths):
     return out
 
 
+def part_tables(dims, split_dim, nparts, part, block=None):
+    """Host tables of a domain part (no GPU needed): dict with site_of_slot
+    and gsite [V_local], nbr [V_local][8] (ghost slots V_local .. V_local +
+    2*face - 1), face_slots [2][face] and ninterior."""
+    dims = tuple(int(d) for d in dims)
+    ok = 0 <= split_dim < 4 and nparts >= 1 and min(dims) >= 1 and dims[split_dim] % nparts == 0
+    # invalid arguments: the library rejects them before writing any table
+    vol = int(np.prod(dims)) // nparts if ok else 1
+    face = vol // (dims[split_dim] // nparts) if ok else 1
+    sos = np.zeros(vol, np.uint32)
+    gs = np.zeros(vol, np.uint32)
+    nbr = np.zeros((vol, 8), np.uint32)
+    fs = np.zeros((2, face), np.uint32)
+    nint = C.c_uint64()
+    check(lib.gbx_part_tables(ivec(dims), int(split_dim), int(nparts), int(part), ivec(block) if block is not None else None,
+                              ptr(sos, C.c_uint32), ptr(gs, C.c_uint32), ptr(nbr, C.c_uint32), ptr(fs, C.c_uint32),
+                              C.byref(nint)), "gbx_part_tables")
+    return {"site_of_slot": sos, "gsite": gs, "nbr": nbr, "face_slots": fs, "ninterior": int(nint.value),
+            "vol": vol, "face": face}
+
+
+def part_split(dims, split_dim, nparts, lex):
+    """Lexicographic spinor array [nrhs][V][Ls][...] of the lattice `dims`
+    -> one array [nrhs][V/nparts][Ls][...] per slab along split_dim, each in
+    its local lexicographic order (the layout SpinorField.upload takes on a
+    Geometry.part_of geometry)."""
+    a = np.asarray(lex)
+    nrhs = a.shape[0]
+    rest = a.shape[2:]
+    a = a.reshape((nrhs, dims[3], dims[2], dims[1], dims[0]) + rest)
+    ax = 1 + (3 - split_dim)
+    return [np.ascontiguousarray(c).reshape((nrhs, -1) + rest) for c in np.split(a, nparts, axis=ax)]
+
+
+def part_merge(dims, split_dim, parts):
+    """Inverse of part_split."""
+    nparts = len(parts)
+    loc = [d // nparts if i == split_dim else d for i, d in enumerate(dims)]
+    nrhs = parts[0].shape[0]
+    rest = parts[0].shape[2:]
+    blocks = [np.asarray(p).reshape((nrhs, loc[3], loc[2], loc[1], loc[0]) + rest) for p in parts]
+    out = np.concatenate(blocks, axis=1 + (3 - split_dim))
+    return out.reshape((nrhs, -1) + rest)
+
+
 def rb_split(dims, lex):
     """Whole-lattice field [..., V, ...] (site axis -1 of the leading shape is
     the lexicographic site) -> (even, odd) halves in checkerboarded
@@ -602,5 +752,6 @@ __all__ = [
     "dhop", "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
-    "static_data_write", "static_data_read_header", "static_data_read",
+    "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
+    "part_merge", "DeviceBuffer", "part_tables",
 ]

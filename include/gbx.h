@@ -177,6 +177,33 @@ int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gb
 int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geom *even, gbx_geom *odd);
 /* -1 for a whole lattice, 0 / 1 for the even / odd half of a pair. */
 int gbx_geom_parity(gbx_geom g, int *parity);
+/* Domain part: slab `part` (0 .. nparts-1) of the periodic lattice `dims`  */
+/* cut into nparts equal slabs along split_dim, as one GPU of a domain-     */
+/* decomposed run holds it.  No counterpart in the reference, which runs    */
+/* on one rank (TableGenerate.cc:84 asserts every stencil neighbour is      */
+/* local); this is the single-GPU share of a multi-GPU decomposition.       */
+/*  - local dims = dims with dims[split_dim] / nparts; spinor host arrays    */
+/*    are [nrhs][V_local][Ls][24] in local lexicographic order;             */
+/*  - gbx_gauge_upload reads the GLOBAL doubled field [V][8][18] and keeps  */
+/*    the part's own sites' links;                                          */
+/*  - a spinor field holds 2*face ghost slots after its V_local sites       */
+/*    (face = V_local / local dims[split_dim]): legs that cross the cut     */
+/*    read them, so they must hold the neighbouring parts' faces (see the   */
+/*    halo functions below) before a Dhop;                                  */
+/*  - slots 0 .. ninterior-1 are the interior sites, which read no ghost;   */
+/*    slots ninterior .. V_local-1 the sites of the two faces.              */
+/* Parts run the VALU and MFMA kernels (not the tiled kernel) and gbx_dhop,   */
+/* gbx_dhop_range, gbx_dhop_xpay; solvers take whole lattices.              */
+int gbx_geom_create_part(const int dims[4], int split_dim, int nparts, int part, const int block[4], gbx_geom *out);
+/* split_dim = -1, nparts = 1, part = 0, face = 0, ninterior = V for a      */
+/* geometry that is not a part.                                             */
+int gbx_geom_part_info(gbx_geom g, int *split_dim, int *nparts, int *part, uint64_t *face, uint64_t *ninterior);
+/* The part's tables on the host, without a device (what the geometry      */
+/* uploads): site_of_slot (local lexicographic site of each slot) and gsite */
+/* (global site) [V_local], nbr [V_local][8] (ghost slots V_local .. V_local */
+/* + 2*face - 1), face_slots [2][face] (slots of the bottom, top faces).   */
+int gbx_part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4], uint32_t *site_of_slot,
+                    uint32_t *gsite, uint32_t *nbr, uint32_t *face_slots, uint64_t *ninterior);
 int gbx_geom_destroy(gbx_geom g);
 int gbx_geom_volume(gbx_geom g, uint64_t *vol);
 /* The geometry's stream (hipStream_t) on which all its work runs. */
@@ -257,6 +284,12 @@ int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_
 /* gbx_dhop_xpay with an explicit VALU-kernel configuration (kind 0).     */
 int gbx_dhop_xpay_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta,
                       const gbx_kcfg *cfg);
+/* gbx_dhop on output slots slot_begin .. slot_end-1 only (device order,   */
+/* 0 <= begin <= end <= V; other output slots are not written).  On a     */
+/* domain part, [0, ninterior) needs no ghost: it can run while the halo   */
+/* is in flight, then [ninterior, V) once it has landed.  VALU or MFMA     */
+/* kernel (a tuned tiled configuration falls back to the VALU default).    */
+int gbx_dhop_range(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, uint64_t slot_begin, uint64_t slot_end);
 /* Configuration gbx_dhop would use now (autotuned if gbx_tune ran). */
 int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg);
 /* Average device time per application over nrep applications after one  */
@@ -280,6 +313,39 @@ int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *ca
 #define GBX_KIND_TILE 4
 int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kinds, const char *cache_path,
                    gbx_kcfg *best, double *best_usec);
+
+/* ---------------------------------------------------------------------- */
+/* Halo exchange of a domain part (gbx_geom_create_part).  No counterpart  */
+/* in the reference (single rank).  Part p's top face (coordinate          */
+/* ld-1 along the cut) goes to part p+1's minus ghosts (side 0), its bottom */
+/* face (coordinate 0) to part p-1's plus ghosts (side 1), modulo nparts.   */
+/* A face is [face][nch][Ls*nrhs] 16-byte chunks, face site f ordered      */
+/* lexicographically over the other three dimensions: exactly the layout   */
+/* of a side's ghost slots, so a receive (RCCL recv, peer copy or a pack   */
+/* into peer memory) can land in gbx_halo_ghost's range directly.  All      */
+/* enqueued on the field's geometry stream, no synchronisation; device      */
+/* buffers, 16-byte aligned, not overlapping the field.                    */
+/* ---------------------------------------------------------------------- */
+/* Bytes of one face: proj = 0 whole spinors, 1 spin-projected halves.     */
+int gbx_halo_bytes(gbx_spinor x, int proj, uint64_t *bytes);
+/* Device address of the ghost slots of side 0 (minus) or 1 (plus).        */
+int gbx_halo_ghost(gbx_spinor x, int side, void **dptr);
+/* Gather face 0 (bottom) or 1 (top) into dbuf (gbx_halo_bytes(x, 0)).     */
+int gbx_halo_pack(gbx_spinor x, int face, void *dbuf);
+/* Copy a received whole-spinor face into the ghosts of `side`.            */
+int gbx_halo_unpack(gbx_spinor x, int side, const void *dbuf);
+/* Spin-projected face (half the bytes; FP64 / FP32 fields): the two-spinor */
+/* the receiving part's leg forms from it for Dhop (dag = 0) or DhopDag    */
+/* (dag = 1), in the storage precision.  gbx_halo_unpack_proj expands it   */
+/* into ghosts whose projection is that two-spinor exactly, so the result  */
+/* is bit-identical to the one from whole-spinor ghosts.                   */
+int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf);
+int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf);
+/* Device buffers for faces (hipMalloc; destroy synchronises the device),  */
+/* and a blocking copy of one to the host.                                  */
+int gbx_buffer_create(uint64_t bytes, void **dptr);
+int gbx_buffer_destroy(void *dptr);
+int gbx_buffer_download(const void *dptr, uint64_t bytes, void *host);
 
 /* ---------------------------------------------------------------------- */
 /* Vector algebra on spinors (used by the solver; deterministic order).    */
