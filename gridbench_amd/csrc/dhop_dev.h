@@ -214,9 +214,11 @@ __device__ __forceinline__ void stage_links(const void *__restrict__ gauge, cons
   constexpr int RL = C12 ? 12 : 18;
   const S *g = (const S *)gauge + (uint64_t)q0 * 8 * RL;
   const uint32_t n = nsl * 8 * RL;
+  // Links are read once per launch: non-temporal loads keep them from
+  // displacing neighbour spinors, which are re-read, in the L2.
   for (uint32_t e = threadIdx.x; e < n; e += blockDim.x) {
     const uint32_t link = e / RL, k = e - link * RL;
-    sU[link * 18 + k] = (A)g[e];
+    sU[link * 18 + k] = (A)__builtin_nontemporal_load(g + e);
   }
   for (uint32_t e = threadIdx.x; e < nsl * 8; e += blockDim.x) sN[e] = nbr[(uint64_t)q0 * 8 + e];
   if (C12) {
