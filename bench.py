@@ -33,9 +33,11 @@ Also reported, each with its own seeds:
   halo       24^4 x Ls16 FP64 cut along T into 2 and 4 slabs (domain parts
              with ghost slots): the assembled result against the whole
              lattice's, and for one slab the Dhop time on its share, on its
-             interior and on its faces, the face pack / unpack times and the
-             face bytes; the xGMI transfer time is modelled (bytes / link
-             rate), not measured.
+             interior and on its faces, the face pack / unpack times, the
+             face bytes, and one rank's whole schedule with the faces moved
+             on this GPU (serial, and overlapped with the interior on a
+             second stream); the xGMI transfer time is modelled (bytes /
+             link rate), not measured.
 
 Launch configurations come from the autotuner; its results for this machine
 are cached in gridbench_amd/tuned_mi355x.jsonl (re-validated on every run).
@@ -487,14 +489,20 @@ def halo_section(gbx, tune, dbl, psi, ref64, nrep):
         us_pack = host_timed(lambda: x0.halo_pack(1, x[1].halo_ghost(0)), g0, nrep)
         us_pack_proj = host_timed(lambda: x0.halo_pack(1, bufs[0][1].ptr, proj=True), g0, nrep)
         us_unpack_proj = host_timed(lambda: x0.halo_unpack(0, bufs[1][1].ptr, proj=True), g0, nrep)
+        # one rank's schedule on this GPU (faces moved by a device copy)
+        us_ov, us_se = gbx.halo_overlap_time(u0, y0, x0, proj=True, nrep=nrep)
+        us_ov_w, us_se_w = gbx.halo_overlap_time(u0, y0, x0, proj=False, nrep=nrep)
         fb, fbp = x0.halo_bytes(), nb
         # per application a part sends two faces and receives two: with two
         # parts both go to the one neighbour (one link), with more parts to
         # two neighbours over two links at once
         faces_per_link = 2 if nparts == 2 else 1
         xfer_us = faces_per_link * fbp / (XGMI_GBPS * 1e9) * 1e6
-        comm_us = 2 * us_pack_proj + xfer_us + 2 * us_unpack_proj
-        overlapped = max(us_int, comm_us) + us_bnd
+        # The pack and unpack kernels need CUs, which the interior kernel
+        # holds while it runs (the measured single-GPU schedule shows no
+        # overlap of kernels on two streams): only the transfer, on copy
+        # engines, overlaps the interior.
+        overlapped = 2 * us_pack_proj + max(us_int, xfer_us) + 2 * us_unpack_proj + us_bnd
         r = {"nparts": nparts, "split_dim": "T", "local_dims": list(g0.dims), "local_sites": vol,
              "interior_sites": nint, "face_sites": g0.face, "kernel_cfg": best.as_dict(),
              "rel_l2_assembled_vs_whole_lattice": float(f"{err:.3e}"),
@@ -502,6 +510,10 @@ def halo_section(gbx, tune, dbl, psi, ref64, nrep):
              "us_dhop_faces_host": round(us_bnd, 2), "face_bytes_full": fb, "face_bytes_projected": fbp,
              "us_pack_full_face": round(us_pack, 2), "us_pack_projected_face": round(us_pack_proj, 2),
              "us_unpack_projected_face": round(us_unpack_proj, 2),
+             "us_application_serial_local_move": round(us_se, 2),
+             "us_application_overlapped_local_move": round(us_ov, 2),
+             "us_application_serial_local_move_whole_faces": round(us_se_w, 2),
+             "us_application_overlapped_local_move_whole_faces": round(us_ov_w, 2),
              "model_xgmi_gbps_per_link": XGMI_GBPS, "model_projected_faces_per_link": faces_per_link,
              "model_us_xgmi": round(xfer_us, 1),
              "model_us_per_application_overlapped": round(overlapped, 1),

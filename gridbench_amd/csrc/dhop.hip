@@ -367,18 +367,28 @@ extern "C" int gbx_dhop(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag)
   return dhop_run(u, out, in, dag, k, u->geom->stream);
 }
 
+gbx_kcfg dhop_range_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in)
+{
+  gbx_kcfg k;
+  gbx_dhop_get_cfg(const_cast<gbx_gauge_s *>(u), const_cast<gbx_spinor_s *>(in), &k);
+  if (k.kind == 2) k = to_kcfg(dhop_default_cfg(u, in));  // the tiled kernel covers whole lattices only
+  return k;
+}
+
+int dhop_check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const gbx_spinor_s *in)
+{
+  return check_operands(u, out, in);
+}
+
 extern "C" int gbx_dhop_range(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, uint64_t slot_begin, uint64_t slot_end)
 {
   GBX_TRY(check_operands(u, out, in));
   GBX_CHECK(slot_begin <= slot_end && slot_end <= u->geom->vol, GBX_EINVAL, "gbx_dhop_range: slots %llu .. %llu of %llu",
             (unsigned long long)slot_begin, (unsigned long long)slot_end, (unsigned long long)u->geom->vol);
-  gbx_kcfg k;
-  gbx_dhop_get_cfg(u, in, &k);
-  if (k.kind == 2) k = to_kcfg(dhop_default_cfg(u, in));
   SlotRange r;
   r.begin = slot_begin;
   r.end = slot_end;
-  return dhop_run(u, out, in, dag, k, u->geom->stream, r);
+  return dhop_run(u, out, in, dag, dhop_range_cfg(u, in), u->geom->stream, r);
 }
 
 extern "C" int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta)

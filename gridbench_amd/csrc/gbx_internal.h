@@ -199,6 +199,10 @@ int pack_tile(const int e[4]);
 // Any kernel kind (VALU or MFMA) with an explicit configuration.
 int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st,
              SlotRange range = SlotRange());
+// The configuration gbx_dhop_range runs (the tuned one unless it is the
+// whole-lattice tiled kernel, then the VALU default).
+gbx_kcfg dhop_range_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in);
+int dhop_check_operands(const gbx_gauge_s *u, const gbx_spinor_s *out, const gbx_spinor_s *in);
 std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in);
 void dhop_remember_cfg(const std::string &sig, const gbx_kcfg &c);
 bool dhop_lookup_cfg(const std::string &sig, gbx_kcfg *c);

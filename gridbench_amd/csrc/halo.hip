@@ -18,6 +18,7 @@
 // nch chunks of M 16-byte units, the same bytes as the site's slot.
 
 #include <algorithm>
+#include <functional>
 
 #include "dhop_dev.h"
 #include "gbx_internal.h"
@@ -96,6 +97,13 @@ void launch_pack_proj(int K, dim3 grid, hipStream_t st, const void *src, const u
 
 unsigned grid_for(uint64_t total) { return (unsigned)std::min<uint64_t>((total + 255) / 256, 65536); }
 
+// [b, b + bytes) and the field's allocation share no byte.
+bool disjoint(const void *buf, uint64_t bytes, const gbx_spinor_s *x)
+{
+  const char *b = (const char *)buf, *d = (const char *)x->d;
+  return b + bytes <= d || b >= d + x->bytes;
+}
+
 int part_ok(gbx_spinor_s *x, const char *who)
 {
   GBX_CHECK(x, GBX_ENULL, "%s: NULL field", who);
@@ -126,23 +134,6 @@ extern "C" int gbx_halo_ghost(gbx_spinor x, int side, void **dptr)
   return GBX_OK;
 }
 
-extern "C" int gbx_halo_pack(gbx_spinor x, int face, void *dbuf)
-{
-  GBX_CHECK(dbuf, GBX_ENULL, "gbx_halo_pack: NULL buffer");
-  GBX_TRY(part_ok(x, "gbx_halo_pack"));
-  GBX_CHECK(face == 0 || face == 1, GBX_EINVAL, "gbx_halo_pack: face=%d (0 = bottom, 1 = top)", face);
-  GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "gbx_halo_pack: buffer not 16-byte aligned");
-  const gbx_geom_s *g = x->geom;
-  const uint64_t units = slot_units(x), total = g->face * units;
-  const uint64_t bytes = total * 16;
-  const char *b = (const char *)dbuf, *d = (const char *)x->d;
-  GBX_CHECK(b + bytes <= d || b >= d + x->bytes, GBX_EINVAL, "gbx_halo_pack: buffer overlaps the field");
-  hipLaunchKernelGGL(k_halo_pack, dim3(grid_for(total)), dim3(256), 0, g->stream, (const uint4 *)x->d,
-                     g->face_slots + (uint64_t)face * g->face, (uint4 *)dbuf, g->face, units);
-  GBX_HIP(hipGetLastError());
-  return GBX_OK;
-}
-
 // Projector of the leg that reads the face on the receiving part: the top
 // face feeds the next part's backward leg (direction split_dim + 4), the
 // bottom face the previous part's forward leg (direction split_dim); the
@@ -154,52 +145,175 @@ static int face_projector(int split_dim, int face, int dag)
   return dag ? dir : (dir ^ 4);
 }
 
-extern "C" int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf)
+// Face `face` of x into dbuf (whole or projected), enqueued on st.
+static int pack_impl(gbx_spinor_s *x, int face, int dag, bool proj, void *dbuf, hipStream_t st, const char *who)
 {
-  GBX_CHECK(dbuf, GBX_ENULL, "gbx_halo_pack_proj: NULL buffer");
-  GBX_TRY(part_ok(x, "gbx_halo_pack_proj"));
-  GBX_CHECK(face == 0 || face == 1, GBX_EINVAL, "gbx_halo_pack_proj: face=%d (0 = bottom, 1 = top)", face);
-  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_halo_pack_proj: dag=%d", dag);
-  GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
-            "gbx_halo_pack_proj: FP64 / FP32 fields (16-bit storage is projected in FP32 by the kernels: a projected "
-            "FP32 face is no smaller than the full 16-bit one, use gbx_halo_pack)");
-  GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "gbx_halo_pack_proj: buffer not 16-byte aligned");
+  GBX_CHECK(dbuf, GBX_ENULL, "%s: NULL buffer", who);
+  GBX_TRY(part_ok(x, who));
+  GBX_CHECK(face == 0 || face == 1, GBX_EINVAL, "%s: face=%d (0 = bottom, 1 = top)", who, face);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
+  GBX_CHECK(!proj || x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "%s: projected faces of FP64 / FP32 fields only (16-bit storage is projected in FP32 by the kernels: a "
+            "projected FP32 face is no smaller than the full 16-bit one, use gbx_halo_pack)", who);
+  GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
   const gbx_geom_s *g = x->geom;
-  const uint64_t bytes = g->face * slot_units(x) * 8;
-  const char *b = (const char *)dbuf, *d = (const char *)x->d;
-  GBX_CHECK(b + bytes <= d || b >= d + x->bytes, GBX_EINVAL, "gbx_halo_pack_proj: buffer overlaps the field");
-  const int K = face_projector(g->split_dim, face, dag);
-  const dim3 grid(grid_for(g->face * x->M));
+  const uint64_t units = slot_units(x);
+  GBX_CHECK(disjoint(dbuf, g->face * units * (proj ? 8 : 16), x), GBX_EINVAL, "%s: buffer overlaps the field", who);
   const uint32_t *fs = g->face_slots + (uint64_t)face * g->face;
-  if (x->prec == GBX_F64) launch_pack_proj<GBX_F64>(K, grid, g->stream, x->d, fs, dbuf, g->face, x->M);
-  else launch_pack_proj<GBX_F32>(K, grid, g->stream, x->d, fs, dbuf, g->face, x->M);
+  if (!proj) {
+    hipLaunchKernelGGL(k_halo_pack, dim3(grid_for(g->face * units)), dim3(256), 0, st, (const uint4 *)x->d, fs,
+                       (uint4 *)dbuf, g->face, units);
+  } else {
+    const int K = face_projector(g->split_dim, face, dag);
+    const dim3 grid(grid_for(g->face * x->M));
+    if (x->prec == GBX_F64) launch_pack_proj<GBX_F64>(K, grid, st, x->d, fs, dbuf, g->face, x->M);
+    else launch_pack_proj<GBX_F32>(K, grid, st, x->d, fs, dbuf, g->face, x->M);
+  }
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }
 
-extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf)
+// The ghosts of `side` from dbuf (whole or projected face), enqueued on st.
+static int unpack_impl(gbx_spinor_s *x, int side, bool proj, const void *dbuf, hipStream_t st, const char *who)
 {
-  GBX_CHECK(dbuf, GBX_ENULL, "gbx_halo_unpack_proj: NULL buffer");
+  GBX_CHECK(dbuf, GBX_ENULL, "%s: NULL buffer", who);
   void *ghost;
   GBX_TRY(gbx_halo_ghost(x, side, &ghost));
-  GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "gbx_halo_unpack_proj: FP64 / FP32 fields");
-  GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "gbx_halo_unpack_proj: buffer not 16-byte aligned");
   const gbx_geom_s *g = x->geom;
+  const uint64_t units = slot_units(x);
+  if (!proj) {
+    if (dbuf == ghost) return GBX_OK;  // the face was packed or received in place
+    GBX_CHECK(disjoint(dbuf, g->face * units * 16, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+    GBX_HIP(hipMemcpyAsync(ghost, dbuf, g->face * units * 16, hipMemcpyDeviceToDevice, st));
+    return GBX_OK;
+  }
+  GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "%s: FP64 / FP32 fields", who);
+  GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
+  GBX_CHECK(disjoint(dbuf, g->face * units * 8, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
   const uint32_t nch = (uint32_t)spinor_nchunk(x->prec);
-  hipLaunchKernelGGL(k_halo_unpack_proj, dim3(grid_for(g->face * nch * x->M)), dim3(256), 0, g->stream,
-                     (const uint4 *)dbuf, (uint4 *)ghost, g->face, nch, x->M);
+  hipLaunchKernelGGL(k_halo_unpack_proj, dim3(grid_for(g->face * nch * x->M)), dim3(256), 0, st, (const uint4 *)dbuf,
+                     (uint4 *)ghost, g->face, nch, x->M);
   GBX_HIP(hipGetLastError());
   return GBX_OK;
+}
+
+extern "C" int gbx_halo_pack(gbx_spinor x, int face, void *dbuf)
+{
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_pack: NULL field");
+  return pack_impl(x, face, 0, false, dbuf, x->geom->stream, "gbx_halo_pack");
+}
+
+extern "C" int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf)
+{
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_pack_proj: NULL field");
+  return pack_impl(x, face, dag, true, dbuf, x->geom->stream, "gbx_halo_pack_proj");
 }
 
 extern "C" int gbx_halo_unpack(gbx_spinor x, int side, const void *dbuf)
 {
-  GBX_CHECK(dbuf, GBX_ENULL, "gbx_halo_unpack: NULL buffer");
-  void *ghost;
-  GBX_TRY(gbx_halo_ghost(x, side, &ghost));
-  uint64_t bytes;
-  GBX_TRY(gbx_halo_bytes(x, 0, &bytes));
-  GBX_HIP(hipMemcpyAsync(ghost, dbuf, bytes, hipMemcpyDeviceToDevice, x->geom->stream));
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_unpack: NULL field");
+  return unpack_impl(x, side, false, dbuf, x->geom->stream, "gbx_halo_unpack");
+}
+
+extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf)
+{
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_unpack_proj: NULL field");
+  return unpack_impl(x, side, true, dbuf, x->geom->stream, "gbx_halo_unpack_proj");
+}
+
+// One rank's application on one GPU, timed two ways (HIP events on the
+// geometry's stream, nrep applications after one warm-up each):
+//  serial:  pack both faces, move them, unpack, Dhop on every slot, all on
+//           one stream;
+//  overlap: pack, then move / unpack on a second stream while the interior
+//           slots run, then the face slots once the ghosts have landed.
+// The move stands in for the transfer between GPUs: a device-to-device copy
+// of each packed face into a receive buffer on this GPU.  The received faces
+// are this part's own, so the face sites of `out` hold no meaningful values
+// afterwards; the interior sites hold Dhop(in).
+namespace {
+struct OverlapRes {
+  void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  hipStream_t cs = nullptr;
+  hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+  ~OverlapRes()
+  {
+    if (cs) (void)hipStreamSynchronize(cs);
+    for (void *b : buf) (void)hipFree(b);
+    for (hipEvent_t e : ev)
+      if (e) (void)hipEventDestroy(e);
+    if (cs) (void)hipStreamDestroy(cs);
+  }
+};
+}  // namespace
+
+extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int proj, int nrep,
+                                     double *us_overlap, double *us_serial)
+{
+  GBX_CHECK(us_overlap && us_serial, GBX_ENULL, "gbx_halo_overlap_time: NULL result");
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_halo_overlap_time: nrep=%d", nrep);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_halo_overlap_time: dag=%d", dag);
+  GBX_TRY(dhop_check_operands(u, out, in));
+  GBX_TRY(part_ok(in, "gbx_halo_overlap_time"));
+  GBX_CHECK(!proj || in->prec == GBX_F64 || in->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "gbx_halo_overlap_time: projected faces of FP64 / FP32 fields only");
+  const gbx_geom_s *g = in->geom;
+  const uint64_t fb = g->face * slot_units(in) * (proj ? 8 : 16);
+  OverlapRes r;
+  for (void *&b : r.buf)
+    if (hipMalloc(&b, fb) != hipSuccess) {
+      (void)hipGetLastError();
+      b = nullptr;
+      return gbx_fail(GBX_ENOMEM, "gbx_halo_overlap_time: face buffers");
+    }
+  GBX_HIP(hipStreamCreate(&r.cs));
+  for (hipEvent_t &e : r.ev) GBX_HIP(hipEventCreate(&e));
+  hipStream_t ms = g->stream;
+  const gbx_kcfg k = dhop_range_cfg(u, in);
+  SlotRange interior, faces;
+  interior.end = g->ninterior;
+  faces.begin = g->ninterior;
+  // faces out (before anything else: the pack kernels need CUs, which the
+  // interior kernel would otherwise hold), then moved and into the ghosts:
+  // the top face feeds the minus side
+  auto pack = [&](hipStream_t st) -> int {
+    GBX_TRY(pack_impl(in, 0, dag, proj != 0, r.buf[0], st, "gbx_halo_overlap_time"));
+    return pack_impl(in, 1, dag, proj != 0, r.buf[1], st, "gbx_halo_overlap_time");
+  };
+  auto move_unpack = [&](hipStream_t st) -> int {
+    GBX_HIP(hipMemcpyAsync(r.buf[2], r.buf[0], fb, hipMemcpyDeviceToDevice, st));
+    GBX_HIP(hipMemcpyAsync(r.buf[3], r.buf[1], fb, hipMemcpyDeviceToDevice, st));
+    GBX_TRY(unpack_impl(in, 0, proj != 0, r.buf[3], st, "gbx_halo_overlap_time"));
+    return unpack_impl(in, 1, proj != 0, r.buf[2], st, "gbx_halo_overlap_time");
+  };
+  auto serial = [&]() -> int {
+    GBX_TRY(pack(ms));
+    GBX_TRY(move_unpack(ms));
+    return dhop_run(u, out, in, dag, k, ms);
+  };
+  auto overlapped = [&]() -> int {
+    GBX_TRY(pack(ms));
+    GBX_HIP(hipEventRecord(r.ev[2], ms));  // faces packed, previous face slots done
+    GBX_HIP(hipStreamWaitEvent(r.cs, r.ev[2], 0));
+    GBX_TRY(move_unpack(r.cs));
+    GBX_HIP(hipEventRecord(r.ev[3], r.cs));
+    GBX_TRY(dhop_run(u, out, in, dag, k, ms, interior));
+    GBX_HIP(hipStreamWaitEvent(ms, r.ev[3], 0));
+    return dhop_run(u, out, in, dag, k, ms, faces);
+  };
+  auto timed = [&](const std::function<int()> &f, double *us) -> int {
+    GBX_TRY(f());  // warm-up
+    GBX_HIP(hipEventRecord(r.ev[0], ms));
+    for (int i = 0; i < nrep; i++) GBX_TRY(f());
+    GBX_HIP(hipEventRecord(r.ev[1], ms));
+    GBX_HIP(hipEventSynchronize(r.ev[1]));
+    float t = 0.f;
+    GBX_HIP(hipEventElapsedTime(&t, r.ev[0], r.ev[1]));
+    *us = 1000.0 * (double)t / nrep;
+    return GBX_OK;
+  };
+  GBX_TRY(timed(serial, us_serial));
+  GBX_TRY(timed(overlapped, us_overlap));
   return GBX_OK;
 }
 

@@ -451,6 +451,17 @@ def dhop_range(U, out, inp, slot_begin, slot_end, dag=False):
     return out
 
 
+def halo_overlap_time(U, out, inp, dag=False, proj=True, nrep=20):
+    """(us_overlap, us_serial) per application of one part: face exchange
+    (faces moved on this GPU) overlapped with the interior slots, or run
+    before them on one stream.  Leaves out's face sites and inp's ghosts
+    without meaningful values."""
+    ov, se = C.c_double(), C.c_double()
+    check(lib.gbx_halo_overlap_time(U.h, out.h, inp.h, int(bool(dag)), int(bool(proj)), int(nrep), C.byref(ov),
+                                    C.byref(se)), "gbx_halo_overlap_time")
+    return ov.value, se.value
+
+
 def halo_exchange(fields, proj=False, dag=False, bufs=None):
     """Exchange the faces of the parts `fields` (part p = fields[p], all
     split the same way), as the ranks of a domain decomposition would:
@@ -753,5 +764,5 @@ __all__ = [
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
-    "part_merge", "DeviceBuffer", "part_tables",
+    "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time",
 ]

@@ -341,6 +341,18 @@ int gbx_halo_unpack(gbx_spinor x, int side, const void *dbuf);
 /* is bit-identical to the one from whole-spinor ghosts.                   */
 int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf);
 int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf);
+/* One rank's application, timed on this GPU (microseconds per application, */
+/* HIP events on the geometry's stream, nrep after one warm-up each):       */
+/*  serial:  pack both faces, move each into a receive buffer (a device-to- */
+/*           device copy standing in for the transfer between GPUs), unpack, */
+/*           Dhop on every slot, all on the geometry's stream;              */
+/*  overlap: pack, then move / unpack on a second stream while the interior */
+/*           slots run, then the face slots after an event.                 */
+/* The moved faces are the part's own, so afterwards out's face sites hold  */
+/* no meaningful values (its interior sites hold Dhop(in)), and the ghosts */
+/* of in are overwritten.  proj = 1: spin-projected faces (FP64 / FP32).   */
+int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int proj, int nrep, double *us_overlap,
+                          double *us_serial);
 /* Device buffers for faces (hipMalloc; destroy synchronises the device),  */
 /* and a blocking copy of one to the host.                                  */
 int gbx_buffer_create(uint64_t bytes, void **dptr);

@@ -170,7 +170,30 @@ def test_halo_errors(gpu, gbx, problem):
         parts.src[0].halo_pack(2, buf.ptr)
     with pytest.raises(gbx.GbxError):
         parts.src[0].halo_pack(0, parts.src[0].halo_ghost(0))  # overlaps the field
+    with pytest.raises(gbx.GbxError):
+        parts.src[0].halo_unpack(1, parts.src[0].halo_ghost(0))  # overlaps the field
+    parts.src[0].halo_unpack(0, parts.src[0].halo_ghost(0))      # in place: nothing to copy
     with pytest.raises(gbx.GbxError):  # the tiled kernel takes whole lattices only
         gbx.dhop(parts.U[0], parts.dst[0], parts.src[0], cfg=gbx.kcfg(0, 256, kind=2, tile=gbx.pack_tile((2, 2, 2, 2))))
     with pytest.raises(gbx.GbxError):
         gbx.Geometry.part_of(p.dims, 3, 3, 0)
+
+
+@pytest.mark.parametrize("proj", [False, True])
+def test_overlap_timing_entry(gpu, gbx, problem, proj):
+    """The timed overlap schedule runs, and the interior it computes is the
+    true Dhop (the face sites are stand-in values by design)."""
+    p = problem((4, 4, 4, 12), 4)
+    parts = Parts(gbx, p, 3, 2, "f64")
+    parts.exchange()
+    for U, d, s in zip(parts.U, parts.dst, parts.src):
+        gbx.dhop(U, d, s)
+    want = parts.dst[0].download()
+    g0 = parts.geoms[0]
+    ov, se = gbx.halo_overlap_time(parts.U[0], parts.dst[0], parts.src[0], proj=proj, nrep=3)
+    assert ov > 0 and se > 0
+    got = parts.dst[0].download()
+    # interior sites (local t = 1 .. 4 of 6) are unchanged
+    got_i = got[0].reshape(6, 4 * 4 * 4, -1)[1:5]
+    want_i = want[0].reshape(6, 4 * 4 * 4, -1)[1:5]
+    assert np.array_equal(got_i, want_i)
