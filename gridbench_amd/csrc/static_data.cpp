@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <limits>
 #include <string>
 #include <vector>
 
@@ -78,7 +79,8 @@ int parse_array(const std::string &t, size_t p, bool integer, std::vector<T> &ou
     if (integer) {
       errno = 0;
       const unsigned long long v = strtoull(s, &e, 0);
-      GBX_CHECK(e != s && errno == 0, GBX_EINVAL, "static_data: bad integer in %s at element %zu", name, out.size());
+      GBX_CHECK(e != s && errno == 0 && *s != '-' && v <= (unsigned long long)std::numeric_limits<T>::max(), GBX_EINVAL,
+                "static_data: bad integer in %s at element %zu", name, out.size());
       out.push_back((T)v);
     } else {
       const double v = strtod(s, &e);
@@ -185,10 +187,13 @@ extern "C" int gbx_static_data_read_header(const char *h_path, uint64_t *nsite, 
   GBX_TRY(read_all(h_path, t));
   const size_t a = t.find("nsite"), b = t.find("Ls");
   GBX_CHECK(a != std::string::npos && b != std::string::npos, GBX_EINVAL, "static_data: %s has no nsite / Ls", h_path);
+  const size_t ea = t.find('=', a), eb = t.find('=', b);
+  GBX_CHECK(ea != std::string::npos && eb != std::string::npos, GBX_EINVAL, "static_data: no value for nsite / Ls in %s",
+            h_path);
   unsigned long long ns = 0;
   int ls = 0;
-  GBX_CHECK(sscanf(t.c_str() + t.find('=', a), "= %llu", &ns) == 1 && sscanf(t.c_str() + t.find('=', b), "= %d", &ls) == 1,
-            GBX_EINVAL, "static_data: cannot parse nsite / Ls in %s", h_path);
+  GBX_CHECK(sscanf(t.c_str() + ea, "= %llu", &ns) == 1 && sscanf(t.c_str() + eb, "= %d", &ls) == 1, GBX_EINVAL,
+            "static_data: cannot parse nsite / Ls in %s", h_path);
   GBX_CHECK(ns >= 1 && ls >= 1, GBX_EINVAL, "static_data: nsite=%llu Ls=%d", ns, ls);
   *nsite = ns;
   *Ls = ls;

@@ -92,6 +92,34 @@ def test_split_files_and_errors(gbx, tmp_path):
     assert e.value.status == 4
 
 
+def test_malformed_blobs_are_rejected(gbx, tmp_path):
+    """Truncated arrays, negative or oversized integers and headers without
+    values fail with GBX_EINVAL instead of being read past or wrapped."""
+    nsite, Ls, nsimd, U, Phi, Psi, nbr, prm = small_blob(nsite=4, Ls=2, nsimd=1)
+    cc, h = tmp_path / "s.cc", tmp_path / "s.h"
+    gbx.static_data_write(cc, h, nsite, Ls, nsimd, U, Phi, Psi, nbr, prm)
+    text = cc.read_text()
+    bad = tmp_path / "bad.cc"
+    for cut in (0, 10, len(text) // 3, len(text) // 2, len(text) - 5):
+        bad.write_text(text[:cut])
+        with pytest.raises(gbx.GbxError) as e:
+            gbx.static_data_read(bad)
+        assert e.value.status == 1, cut
+    for arrays, status in (("uint64_t nbr_static[] = { 0, -1, 0 };", 1),
+                           ("uint64_t nbr_static[] = { 0, 99999999999999999999999, 0 };", 1),
+                           ("uint8_t prm_static[] = { 0, 300, 0 };", 4)):  # 4: an invalid table entry
+        bad.write_text(text.split("uint64_t nbr_static")[0] + arrays + "\n")
+        with pytest.raises(gbx.GbxError) as e:
+            gbx.static_data_read(bad)
+        assert e.value.status == status, arrays
+    hb = tmp_path / "bad.h"
+    for header in ("const uint64_t nsite ; const int Ls ;", "const uint64_t nsite = 0 ; const int Ls = 2 ;", ""):
+        hb.write_text(header)
+        with pytest.raises(gbx.GbxError) as e:
+            gbx.static_data_read_header(hb)
+        assert e.value.status == 1, header
+
+
 @pytest.mark.skipif(not os.path.isdir(REFERENCE), reason="reference sources not present")
 def test_links_against_the_references_declarations(gbx, tmp_path):
     """The written arrays satisfy the extern declarations the reference's
