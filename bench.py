@@ -30,14 +30,14 @@ Also reported, each with its own seeds:
              {8,12,16}, Nrhs in 1..8, drawn with seed 505), FP32, one
              persistent graph-captured launch; aggregate GFLOP/s and the
              per-lattice error against FP64.
-  halo       24^4 x Ls16 FP64 cut along T into 2 and 4 slabs (domain parts
-             with ghost slots): the assembled result against the whole
-             lattice's, and for one slab the Dhop time on its share, on its
-             interior and on its faces, the face pack / unpack times, the
-             face bytes, and one rank's whole schedule with the faces moved
-             on this GPU (serial, and overlapped with the interior on a
-             second stream); the xGMI transfer time is modelled (bytes /
-             link rate), not measured.
+  halo       24^4 x Ls16 FP64 cut by process grids 1x1x1x2, 1x1x1x4 and
+             1x2x2x2 (domain parts with ghost slots): the assembled result
+             against the whole lattice's, and for one block the Dhop time on
+             its share, on its interior and on its faces, the face pack /
+             unpack times, the face bytes, and one rank's whole schedule with
+             the faces moved on this GPU (serial, and overlapped with the
+             interior on a second stream); the xGMI transfer time is
+             modelled (bytes / link rate), not measured.
 
 Launch configurations come from the autotuner; its results for this machine
 are cached in gridbench_amd/tuned_mi355x.jsonl (re-validated on every run).
@@ -456,78 +456,93 @@ def host_timed(fn, geom, nrep):
 
 
 def halo_section(gbx, tune, dbl, psi, ref64, nrep):
-    """One GPU's share of a domain-decomposed 24^4 x Ls16 FP64 Dhop: the
-    lattice cut along T into 2 and 4 slabs.  Every slab is built and its
-    faces exchanged on this GPU (checks the assembled result against the
-    whole-lattice application); part 0 is timed: Dhop on its whole share,
-    on the interior slots alone, on the face slots alone, and the face
-    pack / unpack kernels.  The xGMI transfer itself needs two GPUs: its
-    time is the face bytes at one link's published rate (a model, labelled
-    as such)."""
+    """One GPU's share of a domain-decomposed 24^4 x Ls16 FP64 Dhop, for
+    process grids 1x1x1x2 and 1x1x1x4 (slabs along T) and 1x2x2x2 (one
+    8-GPU node).  Every block is built and its faces exchanged on this GPU
+    (checks the assembled result against the whole-lattice application);
+    block 0 is timed: Dhop on its whole share, on the interior slots alone,
+    on the face slots alone, the face pack / unpack kernels, and one rank's
+    whole schedule with the faces moved on this GPU.  The xGMI transfer
+    itself needs several GPUs: its time is the face bytes at one link's
+    published rate (a model, labelled as such)."""
     rows = []
-    for nparts in (2, 4):
-        geoms = [gbx.Geometry.part_of(DIMS, 3, nparts, k) for k in range(nparts)]
-        loc = gbx.part_split(DIMS, 3, nparts, psi[None])
-        U = [gbx.GaugeField(g, "f64").upload(dbl) for g in geoms]
-        x = [gbx.SpinorField(g, LS, 1, "f64").upload(a) for g, a in zip(geoms, loc)]
-        y = [gbx.SpinorField(g, LS, 1, "f64") for g in geoms]
+    for grid in ((1, 1, 1, 2), (1, 1, 1, 4), (1, 2, 2, 2)):
+        coords = gbx.grid_coords(grid)
+        geoms = {c: gbx.Geometry.part_of(DIMS, grid, c) for c in coords}
+        loc = gbx.part_split(DIMS, grid, psi[None])
+        U = {c: gbx.GaugeField(g, "f64").upload(dbl) for c, g in geoms.items()}
+        x = {c: gbx.SpinorField(g, LS, 1, "f64").upload(loc[c]) for c, g in geoms.items()}
+        y = {c: gbx.SpinorField(g, LS, 1, "f64") for c, g in geoms.items()}
         del loc
-        best, _ = tune(U[0], y[0], x[0])
-        nb = x[0].halo_bytes(proj=True)
-        bufs = [(gbx.DeviceBuffer(nb), gbx.DeviceBuffer(nb)) for _ in range(nparts)]
+        c0 = coords[0]
+        g0, u0, x0, y0 = geoms[c0], U[c0], x[c0], y[c0]
+        best, _ = tune(u0, y0, x0)
+        bufs = gbx.halo_buffers(x)
         gbx.halo_exchange(x, proj=True, bufs=bufs)
-        for u_, y_, x_ in zip(U, y, x):
-            gbx.dhop(u_, y_, x_)
-        got = gbx.part_merge(DIMS, 3, [y_.download() for y_ in y])[0]
+        for c in coords:
+            gbx.dhop(U[c], y[c], x[c])
+        got = gbx.part_merge(DIMS, grid, {c: y[c].download() for c in coords})[0]
         err = float(np.linalg.norm(got - ref64) / np.linalg.norm(ref64))
         del got
-        g0, u0, x0, y0 = geoms[0], U[0], x[0], y[0]
+        cut = [d for d in range(4) if grid[d] > 1]
         us_part = gbx.dhop_time(u0, y0, x0, nrep=nrep)
         nint, vol = g0.ninterior, g0.volume
         us_int = host_timed(lambda: gbx.dhop_range(u0, y0, x0, 0, nint), g0, nrep)
         us_bnd = host_timed(lambda: gbx.dhop_range(u0, y0, x0, nint, vol), g0, nrep)
-        us_pack = host_timed(lambda: x0.halo_pack(1, x[1].halo_ghost(0)), g0, nrep)
-        us_pack_proj = host_timed(lambda: x0.halo_pack(1, bufs[0][1].ptr, proj=True), g0, nrep)
-        us_unpack_proj = host_timed(lambda: x0.halo_unpack(0, bufs[1][1].ptr, proj=True), g0, nrep)
+
+        def pack_all(proj):
+            for d in cut:
+                for f in (0, 1):
+                    x0.halo_pack(d, f, bufs[(c0, d)][f].ptr if proj else y0.halo_ghost(d, f), proj=proj)
+
+        def unpack_all():
+            for d in cut:
+                for side in (0, 1):
+                    x0.halo_unpack(d, side, bufs[(c0, d)][1 - side].ptr, proj=True)
+
+        # (whole faces are packed into y0's own ghosts: a scratch target of the right size)
+        us_pack = host_timed(lambda: pack_all(False), g0, nrep)
+        us_pack_proj = host_timed(lambda: pack_all(True), g0, nrep)
+        us_unpack_proj = host_timed(unpack_all, g0, nrep)
         # one rank's schedule on this GPU (faces moved by a device copy)
         us_ov, us_se = gbx.halo_overlap_time(u0, y0, x0, proj=True, nrep=nrep)
         us_ov_w, us_se_w = gbx.halo_overlap_time(u0, y0, x0, proj=False, nrep=nrep)
-        fb, fbp = x0.halo_bytes(), nb
-        # per application a part sends two faces and receives two: with two
-        # parts both go to the one neighbour (one link), with more parts to
-        # two neighbours over two links at once
-        faces_per_link = 2 if nparts == 2 else 1
-        xfer_us = faces_per_link * fbp / (XGMI_GBPS * 1e9) * 1e6
+        # Per application a block sends two faces per cut dimension.  With two
+        # blocks along a dimension both go to one neighbour (one link), with
+        # more to two neighbours; different neighbours use different links,
+        # which run at once.
+        xfer_us = max((2 if grid[d] == 2 else 1) * x0.halo_bytes(d, True) for d in cut) / (XGMI_GBPS * 1e9) * 1e6
         # The pack and unpack kernels need CUs, which the interior kernel
         # holds while it runs (the measured single-GPU schedule shows no
         # overlap of kernels on two streams): only the transfer, on copy
         # engines, overlaps the interior.
-        overlapped = 2 * us_pack_proj + max(us_int, xfer_us) + 2 * us_unpack_proj + us_bnd
-        r = {"nparts": nparts, "split_dim": "T", "local_dims": list(g0.dims), "local_sites": vol,
-             "interior_sites": nint, "face_sites": g0.face, "kernel_cfg": best.as_dict(),
+        overlapped = us_pack_proj + max(us_int, xfer_us) + us_unpack_proj + us_bnd
+        r = {"grid": list(grid), "local_dims": list(g0.dims), "local_sites": vol, "interior_sites": nint,
+             "face_sites": {"xyzt"[d]: g0.face[d] for d in cut}, "kernel_cfg": best.as_dict(),
              "rel_l2_assembled_vs_whole_lattice": float(f"{err:.3e}"),
              "us_dhop_local_share_events": round(us_part, 2), "us_dhop_interior_host": round(us_int, 2),
-             "us_dhop_faces_host": round(us_bnd, 2), "face_bytes_full": fb, "face_bytes_projected": fbp,
-             "us_pack_full_face": round(us_pack, 2), "us_pack_projected_face": round(us_pack_proj, 2),
-             "us_unpack_projected_face": round(us_unpack_proj, 2),
+             "us_dhop_faces_host": round(us_bnd, 2),
+             "face_bytes_full": {"xyzt"[d]: x0.halo_bytes(d) for d in cut},
+             "face_bytes_projected": {"xyzt"[d]: x0.halo_bytes(d, True) for d in cut},
+             "us_pack_all_faces_full": round(us_pack, 2), "us_pack_all_faces_projected": round(us_pack_proj, 2),
+             "us_unpack_all_faces_projected": round(us_unpack_proj, 2),
              "us_application_serial_local_move": round(us_se, 2),
              "us_application_overlapped_local_move": round(us_ov, 2),
              "us_application_serial_local_move_whole_faces": round(us_se_w, 2),
              "us_application_overlapped_local_move_whole_faces": round(us_ov_w, 2),
-             "model_xgmi_gbps_per_link": XGMI_GBPS, "model_projected_faces_per_link": faces_per_link,
-             "model_us_xgmi": round(xfer_us, 1),
-             "model_us_per_application_overlapped": round(overlapped, 1),
+             "model_xgmi_gbps_per_link": XGMI_GBPS, "model_us_xgmi": round(xfer_us, 1),
+             "model_us_per_application": round(overlapped, 1),
              "gflops_local_share": round(1320.0 * vol * LS / (us_part * 1e-6) / 1e9, 1)}
         log(f"halo {r}")
         rows.append(r)
-        for f in [*bufs]:
-            f[0].close()
-            f[1].close()
-        for f in (*y, *x, *U):
+        for pair in bufs.values():
+            pair[0].close()
+            pair[1].close()
+        for f in (*y.values(), *x.values(), *U.values()):
             f.close()
     return {"dims": list(DIMS), "Ls": LS, "prec": "f64",
-            "note": "one GPU holds every slab; the xGMI time is the model bytes / link rate, not a measurement",
-            "splits": rows}
+            "note": "one GPU holds every block; the xGMI time is the model bytes / link rate, not a measurement",
+            "grids": rows}
 
 
 def main():

@@ -97,9 +97,9 @@ PROTOTYPES = {
     "gbx_geom_create_levels": (i32, [pi32, i32, pi32, C.POINTER(vp)]),
     "gbx_geom_create_rb": (i32, [pi32, pi32, C.POINTER(vp), C.POINTER(vp)]),
     "gbx_geom_parity": (i32, [vp, pi32]),
-    "gbx_geom_create_part": (i32, [pi32, i32, i32, i32, pi32, C.POINTER(vp)]),
-    "gbx_part_tables": (i32, [pi32, i32, i32, i32, pi32, pu32, pu32, pu32, pu32, pu64]),
-    "gbx_geom_part_info": (i32, [vp, pi32, pi32, pi32, pu64, pu64]),
+    "gbx_geom_create_part": (i32, [pi32, pi32, pi32, pi32, C.POINTER(vp)]),
+    "gbx_part_tables": (i32, [pi32, pi32, pi32, pi32, pu32, pu32, pu32, pu32, pu64]),
+    "gbx_geom_part_info": (i32, [vp, pi32, pi32, pu64, pu64]),
     "gbx_geom_destroy": (i32, [vp]),
     "gbx_geom_volume": (i32, [vp, pu64]),
     "gbx_geom_stream": (i32, [vp, C.POINTER(vp)]),
@@ -124,12 +124,12 @@ PROTOTYPES = {
     "gbx_spinor_fill_gaussian": (i32, [vp, u64]),
     "gbx_dhop": (i32, [vp, vp, vp, i32]),
     "gbx_dhop_range": (i32, [vp, vp, vp, i32, u64, u64]),
-    "gbx_halo_bytes": (i32, [vp, i32, pu64]),
-    "gbx_halo_ghost": (i32, [vp, i32, C.POINTER(vp)]),
-    "gbx_halo_pack": (i32, [vp, i32, vp]),
-    "gbx_halo_unpack": (i32, [vp, i32, vp]),
-    "gbx_halo_pack_proj": (i32, [vp, i32, i32, vp]),
-    "gbx_halo_unpack_proj": (i32, [vp, i32, vp]),
+    "gbx_halo_bytes": (i32, [vp, i32, i32, pu64]),
+    "gbx_halo_ghost": (i32, [vp, i32, i32, C.POINTER(vp)]),
+    "gbx_halo_pack": (i32, [vp, i32, i32, vp]),
+    "gbx_halo_unpack": (i32, [vp, i32, i32, vp]),
+    "gbx_halo_pack_proj": (i32, [vp, i32, i32, i32, vp]),
+    "gbx_halo_unpack_proj": (i32, [vp, i32, i32, vp]),
     "gbx_halo_overlap_time": (i32, [vp, vp, vp, i32, i32, i32, pdbl, pdbl]),
     "gbx_buffer_create": (i32, [u64, C.POINTER(vp)]),
     "gbx_buffer_destroy": (i32, [vp]),

@@ -132,7 +132,10 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
            g->dims[2], g->dims[3]);
   std::string s = buf;
   if (g->parity >= 0) s += g->parity ? "_odd" : "_even";
-  if (g->nghost) s += "_part" + std::to_string(g->split_dim);  // ghost slots: no tiled kernel
+  if (g->nghost) {  // a domain part: ghost slots, no tiled kernel
+    snprintf(buf, sizeof buf, "_part%dx%dx%dx%d", g->grid[0], g->grid[1], g->grid[2], g->grid[3]);
+    s += buf;
+  }
   for (int l = 0; l < g->nlevels; l++) {
     snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
              g->levels[l * 4 + 3]);

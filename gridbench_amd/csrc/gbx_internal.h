@@ -92,19 +92,21 @@ struct gbx_geom_s {
   // Sites of the host gauge array an upload reads: the whole lattice (2*vol
   // for a red-black half, the global volume for a domain part).
   uint64_t gauge_vol = 0;
-  // Domain part (gbx_geom_create_part): this geometry is part `part` of
-  // `nparts` slabs of the global lattice `gdims` along `split_dim`.  Fields
-  // carry nghost = 2*face ghost slots after the vol local ones: slots vol ..
-  // vol+face-1 hold the previous part's top face, vol+face .. vol+2*face-1
-  // the next part's bottom face; nbr points into them across the cut.
-  // face_slots[0][f] / [1][f]: the local slots of the bottom (coordinate 0)
-  // and top (coordinate dims[split_dim]-1) faces, f = lexicographic index
-  // over the other three dimensions.
-  int split_dim = -1, nparts = 1, part = 0;
+  // Domain part (gbx_geom_create_part): the block at grid coordinate
+  // `coord` of the global lattice `gdims` cut into grid[0] x .. x grid[3]
+  // equal blocks.  Fields carry nghost ghost slots after the vol local ones:
+  // for each cut dimension d (grid[d] > 1), ascending, face[d] slots of the
+  // previous part's top face (ghost_base[d] ..) then face[d] slots of the
+  // next part's bottom face; nbr points into them across the cuts.
+  // face_slots + fs_base[d]: [2][face[d]] local slots of the bottom
+  // (coordinate 0) and top (coordinate dims[d]-1) faces of dimension d, face
+  // index f = lexicographic over the other three dimensions.
+  int grid[4] = {1, 1, 1, 1}, coord[4] = {0, 0, 0, 0};
   int gdims[4] = {0, 0, 0, 0};
+  uint64_t face[4] = {0, 0, 0, 0}, ghost_base[4] = {0, 0, 0, 0}, fs_base[4] = {0, 0, 0, 0};
   // Slots 0 .. ninterior-1 are the interior sites (no leg reaches a ghost),
-  // slots ninterior .. vol-1 the two faces.
-  uint64_t face = 0, nghost = 0, ninterior = 0;
+  // slots ninterior .. vol-1 the face sites.
+  uint64_t nghost = 0, ninterior = 0;
   uint32_t *face_slots = nullptr;
 };
 

@@ -547,27 +547,32 @@ extern "C" int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geo
   return GBX_OK;
 }
 
-// Part `part` of `nparts` slabs along `split_dim`, with ghost slots for the
-// two cut faces (the local share of a domain-decomposed lattice; the faces
-// are exchanged with gbx_halo_pack / gbx_halo_unpack).
+// The block at `coord` of the lattice cut into grid[0] x .. x grid[3] equal
+// blocks, with ghost slots for its cut faces (the local share of a domain-
+// decomposed lattice; the faces are exchanged with gbx_halo_pack / _unpack).
 namespace {
 
 struct PartTables {
   int ld[4], blk[4];
-  uint64_t vol = 0, gvol = 0, face = 0, ninterior = 0;
+  uint64_t vol = 0, gvol = 0, ninterior = 0, nghost = 0;
+  uint64_t face[4] = {0, 0, 0, 0}, ghost_base[4] = {0, 0, 0, 0}, fs_base[4] = {0, 0, 0, 0};
   std::vector<uint32_t> slot_of_site, site_of_slot, gsite, nbr, fslots;
 };
 
-int part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4], PartTables &t)
+int part_tables(const int dims[4], const int grid[4], const int coord[4], const int block[4], PartTables &t)
 {
-  GBX_CHECK(dims, GBX_ENULL, "domain part: NULL dims");
-  GBX_CHECK(split_dim >= 0 && split_dim < 4, GBX_EINVAL, "domain part: split_dim=%d", split_dim);
-  GBX_CHECK(nparts >= 2 && part >= 0 && part < nparts, GBX_EINVAL, "domain part: part %d of %d", part, nparts);
-  for (int d = 0; d < 4; d++) GBX_CHECK(dims[d] >= 1 && dims[d] <= 1024, GBX_EINVAL, "domain part: dims[%d]=%d", d, dims[d]);
-  GBX_CHECK(dims[split_dim] % nparts == 0, GBX_EINVAL, "domain part: %d parts do not divide dims[%d]=%d", nparts, split_dim,
-            dims[split_dim]);
+  GBX_CHECK(dims && grid && coord, GBX_ENULL, "domain part: NULL dims / grid / coord");
+  int ncut = 0;
+  for (int d = 0; d < 4; d++) {
+    GBX_CHECK(dims[d] >= 1 && dims[d] <= 1024, GBX_EINVAL, "domain part: dims[%d]=%d", d, dims[d]);
+    GBX_CHECK(grid[d] >= 1 && dims[d] % grid[d] == 0, GBX_EINVAL, "domain part: grid[%d]=%d does not divide %d", d,
+              grid[d], dims[d]);
+    GBX_CHECK(coord[d] >= 0 && coord[d] < grid[d], GBX_EINVAL, "domain part: coord[%d]=%d of %d", d, coord[d], grid[d]);
+    ncut += grid[d] > 1;
+  }
+  GBX_CHECK(ncut >= 1, GBX_EINVAL, "domain part: the grid cuts no dimension (use gbx_geom_create)");
   int *ld = t.ld, *blk = t.blk;
-  for (int d = 0; d < 4; d++) ld[d] = d == split_dim ? dims[d] / nparts : dims[d];
+  for (int d = 0; d < 4; d++) ld[d] = dims[d] / grid[d];
   for (int d = 0; d < 4; d++) blk[d] = block ? block[d] : largest_divisor_le(ld[d], 6);
   uint64_t vol;
   GBX_TRY(check_dims_levels(ld, 1, blk, &vol));
@@ -575,27 +580,41 @@ int part_tables(const int dims[4], int split_dim, int nparts, int part, const in
   t.gvol = 1;
   for (int d = 0; d < 4; d++) t.gvol *= (uint64_t)dims[d];
   GBX_CHECK(t.gvol < (1ull << 32), GBX_EINVAL, "domain part: global volume %llu", (unsigned long long)t.gvol);
-  const uint64_t face = vol / (uint64_t)ld[split_dim];
-  t.face = face;
+  uint64_t gb = vol, fb = 0;
+  for (int d = 0; d < 4; d++)
+    if (grid[d] > 1) {
+      t.face[d] = vol / (uint64_t)ld[d];
+      t.ghost_base[d] = gb;
+      t.fs_base[d] = fb;
+      gb += 2 * t.face[d];
+      fb += 2 * t.face[d];
+    }
+  t.nghost = gb - vol;
+  GBX_CHECK(gb < (1ull << 32), GBX_EINVAL, "domain part: %llu slots", (unsigned long long)gb);
   auto coords = [&](uint64_t site, int c[4]) {
     for (int d = 0; d < 4; d++) {
       c[d] = (int)(site % (uint64_t)ld[d]);
       site /= (uint64_t)ld[d];
     }
   };
-  // lexicographic index over the three dimensions other than split_dim
-  auto face_index = [&](const int c[4]) {
+  // lexicographic index over the three dimensions other than d
+  auto face_index = [&](const int c[4], int d) {
     uint64_t f = 0, stride = 1;
-    for (int d = 0; d < 4; d++) {
-      if (d == split_dim) continue;
-      f += (uint64_t)c[d] * stride;
-      stride *= (uint64_t)ld[d];
+    for (int e = 0; e < 4; e++) {
+      if (e == d) continue;
+      f += (uint64_t)c[e] * stride;
+      stride *= (uint64_t)ld[e];
     }
     return f;
   };
+  auto on_face = [&](const int c[4]) {
+    for (int d = 0; d < 4; d++)
+      if (grid[d] > 1 && (c[d] == 0 || c[d] == ld[d] - 1)) return true;
+    return false;
+  };
   // Blocked order, then stably partitioned: the interior sites (no leg
-  // crosses the cut) take slots 0 .. ninterior-1, the two faces the rest,
-  // so the interior and the boundary are each one slot range.
+  // crosses a cut) take slots 0 .. ninterior-1, the face sites the rest,
+  // so the interior and the faces are each one slot range.
   const std::vector<uint32_t> bslot = blocked_slots(ld, 1, blk, vol);
   std::vector<uint32_t> border(vol);
   for (uint64_t site = 0; site < vol; site++) border[bslot[site]] = (uint32_t)site;
@@ -607,36 +626,39 @@ int part_tables(const int dims[4], int split_dim, int nparts, int part, const in
       const uint32_t site = border[b];
       int c[4];
       coords(site, c);
-      const bool bnd = c[split_dim] == 0 || c[split_dim] == ld[split_dim] - 1;
-      if (bnd != (pass == 1)) continue;
+      if (on_face(c) != (pass == 1)) continue;
       t.slot_of_site[site] = (uint32_t)n;
       t.site_of_slot[n] = site;
       n++;
+      if (pass == 0) t.ninterior = n;
     }
-  t.ninterior = ld[split_dim] > 2 ? vol - 2 * face : 0;
-  // Neighbours: a leg that crosses the cut reads ghost slot vol + f (the
-  // previous part's top face) or vol + face + f (the next part's bottom face).
+  // Neighbours: a leg that crosses the cut of dimension d reads ghost slot
+  // ghost_base[d] + f (the previous part's top face) or ghost_base[d] +
+  // face[d] + f (the next part's bottom face).
   t.gsite.assign(vol, 0);
   t.nbr.assign(vol * 8, 0);
-  t.fslots.assign(2 * face, 0);
-  const int off = part * ld[split_dim];
+  t.fslots.assign(fb, 0);
   for (uint64_t q = 0; q < vol; q++) {
     const uint64_t site = t.site_of_slot[q];
     int c[4];
     coords(site, c);
     uint64_t gs = 0, stride = 1;
     for (int d = 0; d < 4; d++) {
-      gs += (uint64_t)(d == split_dim ? c[d] + off : c[d]) * stride;
+      gs += (uint64_t)(c[d] + coord[d] * ld[d]) * stride;
       stride *= (uint64_t)dims[d];
     }
     t.gsite[q] = (uint32_t)gs;
-    const uint64_t f = face_index(c);
-    if (c[split_dim] == 0) t.fslots[f] = (uint32_t)q;
-    if (c[split_dim] == ld[split_dim] - 1) t.fslots[face + f] = (uint32_t)q;
+    for (int d = 0; d < 4; d++) {
+      if (grid[d] == 1) continue;
+      const uint64_t f = face_index(c, d);
+      if (c[d] == 0) t.fslots[t.fs_base[d] + f] = (uint32_t)q;
+      if (c[d] == ld[d] - 1) t.fslots[t.fs_base[d] + t.face[d] + f] = (uint32_t)q;
+    }
     for (int mu = 0; mu < 8; mu++) {
       const int d = mu & 3;
-      if (d == split_dim && mu < 4 && c[d] == ld[d] - 1) t.nbr[q * 8 + mu] = (uint32_t)(vol + face + f);
-      else if (d == split_dim && mu >= 4 && c[d] == 0) t.nbr[q * 8 + mu] = (uint32_t)(vol + f);
+      const bool cut = grid[d] > 1;
+      if (cut && mu < 4 && c[d] == ld[d] - 1) t.nbr[q * 8 + mu] = (uint32_t)(t.ghost_base[d] + t.face[d] + face_index(c, d));
+      else if (cut && mu >= 4 && c[d] == 0) t.nbr[q * 8 + mu] = (uint32_t)(t.ghost_base[d] + face_index(c, d));
       else t.nbr[q * 8 + mu] = t.slot_of_site[nbr_site(site, mu, ld)];
     }
   }
@@ -646,15 +668,15 @@ int part_tables(const int dims[4], int split_dim, int nparts, int part, const in
 }  // namespace
 
 // The tables of a domain part, on the host (no device needed): site_of_slot
-// and gsite [V_local], nbr [V_local][8], face_slots [2][face]; sizes from
-// V_local = prod(dims) / nparts and face = V_local / (dims[split_dim] / nparts).
-extern "C" int gbx_part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4],
+// and gsite [V_local], nbr [V_local][8], face_slots: for each cut dimension d
+// ascending, [2][face_d] with face_d = V_local / (dims[d] / grid[d]).
+extern "C" int gbx_part_tables(const int dims[4], const int grid[4], const int coord[4], const int block[4],
                                uint32_t *site_of_slot, uint32_t *gsite, uint32_t *nbr, uint32_t *face_slots,
                                uint64_t *ninterior)
 {
   GBX_CHECK(site_of_slot && gsite && nbr && face_slots && ninterior, GBX_ENULL, "gbx_part_tables: NULL argument");
   PartTables t;
-  GBX_TRY(part_tables(dims, split_dim, nparts, part, block, t));
+  GBX_TRY(part_tables(dims, grid, coord, block, t));
   std::copy(t.site_of_slot.begin(), t.site_of_slot.end(), site_of_slot);
   std::copy(t.gsite.begin(), t.gsite.end(), gsite);
   std::copy(t.nbr.begin(), t.nbr.end(), nbr);
@@ -663,24 +685,27 @@ extern "C" int gbx_part_tables(const int dims[4], int split_dim, int nparts, int
   return GBX_OK;
 }
 
-extern "C" int gbx_geom_create_part(const int dims[4], int split_dim, int nparts, int part, const int block[4],
+extern "C" int gbx_geom_create_part(const int dims[4], const int grid[4], const int coord[4], const int block[4],
                                     gbx_geom *out)
 {
-  GBX_CHECK(dims && out, GBX_ENULL, "gbx_geom_create_part: NULL argument");
+  GBX_CHECK(out, GBX_ENULL, "gbx_geom_create_part: NULL out");
   *out = nullptr;
   PartTables t;
-  GBX_TRY(part_tables(dims, split_dim, nparts, part, block, t));
+  GBX_TRY(part_tables(dims, grid, coord, block, t));
   GBX_TRY(gbx_require_device());
   int device;
   GBX_HIP(hipGetDevice(&device));
   gbx_geom_s *g = geom_new(t.ld, 1, t.blk, t.vol, device);
   g->gauge_vol = t.gvol;
-  g->split_dim = split_dim;
-  g->nparts = nparts;
-  g->part = part;
-  for (int d = 0; d < 4; d++) g->gdims[d] = dims[d];
-  g->face = t.face;
-  g->nghost = 2 * t.face;
+  for (int d = 0; d < 4; d++) {
+    g->grid[d] = grid[d];
+    g->coord[d] = coord[d];
+    g->gdims[d] = dims[d];
+    g->face[d] = t.face[d];
+    g->ghost_base[d] = t.ghost_base[d];
+    g->fs_base[d] = t.fs_base[d];
+  }
+  g->nghost = t.nghost;
   g->ninterior = t.ninterior;
   g->slot_of_site_h = std::move(t.slot_of_site);
   g->site_of_slot_h = std::move(t.site_of_slot);
@@ -695,13 +720,14 @@ extern "C" int gbx_geom_create_part(const int dims[4], int split_dim, int nparts
   return GBX_OK;
 }
 
-extern "C" int gbx_geom_part_info(gbx_geom g, int *split_dim, int *nparts, int *part, uint64_t *face, uint64_t *ninterior)
+extern "C" int gbx_geom_part_info(gbx_geom g, int grid[4], int coord[4], uint64_t face[4], uint64_t *ninterior)
 {
-  GBX_CHECK(g && split_dim && nparts && part && face && ninterior, GBX_ENULL, "gbx_geom_part_info: NULL argument");
-  *split_dim = g->split_dim;
-  *nparts = g->nparts;
-  *part = g->part;
-  *face = g->face;
+  GBX_CHECK(g && grid && coord && face && ninterior, GBX_ENULL, "gbx_geom_part_info: NULL argument");
+  for (int d = 0; d < 4; d++) {
+    grid[d] = g->grid[d];
+    coord[d] = g->coord[d];
+    face[d] = g->face[d];
+  }
   *ninterior = g->nghost ? g->ninterior : g->vol;
   return GBX_OK;
 }

@@ -1,24 +1,25 @@
 // Halo exchange of a domain-decomposed lattice: the face pack and the ghost
 // slots of a domain part (gbx_geom_create_part).
 //
-// A part holds its vol local slots followed by 2*face ghost slots.  Its
-// neighbour table sends every leg that crosses the cut to a ghost slot, so
-// the Dhop kernels run on a part unchanged once the ghosts hold the
-// neighbouring parts' faces:
+// A part holds its vol local slots followed by ghost slots, two faces' worth
+// for every cut dimension.  Its neighbour table sends every leg that crosses
+// a cut to a ghost slot, so the Dhop kernels run on a part unchanged once the
+// ghosts hold the neighbouring parts' faces.  Along a cut dimension d:
 //
-//   part p's top face    (coordinate ld-1) -> part p+1's minus ghosts (side 0)
-//   part p's bottom face (coordinate 0)    -> part p-1's plus ghosts  (side 1)
+//   part c's top face    (coordinate ld-1) -> part c+1's minus ghosts (side 0)
+//   part c's bottom face (coordinate 0)    -> part c-1's plus ghosts  (side 1)
 //
-// with p+1 and p-1 taken modulo nparts (the lattice is periodic).  Ghost
-// slot f of a side is face site f, f = lexicographic index over the three
-// dimensions other than the split one, so a packed face is exactly the
-// receiving side's ghost range and a receive can land there directly.
+// with c+1 and c-1 the grid neighbours along d, periodic.  Ghost slot f of a
+// side is face site f, f = lexicographic index over the three dimensions
+// other than d, so a packed face is exactly the receiving side's ghost range
+// and a receive can land there directly.
 //
 // A face element is one site's whole spinor over all (s, RHS) minor indices:
 // nch chunks of M 16-byte units, the same bytes as the site's slot.
 
 #include <algorithm>
 #include <functional>
+#include <vector>
 
 #include "dhop_dev.h"
 #include "gbx_internal.h"
@@ -104,10 +105,11 @@ bool disjoint(const void *buf, uint64_t bytes, const gbx_spinor_s *x)
   return b + bytes <= d || b >= d + x->bytes;
 }
 
-int part_ok(gbx_spinor_s *x, const char *who)
+int part_ok(gbx_spinor_s *x, int dim, const char *who)
 {
   GBX_CHECK(x, GBX_ENULL, "%s: NULL field", who);
   GBX_CHECK(x->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
+  GBX_CHECK(dim >= 0 && dim < 4 && x->geom->grid[dim] > 1, GBX_EINVAL, "%s: dimension %d is not cut", who, dim);
   return GBX_OK;
 }
 
@@ -115,41 +117,40 @@ uint64_t slot_units(const gbx_spinor_s *x) { return (uint64_t)x->M * 24 * prec_b
 
 }  // namespace
 
-extern "C" int gbx_halo_bytes(gbx_spinor x, int proj, uint64_t *bytes)
+extern "C" int gbx_halo_bytes(gbx_spinor x, int dim, int proj, uint64_t *bytes)
 {
   GBX_CHECK(bytes, GBX_ENULL, "gbx_halo_bytes: NULL bytes");
-  GBX_TRY(part_ok(x, "gbx_halo_bytes"));
+  GBX_TRY(part_ok(x, dim, "gbx_halo_bytes"));
   GBX_CHECK(proj == 0 || proj == 1, GBX_EINVAL, "gbx_halo_bytes: proj=%d", proj);
-  *bytes = x->geom->face * slot_units(x) * 16 / (proj ? 2 : 1);
+  *bytes = x->geom->face[dim] * slot_units(x) * 16 / (proj ? 2 : 1);
   return GBX_OK;
 }
 
-extern "C" int gbx_halo_ghost(gbx_spinor x, int side, void **dptr)
+extern "C" int gbx_halo_ghost(gbx_spinor x, int dim, int side, void **dptr)
 {
   GBX_CHECK(dptr, GBX_ENULL, "gbx_halo_ghost: NULL dptr");
-  GBX_TRY(part_ok(x, "gbx_halo_ghost"));
+  GBX_TRY(part_ok(x, dim, "gbx_halo_ghost"));
   GBX_CHECK(side == 0 || side == 1, GBX_EINVAL, "gbx_halo_ghost: side=%d (0 = minus, 1 = plus)", side);
   const gbx_geom_s *g = x->geom;
-  *dptr = (char *)x->d + (g->vol + (uint64_t)side * g->face) * slot_units(x) * 16;
+  *dptr = (char *)x->d + (g->ghost_base[dim] + (uint64_t)side * g->face[dim]) * slot_units(x) * 16;
   return GBX_OK;
 }
 
 // Projector of the leg that reads the face on the receiving part: the top
-// face feeds the next part's backward leg (direction split_dim + 4), the
-// bottom face the previous part's forward leg (direction split_dim); the
-// non-dagger operator uses the opposite projector type on each leg, the
-// dagger the same.
-static int face_projector(int split_dim, int face, int dag)
+// face feeds the next part's backward leg (direction dim + 4), the bottom
+// face the previous part's forward leg (direction dim); the non-dagger
+// operator uses the opposite projector type on each leg, the dagger the same.
+static int face_projector(int dim, int face, int dag)
 {
-  const int dir = face ? split_dim + 4 : split_dim;
+  const int dir = face ? dim + 4 : dim;
   return dag ? dir : (dir ^ 4);
 }
 
-// Face `face` of x into dbuf (whole or projected), enqueued on st.
-static int pack_impl(gbx_spinor_s *x, int face, int dag, bool proj, void *dbuf, hipStream_t st, const char *who)
+// Face `face` of dimension dim of x into dbuf (whole or projected), enqueued on st.
+static int pack_impl(gbx_spinor_s *x, int dim, int face, int dag, bool proj, void *dbuf, hipStream_t st, const char *who)
 {
   GBX_CHECK(dbuf, GBX_ENULL, "%s: NULL buffer", who);
-  GBX_TRY(part_ok(x, who));
+  GBX_TRY(part_ok(x, dim, who));
   GBX_CHECK(face == 0 || face == 1, GBX_EINVAL, "%s: face=%d (0 = bottom, 1 = top)", who, face);
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
   GBX_CHECK(!proj || x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
@@ -157,74 +158,74 @@ static int pack_impl(gbx_spinor_s *x, int face, int dag, bool proj, void *dbuf, 
             "projected FP32 face is no smaller than the full 16-bit one, use gbx_halo_pack)", who);
   GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
   const gbx_geom_s *g = x->geom;
-  const uint64_t units = slot_units(x);
-  GBX_CHECK(disjoint(dbuf, g->face * units * (proj ? 8 : 16), x), GBX_EINVAL, "%s: buffer overlaps the field", who);
-  const uint32_t *fs = g->face_slots + (uint64_t)face * g->face;
+  const uint64_t units = slot_units(x), F = g->face[dim];
+  GBX_CHECK(disjoint(dbuf, F * units * (proj ? 8 : 16), x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+  const uint32_t *fs = g->face_slots + g->fs_base[dim] + (uint64_t)face * F;
   if (!proj) {
-    hipLaunchKernelGGL(k_halo_pack, dim3(grid_for(g->face * units)), dim3(256), 0, st, (const uint4 *)x->d, fs,
-                       (uint4 *)dbuf, g->face, units);
+    hipLaunchKernelGGL(k_halo_pack, dim3(grid_for(F * units)), dim3(256), 0, st, (const uint4 *)x->d, fs, (uint4 *)dbuf,
+                       F, units);
   } else {
-    const int K = face_projector(g->split_dim, face, dag);
-    const dim3 grid(grid_for(g->face * x->M));
-    if (x->prec == GBX_F64) launch_pack_proj<GBX_F64>(K, grid, st, x->d, fs, dbuf, g->face, x->M);
-    else launch_pack_proj<GBX_F32>(K, grid, st, x->d, fs, dbuf, g->face, x->M);
+    const int K = face_projector(dim, face, dag);
+    const dim3 grid(grid_for(F * x->M));
+    if (x->prec == GBX_F64) launch_pack_proj<GBX_F64>(K, grid, st, x->d, fs, dbuf, F, x->M);
+    else launch_pack_proj<GBX_F32>(K, grid, st, x->d, fs, dbuf, F, x->M);
   }
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }
 
-// The ghosts of `side` from dbuf (whole or projected face), enqueued on st.
-static int unpack_impl(gbx_spinor_s *x, int side, bool proj, const void *dbuf, hipStream_t st, const char *who)
+// The ghosts of `side` of dimension dim from dbuf (whole or projected face), enqueued on st.
+static int unpack_impl(gbx_spinor_s *x, int dim, int side, bool proj, const void *dbuf, hipStream_t st, const char *who)
 {
   GBX_CHECK(dbuf, GBX_ENULL, "%s: NULL buffer", who);
   void *ghost;
-  GBX_TRY(gbx_halo_ghost(x, side, &ghost));
+  GBX_TRY(gbx_halo_ghost(x, dim, side, &ghost));
   const gbx_geom_s *g = x->geom;
-  const uint64_t units = slot_units(x);
+  const uint64_t units = slot_units(x), F = g->face[dim];
   if (!proj) {
     if (dbuf == ghost) return GBX_OK;  // the face was packed or received in place
-    GBX_CHECK(disjoint(dbuf, g->face * units * 16, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
-    GBX_HIP(hipMemcpyAsync(ghost, dbuf, g->face * units * 16, hipMemcpyDeviceToDevice, st));
+    GBX_CHECK(disjoint(dbuf, F * units * 16, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+    GBX_HIP(hipMemcpyAsync(ghost, dbuf, F * units * 16, hipMemcpyDeviceToDevice, st));
     return GBX_OK;
   }
   GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "%s: FP64 / FP32 fields", who);
   GBX_CHECK(((uintptr_t)dbuf & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
-  GBX_CHECK(disjoint(dbuf, g->face * units * 8, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+  GBX_CHECK(disjoint(dbuf, F * units * 8, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
   const uint32_t nch = (uint32_t)spinor_nchunk(x->prec);
-  hipLaunchKernelGGL(k_halo_unpack_proj, dim3(grid_for(g->face * nch * x->M)), dim3(256), 0, st, (const uint4 *)dbuf,
-                     (uint4 *)ghost, g->face, nch, x->M);
+  hipLaunchKernelGGL(k_halo_unpack_proj, dim3(grid_for(F * nch * x->M)), dim3(256), 0, st, (const uint4 *)dbuf,
+                     (uint4 *)ghost, F, nch, x->M);
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }
 
-extern "C" int gbx_halo_pack(gbx_spinor x, int face, void *dbuf)
+extern "C" int gbx_halo_pack(gbx_spinor x, int dim, int face, void *dbuf)
 {
   GBX_CHECK(x, GBX_ENULL, "gbx_halo_pack: NULL field");
-  return pack_impl(x, face, 0, false, dbuf, x->geom->stream, "gbx_halo_pack");
+  return pack_impl(x, dim, face, 0, false, dbuf, x->geom->stream, "gbx_halo_pack");
 }
 
-extern "C" int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf)
+extern "C" int gbx_halo_pack_proj(gbx_spinor x, int dim, int face, int dag, void *dbuf)
 {
   GBX_CHECK(x, GBX_ENULL, "gbx_halo_pack_proj: NULL field");
-  return pack_impl(x, face, dag, true, dbuf, x->geom->stream, "gbx_halo_pack_proj");
+  return pack_impl(x, dim, face, dag, true, dbuf, x->geom->stream, "gbx_halo_pack_proj");
 }
 
-extern "C" int gbx_halo_unpack(gbx_spinor x, int side, const void *dbuf)
+extern "C" int gbx_halo_unpack(gbx_spinor x, int dim, int side, const void *dbuf)
 {
   GBX_CHECK(x, GBX_ENULL, "gbx_halo_unpack: NULL field");
-  return unpack_impl(x, side, false, dbuf, x->geom->stream, "gbx_halo_unpack");
+  return unpack_impl(x, dim, side, false, dbuf, x->geom->stream, "gbx_halo_unpack");
 }
 
-extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf)
+extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int dim, int side, const void *dbuf)
 {
   GBX_CHECK(x, GBX_ENULL, "gbx_halo_unpack_proj: NULL field");
-  return unpack_impl(x, side, true, dbuf, x->geom->stream, "gbx_halo_unpack_proj");
+  return unpack_impl(x, dim, side, true, dbuf, x->geom->stream, "gbx_halo_unpack_proj");
 }
 
 // One rank's application on one GPU, timed two ways (HIP events on the
 // geometry's stream, nrep applications after one warm-up each):
-//  serial:  pack both faces, move them, unpack, Dhop on every slot, all on
-//           one stream;
+//  serial:  pack the faces of every cut dimension, move them, unpack, Dhop on
+//           every slot, all on one stream;
 //  overlap: pack, then move / unpack on a second stream while the interior
 //           slots run, then the face slots once the ghosts have landed.
 // The move stands in for the transfer between GPUs: a device-to-device copy
@@ -233,7 +234,7 @@ extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf)
 // afterwards; the interior sites hold Dhop(in).
 namespace {
 struct OverlapRes {
-  void *buf[4] = {nullptr, nullptr, nullptr, nullptr};
+  std::vector<void *> buf;  // per cut dimension: send bottom, send top, receive bottom, receive top
   hipStream_t cs = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~OverlapRes()
@@ -250,22 +251,30 @@ struct OverlapRes {
 extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int proj, int nrep,
                                      double *us_overlap, double *us_serial)
 {
-  GBX_CHECK(us_overlap && us_serial, GBX_ENULL, "gbx_halo_overlap_time: NULL result");
-  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_halo_overlap_time: nrep=%d", nrep);
-  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_halo_overlap_time: dag=%d", dag);
+  static const char *who = "gbx_halo_overlap_time";
+  GBX_CHECK(us_overlap && us_serial, GBX_ENULL, "%s: NULL result", who);
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "%s: nrep=%d", who, nrep);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
   GBX_TRY(dhop_check_operands(u, out, in));
-  GBX_TRY(part_ok(in, "gbx_halo_overlap_time"));
+  GBX_CHECK(in->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
   GBX_CHECK(!proj || in->prec == GBX_F64 || in->prec == GBX_F32, GBX_EUNSUPPORTED,
-            "gbx_halo_overlap_time: projected faces of FP64 / FP32 fields only");
+            "%s: projected faces of FP64 / FP32 fields only", who);
   const gbx_geom_s *g = in->geom;
-  const uint64_t fb = g->face * slot_units(in) * (proj ? 8 : 16);
+  std::vector<int> cut;
+  for (int d = 0; d < 4; d++)
+    if (g->grid[d] > 1) cut.push_back(d);
   OverlapRes r;
-  for (void *&b : r.buf)
-    if (hipMalloc(&b, fb) != hipSuccess) {
-      (void)hipGetLastError();
-      b = nullptr;
-      return gbx_fail(GBX_ENOMEM, "gbx_halo_overlap_time: face buffers");
-    }
+  r.buf.assign(4 * cut.size(), nullptr);
+  std::vector<uint64_t> fb(cut.size());
+  for (size_t c = 0; c < cut.size(); c++) {
+    fb[c] = g->face[cut[c]] * slot_units(in) * (proj ? 8 : 16);
+    for (int k = 0; k < 4; k++)
+      if (hipMalloc(&r.buf[4 * c + k], fb[c]) != hipSuccess) {
+        (void)hipGetLastError();
+        r.buf[4 * c + k] = nullptr;
+        return gbx_fail(GBX_ENOMEM, "%s: face buffers", who);
+      }
+  }
   GBX_HIP(hipStreamCreate(&r.cs));
   for (hipEvent_t &e : r.ev) GBX_HIP(hipEventCreate(&e));
   hipStream_t ms = g->stream;
@@ -277,14 +286,19 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   // interior kernel would otherwise hold), then moved and into the ghosts:
   // the top face feeds the minus side
   auto pack = [&](hipStream_t st) -> int {
-    GBX_TRY(pack_impl(in, 0, dag, proj != 0, r.buf[0], st, "gbx_halo_overlap_time"));
-    return pack_impl(in, 1, dag, proj != 0, r.buf[1], st, "gbx_halo_overlap_time");
+    for (size_t c = 0; c < cut.size(); c++)
+      for (int f = 0; f < 2; f++) GBX_TRY(pack_impl(in, cut[c], f, dag, proj != 0, r.buf[4 * c + f], st, who));
+    return GBX_OK;
   };
   auto move_unpack = [&](hipStream_t st) -> int {
-    GBX_HIP(hipMemcpyAsync(r.buf[2], r.buf[0], fb, hipMemcpyDeviceToDevice, st));
-    GBX_HIP(hipMemcpyAsync(r.buf[3], r.buf[1], fb, hipMemcpyDeviceToDevice, st));
-    GBX_TRY(unpack_impl(in, 0, proj != 0, r.buf[3], st, "gbx_halo_overlap_time"));
-    return unpack_impl(in, 1, proj != 0, r.buf[2], st, "gbx_halo_overlap_time");
+    for (size_t c = 0; c < cut.size(); c++) {
+      void **b = &r.buf[4 * c];
+      GBX_HIP(hipMemcpyAsync(b[2], b[0], fb[c], hipMemcpyDeviceToDevice, st));
+      GBX_HIP(hipMemcpyAsync(b[3], b[1], fb[c], hipMemcpyDeviceToDevice, st));
+      GBX_TRY(unpack_impl(in, cut[c], 0, proj != 0, b[3], st, who));
+      GBX_TRY(unpack_impl(in, cut[c], 1, proj != 0, b[2], st, who));
+    }
+    return GBX_OK;
   };
   auto serial = [&]() -> int {
     GBX_TRY(pack(ms));

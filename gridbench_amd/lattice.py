@@ -171,7 +171,7 @@ class Geometry:
     `levels` = [innermost block, ..., outermost block] for a multi-level
     (blocks-of-blocks) site order.  Geometry.rb_pair(dims) gives the even and
     odd halves of a red-black pair (parity -1 / 0 / 1; volume = sites held).
-    Geometry.part(dims, split_dim, nparts, part) gives one slab of a domain
+    Geometry.part_of(dims, grid, coord) gives one block of a domain
     decomposition (dims = its local dims, gauge_dims = the whole lattice's)."""
 
     def __init__(self, dims, block=None, levels=None, _handle=None):
@@ -194,32 +194,21 @@ class Geometry:
         par = C.c_int()
         check(lib.gbx_geom_parity(h, C.byref(par)), "gbx_geom_parity")
         self.parity = int(par.value)
-        sd, npart, pt, face, nint = C.c_int(), C.c_int(), C.c_int(), C.c_uint64(), C.c_uint64()
-        check(lib.gbx_geom_part_info(h, C.byref(sd), C.byref(npart), C.byref(pt), C.byref(face), C.byref(nint)),
-              "gbx_geom_part_info")
-        self.split_dim, self.nparts, self.part = int(sd.value), int(npart.value), int(pt.value)
-        self.face, self.ninterior = int(face.value), int(nint.value)
+        grid, coord, face, nint = (C.c_int * 4)(), (C.c_int * 4)(), (C.c_uint64 * 4)(), C.c_uint64()
+        check(lib.gbx_geom_part_info(h, grid, coord, face, C.byref(nint)), "gbx_geom_part_info")
+        self.grid, self.coord, self.face = tuple(grid), tuple(coord), tuple(int(f) for f in face)
+        self.ninterior = int(nint.value)
 
     @classmethod
-    def part_of(cls, dims, split_dim, nparts, part, block=None):
-        """Slab `part` of `nparts` along `split_dim` of the lattice `dims`."""
-        dims = tuple(int(d) for d in dims)
+    def part_of(cls, dims, grid, coord, block=None):
+        """Block `coord` of the lattice `dims` cut into grid[0] x .. x grid[3] blocks."""
+        dims, grid, coord = (tuple(int(v) for v in a) for a in (dims, grid, coord))
         h = C.c_void_p()
-        check(lib.gbx_geom_create_part(ivec(dims), int(split_dim), int(nparts), int(part),
-                                       ivec(block) if block is not None else None, C.byref(h)), "gbx_geom_create_part")
-        local = tuple(d // nparts if i == split_dim else d for i, d in enumerate(dims))
-        g = cls(local, _handle=h)
+        check(lib.gbx_geom_create_part(ivec(dims), ivec(grid), ivec(coord), ivec(block) if block is not None else None,
+                                       C.byref(h)), "gbx_geom_create_part")
+        g = cls(tuple(d // n for d, n in zip(dims, grid)), _handle=h)
         g.gauge_dims = dims
         return g
-
-    @classmethod
-    def rb_pair(cls, dims, block=None):
-        """(even, odd) half-lattice geometries of one red-black pair."""
-        dims = tuple(int(d) for d in dims)
-        he, ho = C.c_void_p(), C.c_void_p()
-        check(lib.gbx_geom_create_rb(ivec(dims), ivec(block) if block is not None else None, C.byref(he),
-                                     C.byref(ho)), "gbx_geom_create_rb")
-        return cls(dims, _handle=he), cls(dims, _handle=ho)
 
     def sync(self):
         check(lib.gbx_sync(self.h), "gbx_sync")
@@ -363,26 +352,27 @@ class SpinorField:
         return b.value
 
     # ---- halo of a domain part: device pointers are plain integers
-    def halo_bytes(self, proj=False):
+    def halo_bytes(self, dim, proj=False):
         b = C.c_uint64()
-        check(lib.gbx_halo_bytes(self.h, int(bool(proj)), C.byref(b)), "gbx_halo_bytes")
+        check(lib.gbx_halo_bytes(self.h, int(dim), int(bool(proj)), C.byref(b)), "gbx_halo_bytes")
         return b.value
 
-    def halo_ghost(self, side):
+    def halo_ghost(self, dim, side):
         p = C.c_void_p()
-        check(lib.gbx_halo_ghost(self.h, int(side), C.byref(p)), "gbx_halo_ghost")
+        check(lib.gbx_halo_ghost(self.h, int(dim), int(side), C.byref(p)), "gbx_halo_ghost")
         return p.value
 
-    def halo_pack(self, face, dptr, proj=False, dag=False):
+    def halo_pack(self, dim, face, dptr, proj=False, dag=False):
         if proj:
-            check(lib.gbx_halo_pack_proj(self.h, int(face), int(bool(dag)), C.c_void_p(dptr)), "gbx_halo_pack_proj")
+            check(lib.gbx_halo_pack_proj(self.h, int(dim), int(face), int(bool(dag)), C.c_void_p(dptr)),
+                  "gbx_halo_pack_proj")
         else:
-            check(lib.gbx_halo_pack(self.h, int(face), C.c_void_p(dptr)), "gbx_halo_pack")
+            check(lib.gbx_halo_pack(self.h, int(dim), int(face), C.c_void_p(dptr)), "gbx_halo_pack")
         return self
 
-    def halo_unpack(self, side, dptr, proj=False):
+    def halo_unpack(self, dim, side, dptr, proj=False):
         fn = lib.gbx_halo_unpack_proj if proj else lib.gbx_halo_unpack
-        check(fn(self.h, int(side), C.c_void_p(dptr)), "gbx_halo_unpack_proj" if proj else "gbx_halo_unpack")
+        check(fn(self.h, int(dim), int(side), C.c_void_p(dptr)), "gbx_halo_unpack_proj" if proj else "gbx_halo_unpack")
         return self
 
     def close(self):
@@ -462,31 +452,51 @@ def halo_overlap_time(U, out, inp, dag=False, proj=True, nrep=20):
     return ov.value, se.value
 
 
+def grid_coords(grid):
+    """Coordinates of a process grid, lexicographic (x fastest)."""
+    return [tuple(int(v) for v in c[::-1]) for c in np.ndindex(*tuple(grid)[::-1])]
+
+
+def halo_buffers(fields, proj=True):
+    """Device buffers for halo_exchange: {(coord, dim): (bottom, top)}."""
+    some = next(iter(fields.values()))
+    grid = some.geom.grid
+    return {(c, d): (DeviceBuffer(x.halo_bytes(d, proj)), DeviceBuffer(x.halo_bytes(d, proj)))
+            for c, x in fields.items() for d in range(4) if grid[d] > 1}
+
+
 def halo_exchange(fields, proj=False, dag=False, bufs=None):
-    """Exchange the faces of the parts `fields` (part p = fields[p], all
-    split the same way), as the ranks of a domain decomposition would:
-    part p's top face into part p+1's minus ghosts, its bottom face into
-    part p-1's plus ghosts.  Whole spinors are packed straight into the
-    receiving ghost slots; spin-projected faces (proj=True) go through the
-    device buffers `bufs` (bufs[p] = (bottom, top) DeviceBuffers of
-    halo_bytes(proj=True) bytes each).  Synchronises every part's stream
-    before returning."""
-    n = len(fields)
-    for p, x in enumerate(fields):
-        up, down = fields[(p + 1) % n], fields[(p - 1) % n]
-        if proj:
-            x.halo_pack(1, bufs[p][1].ptr, proj=True, dag=dag)
-            x.halo_pack(0, bufs[p][0].ptr, proj=True, dag=dag)
-        else:
-            x.halo_pack(1, up.halo_ghost(0))
-            x.halo_pack(0, down.halo_ghost(1))
-    for x in fields:
+    """Exchange the faces of the parts `fields` ({grid coord: SpinorField},
+    every block of one grid), as the ranks of a domain decomposition would:
+    along every cut dimension, part c's top face into the next part's minus
+    ghosts and its bottom face into the previous part's plus ghosts.  Whole
+    spinors are packed straight into the receiving ghost slots; spin-projected
+    faces (proj=True) go through the device buffers `bufs` (halo_buffers).
+    Synchronises every part's stream before returning."""
+    grid = next(iter(fields.values())).geom.grid
+
+    def step(c, d, k):
+        c = list(c)
+        c[d] = (c[d] + k) % grid[d]
+        return tuple(c)
+
+    cut = [d for d in range(4) if grid[d] > 1]
+    for c, x in fields.items():
+        for d in cut:
+            if proj:
+                x.halo_pack(d, 1, bufs[(c, d)][1].ptr, proj=True, dag=dag)
+                x.halo_pack(d, 0, bufs[(c, d)][0].ptr, proj=True, dag=dag)
+            else:
+                x.halo_pack(d, 1, fields[step(c, d, 1)].halo_ghost(d, 0))
+                x.halo_pack(d, 0, fields[step(c, d, -1)].halo_ghost(d, 1))
+    for x in fields.values():
         x.geom.sync()
     if proj:
-        for p, x in enumerate(fields):
-            x.halo_unpack(0, bufs[(p - 1) % n][1].ptr, proj=True)
-            x.halo_unpack(1, bufs[(p + 1) % n][0].ptr, proj=True)
-        for x in fields:
+        for c, x in fields.items():
+            for d in cut:
+                x.halo_unpack(d, 0, bufs[(step(c, d, -1), d)][1].ptr, proj=True)
+                x.halo_unpack(d, 1, bufs[(step(c, d, 1), d)][0].ptr, proj=True)
+        for x in fields.values():
             x.geom.sync()
     return fields
 
@@ -622,48 +632,61 @@ def static_data_read(paths):
     return out
 
 
-def part_tables(dims, split_dim, nparts, part, block=None):
+def part_tables(dims, grid, coord, block=None):
     """Host tables of a domain part (no GPU needed): dict with site_of_slot
-    and gsite [V_local], nbr [V_local][8] (ghost slots V_local .. V_local +
-    2*face - 1), face_slots [2][face] and ninterior."""
-    dims = tuple(int(d) for d in dims)
-    ok = 0 <= split_dim < 4 and nparts >= 1 and min(dims) >= 1 and dims[split_dim] % nparts == 0
+    and gsite [V_local], nbr [V_local][8], face_slots {dim: [2][face_dim]},
+    ghost_base {dim: first ghost slot of dim}, ninterior, vol, face."""
+    dims, grid, coord = (tuple(int(v) for v in a) for a in (dims, grid, coord))
+    ok = all(d >= 1 and n >= 1 and d % n == 0 for d, n in zip(dims, grid))
     # invalid arguments: the library rejects them before writing any table
-    vol = int(np.prod(dims)) // nparts if ok else 1
-    face = vol // (dims[split_dim] // nparts) if ok else 1
+    ld = [d // n if ok else 1 for d, n in zip(dims, grid)]
+    vol = int(np.prod(ld))
+    face = {d: vol // ld[d] for d in range(4) if ok and grid[d] > 1}
+    nfs = max(1, sum(2 * f for f in face.values()))
     sos = np.zeros(vol, np.uint32)
     gs = np.zeros(vol, np.uint32)
     nbr = np.zeros((vol, 8), np.uint32)
-    fs = np.zeros((2, face), np.uint32)
+    fs = np.zeros(nfs, np.uint32)
     nint = C.c_uint64()
-    check(lib.gbx_part_tables(ivec(dims), int(split_dim), int(nparts), int(part), ivec(block) if block is not None else None,
+    check(lib.gbx_part_tables(ivec(dims), ivec(grid), ivec(coord), ivec(block) if block is not None else None,
                               ptr(sos, C.c_uint32), ptr(gs, C.c_uint32), ptr(nbr, C.c_uint32), ptr(fs, C.c_uint32),
                               C.byref(nint)), "gbx_part_tables")
-    return {"site_of_slot": sos, "gsite": gs, "nbr": nbr, "face_slots": fs, "ninterior": int(nint.value),
-            "vol": vol, "face": face}
+    slots, base, ghost = {}, 0, vol
+    ghost_base = {}
+    for d in sorted(face):
+        slots[d] = fs[base:base + 2 * face[d]].reshape(2, face[d])
+        ghost_base[d] = ghost
+        base += 2 * face[d]
+        ghost += 2 * face[d]
+    return {"site_of_slot": sos, "gsite": gs, "nbr": nbr, "face_slots": slots, "ghost_base": ghost_base,
+            "ninterior": int(nint.value), "vol": vol, "face": face}
 
 
-def part_split(dims, split_dim, nparts, lex):
-    """Lexicographic spinor array [nrhs][V][Ls][...] of the lattice `dims`
-    -> one array [nrhs][V/nparts][Ls][...] per slab along split_dim, each in
-    its local lexicographic order (the layout SpinorField.upload takes on a
+def part_split(dims, grid, lex):
+    """Lexicographic spinor array [nrhs][V][Ls][...] of the lattice `dims` ->
+    {grid coord: [nrhs][V_local][Ls][...]}, each block in its local
+    lexicographic order (the layout SpinorField.upload takes on a
     Geometry.part_of geometry)."""
     a = np.asarray(lex)
-    nrhs = a.shape[0]
-    rest = a.shape[2:]
+    nrhs, rest = a.shape[0], a.shape[2:]
     a = a.reshape((nrhs, dims[3], dims[2], dims[1], dims[0]) + rest)
-    ax = 1 + (3 - split_dim)
-    return [np.ascontiguousarray(c).reshape((nrhs, -1) + rest) for c in np.split(a, nparts, axis=ax)]
+    ld = [d // n for d, n in zip(dims, grid)]
+    out = {}
+    for c in grid_coords(grid):
+        sl = tuple(slice(c[d] * ld[d], (c[d] + 1) * ld[d]) for d in (3, 2, 1, 0))
+        out[c] = np.ascontiguousarray(a[(slice(None),) + sl]).reshape((nrhs, -1) + rest)
+    return out
 
 
-def part_merge(dims, split_dim, parts):
+def part_merge(dims, grid, parts):
     """Inverse of part_split."""
-    nparts = len(parts)
-    loc = [d // nparts if i == split_dim else d for i, d in enumerate(dims)]
-    nrhs = parts[0].shape[0]
-    rest = parts[0].shape[2:]
-    blocks = [np.asarray(p).reshape((nrhs, loc[3], loc[2], loc[1], loc[0]) + rest) for p in parts]
-    out = np.concatenate(blocks, axis=1 + (3 - split_dim))
+    some = next(iter(parts.values()))
+    nrhs, rest = some.shape[0], some.shape[2:]
+    ld = [d // n for d, n in zip(dims, grid)]
+    out = np.empty((nrhs, dims[3], dims[2], dims[1], dims[0]) + rest, dtype=some.dtype)
+    for c, p in parts.items():
+        sl = tuple(slice(c[d] * ld[d], (c[d] + 1) * ld[d]) for d in (3, 2, 1, 0))
+        out[(slice(None),) + sl] = np.asarray(p).reshape((nrhs, ld[3], ld[2], ld[1], ld[0]) + rest)
     return out.reshape((nrhs, -1) + rest)
 
 
@@ -764,5 +787,5 @@ __all__ = [
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
-    "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time",
+    "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time", "grid_coords", "halo_buffers",
 ]

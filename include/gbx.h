@@ -177,32 +177,35 @@ int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gb
 int gbx_geom_create_rb(const int dims[4], const int block[4], gbx_geom *even, gbx_geom *odd);
 /* -1 for a whole lattice, 0 / 1 for the even / odd half of a pair. */
 int gbx_geom_parity(gbx_geom g, int *parity);
-/* Domain part: slab `part` (0 .. nparts-1) of the periodic lattice `dims`  */
-/* cut into nparts equal slabs along split_dim, as one GPU of a domain-     */
-/* decomposed run holds it.  No counterpart in the reference, which runs    */
-/* on one rank (TableGenerate.cc:84 asserts every stencil neighbour is      */
-/* local); this is the single-GPU share of a multi-GPU decomposition.       */
-/*  - local dims = dims with dims[split_dim] / nparts; spinor host arrays    */
-/*    are [nrhs][V_local][Ls][24] in local lexicographic order;             */
+/* Domain part: the block at grid coordinate coord (0 <= coord[d] < grid[d]) */
+/* of the periodic lattice `dims` cut into grid[0] x grid[1] x grid[2] x    */
+/* grid[3] equal blocks (at least one grid[d] > 1), as one GPU of a domain-  */
+/* decomposed run holds it.  No counterpart in the reference, which runs on */
+/* one rank (TableGenerate.cc:84 asserts every stencil neighbour is local). */
+/*  - local dims ld[d] = dims[d] / grid[d]; spinor host arrays are          */
+/*    [nrhs][V_local][Ls][24] in local lexicographic order;                 */
 /*  - gbx_gauge_upload reads the GLOBAL doubled field [V][8][18] and keeps  */
 /*    the part's own sites' links;                                          */
-/*  - a spinor field holds 2*face ghost slots after its V_local sites       */
-/*    (face = V_local / local dims[split_dim]): legs that cross the cut     */
-/*    read them, so they must hold the neighbouring parts' faces (see the   */
-/*    halo functions below) before a Dhop;                                  */
+/*  - a spinor field holds, after its V_local sites, two faces of ghost     */
+/*    slots for every cut dimension d (face_d = V_local / ld[d]): legs that */
+/*    cross a cut read them, so they must hold the neighbouring parts'      */
+/*    faces (the halo functions below) before a Dhop;                       */
 /*  - slots 0 .. ninterior-1 are the interior sites, which read no ghost;   */
-/*    slots ninterior .. V_local-1 the sites of the two faces.              */
+/*    slots ninterior .. V_local-1 the sites on a cut face.                 */
 /* Parts run the VALU and MFMA kernels (not the tiled kernel) and gbx_dhop,   */
 /* gbx_dhop_range, gbx_dhop_xpay; solvers take whole lattices.              */
-int gbx_geom_create_part(const int dims[4], int split_dim, int nparts, int part, const int block[4], gbx_geom *out);
-/* split_dim = -1, nparts = 1, part = 0, face = 0, ninterior = V for a      */
-/* geometry that is not a part.                                             */
-int gbx_geom_part_info(gbx_geom g, int *split_dim, int *nparts, int *part, uint64_t *face, uint64_t *ninterior);
+int gbx_geom_create_part(const int dims[4], const int grid[4], const int coord[4], const int block[4], gbx_geom *out);
+/* grid, coord, face (0 for uncut dimensions) and ninterior; grid = 1 1 1 1,  */
+/* ninterior = V for a geometry that is not a part.                          */
+int gbx_geom_part_info(gbx_geom g, int grid[4], int coord[4], uint64_t face[4], uint64_t *ninterior);
 /* The part's tables on the host, without a device (what the geometry      */
 /* uploads): site_of_slot (local lexicographic site of each slot) and gsite */
-/* (global site) [V_local], nbr [V_local][8] (ghost slots V_local .. V_local */
-/* + 2*face - 1), face_slots [2][face] (slots of the bottom, top faces).   */
-int gbx_part_tables(const int dims[4], int split_dim, int nparts, int part, const int block[4], uint32_t *site_of_slot,
+/* (global site) [V_local], nbr [V_local][8] (ghost slots from V_local on),  */
+/* face_slots: for each cut dimension d ascending, [2][face_d] (the slots of */
+/* its bottom and top faces, face site f lexicographic over the other three */
+/* dimensions).  The ghosts of cut dimension d follow those of the cut      */
+/* dimensions before it: minus side, then plus side, face_d slots each.     */
+int gbx_part_tables(const int dims[4], const int grid[4], const int coord[4], const int block[4], uint32_t *site_of_slot,
                     uint32_t *gsite, uint32_t *nbr, uint32_t *face_slots, uint64_t *ninterior);
 int gbx_geom_destroy(gbx_geom g);
 int gbx_geom_volume(gbx_geom g, uint64_t *vol);
@@ -316,36 +319,38 @@ int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kind
 
 /* ---------------------------------------------------------------------- */
 /* Halo exchange of a domain part (gbx_geom_create_part).  No counterpart  */
-/* in the reference (single rank).  Part p's top face (coordinate          */
-/* ld-1 along the cut) goes to part p+1's minus ghosts (side 0), its bottom */
-/* face (coordinate 0) to part p-1's plus ghosts (side 1), modulo nparts.   */
-/* A face is [face][nch][Ls*nrhs] 16-byte chunks, face site f ordered      */
-/* lexicographically over the other three dimensions: exactly the layout   */
-/* of a side's ghost slots, so a receive (RCCL recv, peer copy or a pack   */
-/* into peer memory) can land in gbx_halo_ghost's range directly.  All      */
-/* enqueued on the field's geometry stream, no synchronisation; device      */
-/* buffers, 16-byte aligned, not overlapping the field.                    */
+/* in the reference (single rank).  Along a cut dimension dim, the part's   */
+/* top face (coordinate ld-1) goes to the next part's minus ghosts (side 0),*/
+/* its bottom face (coordinate 0) to the previous part's plus ghosts (side  */
+/* 1), neighbours periodic in the grid.  A face is [face][nch][Ls*nrhs]     */
+/* 16-byte chunks, face site f ordered lexicographically over the other     */
+/* three dimensions: exactly the layout of a side's ghost slots, so a       */
+/* receive (RCCL recv, peer copy or a pack into peer memory) can land in    */
+/* gbx_halo_ghost's range directly.  All enqueued on the field's geometry  */
+/* stream, no synchronisation; device buffers, 16-byte aligned, not         */
+/* overlapping the field.                                                   */
 /* ---------------------------------------------------------------------- */
-/* Bytes of one face: proj = 0 whole spinors, 1 spin-projected halves.     */
-int gbx_halo_bytes(gbx_spinor x, int proj, uint64_t *bytes);
-/* Device address of the ghost slots of side 0 (minus) or 1 (plus).        */
-int gbx_halo_ghost(gbx_spinor x, int side, void **dptr);
-/* Gather face 0 (bottom) or 1 (top) into dbuf (gbx_halo_bytes(x, 0)).     */
-int gbx_halo_pack(gbx_spinor x, int face, void *dbuf);
-/* Copy a received whole-spinor face into the ghosts of `side`.            */
-int gbx_halo_unpack(gbx_spinor x, int side, const void *dbuf);
+/* Bytes of one face of dim: proj = 0 whole spinors, 1 spin-projected.     */
+int gbx_halo_bytes(gbx_spinor x, int dim, int proj, uint64_t *bytes);
+/* Device address of the ghost slots of dim, side 0 (minus) or 1 (plus).   */
+int gbx_halo_ghost(gbx_spinor x, int dim, int side, void **dptr);
+/* Gather face 0 (bottom) or 1 (top) of dim into dbuf.                     */
+int gbx_halo_pack(gbx_spinor x, int dim, int face, void *dbuf);
+/* Copy a received whole-spinor face into the ghosts of (dim, side).        */
+int gbx_halo_unpack(gbx_spinor x, int dim, int side, const void *dbuf);
 /* Spin-projected face (half the bytes; FP64 / FP32 fields): the two-spinor */
 /* the receiving part's leg forms from it for Dhop (dag = 0) or DhopDag    */
 /* (dag = 1), in the storage precision.  gbx_halo_unpack_proj expands it   */
 /* into ghosts whose projection is that two-spinor exactly, so the result  */
 /* is bit-identical to the one from whole-spinor ghosts.                   */
-int gbx_halo_pack_proj(gbx_spinor x, int face, int dag, void *dbuf);
-int gbx_halo_unpack_proj(gbx_spinor x, int side, const void *dbuf);
+int gbx_halo_pack_proj(gbx_spinor x, int dim, int face, int dag, void *dbuf);
+int gbx_halo_unpack_proj(gbx_spinor x, int dim, int side, const void *dbuf);
 /* One rank's application, timed on this GPU (microseconds per application, */
 /* HIP events on the geometry's stream, nrep after one warm-up each):       */
-/*  serial:  pack both faces, move each into a receive buffer (a device-to- */
-/*           device copy standing in for the transfer between GPUs), unpack, */
-/*           Dhop on every slot, all on the geometry's stream;              */
+/*  serial:  pack the faces of every cut dimension, move each into a       */
+/*           receive buffer (a device-to-device copy standing in for the    */
+/*           transfer between GPUs), unpack, Dhop on every slot, all on the */
+/*           geometry's stream;                                             */
 /*  overlap: pack, then move / unpack on a second stream while the interior */
 /*           slots run, then the face slots after an event.                 */
 /* The moved faces are the part's own, so afterwards out's face sites hold  */

@@ -127,24 +127,24 @@ def test_geometry_argument_errors(gbx):
                                       C.byref(h)) == 1
     assert lib.gbx_geom_create_levels((C.c_int * 4)(8, 8, 8, 8), 5, (C.c_int * 20)(), C.byref(h)) == 1
     # domain parts: the decomposition is checked before any device work
-    d4 = (C.c_int * 4)(4, 4, 4, 8)
-    assert lib.gbx_geom_create_part(d4, 4, 2, 0, None, C.byref(h)) == 1   # split_dim
-    assert lib.gbx_geom_create_part(d4, 3, 3, 0, None, C.byref(h)) == 1   # 3 does not divide 8
-    assert lib.gbx_geom_create_part(d4, 3, 2, 2, None, C.byref(h)) == 1   # part out of range
-    assert lib.gbx_geom_create_part(d4, 3, 2, 0, (C.c_int * 4)(3, 4, 4, 4), C.byref(h)) == 1  # block
-    assert lib.gbx_geom_create_part(None, 3, 2, 0, None, C.byref(h)) == 2
+    d4, i4 = (C.c_int * 4)(4, 4, 4, 8), C.c_int * 4
+    assert lib.gbx_geom_create_part(d4, i4(1, 1, 1, 3), i4(0, 0, 0, 0), None, C.byref(h)) == 1  # 3 does not divide 8
+    assert lib.gbx_geom_create_part(d4, i4(1, 1, 1, 1), i4(0, 0, 0, 0), None, C.byref(h)) == 1  # nothing cut
+    assert lib.gbx_geom_create_part(d4, i4(1, 1, 1, 2), i4(0, 0, 0, 2), None, C.byref(h)) == 1  # coord out of range
+    assert lib.gbx_geom_create_part(d4, i4(1, 1, 1, 2), i4(0, 0, 0, 0), i4(3, 4, 4, 4), C.byref(h)) == 1  # block
+    assert lib.gbx_geom_create_part(None, i4(1, 1, 1, 2), i4(0, 0, 0, 0), None, C.byref(h)) == 2
 
 
 def test_halo_argument_errors(gbx):
     from gridbench_amd._lib import lib
     b = C.c_uint64()
     p = C.c_void_p()
-    assert lib.gbx_halo_bytes(None, 0, C.byref(b)) == 2
-    assert lib.gbx_halo_ghost(None, 0, C.byref(p)) == 2
-    assert lib.gbx_halo_pack(None, 0, C.c_void_p(64)) == 2
-    assert lib.gbx_halo_pack_proj(None, 0, 0, None) == 2
-    assert lib.gbx_halo_unpack(None, 1, C.c_void_p(64)) == 2
-    assert lib.gbx_halo_unpack_proj(None, 1, None) == 2
+    assert lib.gbx_halo_bytes(None, 3, 0, C.byref(b)) == 2
+    assert lib.gbx_halo_ghost(None, 3, 0, C.byref(p)) == 2
+    assert lib.gbx_halo_pack(None, 3, 0, C.c_void_p(64)) == 2
+    assert lib.gbx_halo_pack_proj(None, 3, 0, 0, None) == 2
+    assert lib.gbx_halo_unpack(None, 3, 1, C.c_void_p(64)) == 2
+    assert lib.gbx_halo_unpack_proj(None, 3, 1, None) == 2
     assert lib.gbx_buffer_create(0, C.byref(p)) == 1
     assert lib.gbx_buffer_destroy(None) == 0
     assert lib.gbx_dhop_range(None, None, None, 0, 0, 1) == 2

@@ -16,8 +16,9 @@ from conftest import rel_l2
 
 pytestmark = pytest.mark.gpu
 
-SPLITS = [((4, 4, 4, 8), 3, 2), ((4, 4, 4, 8), 3, 4), ((4, 4, 4, 8), 3, 8), ((8, 4, 4, 4), 0, 2),
-          ((4, 6, 4, 4), 1, 3), ((4, 4, 4, 4), 2, 2)]
+GRIDS = [((4, 4, 4, 8), (1, 1, 1, 2)), ((4, 4, 4, 8), (1, 1, 1, 4)), ((4, 4, 4, 8), (1, 1, 1, 8)),
+         ((8, 4, 4, 4), (2, 1, 1, 1)), ((4, 6, 4, 4), (1, 3, 1, 1)), ((4, 4, 4, 4), (1, 1, 2, 1)),
+         ((4, 4, 6, 6), (1, 1, 2, 2)), ((4, 6, 6, 6), (2, 1, 3, 2)), ((4, 4, 4, 4), (2, 2, 2, 2))]
 
 
 def whole_dhop(gbx, p, prec, dag, cfg=None):
@@ -30,66 +31,68 @@ def whole_dhop(gbx, p, prec, dag, cfg=None):
 
 
 class Parts:
-    def __init__(self, gbx, p, split, nparts, prec):
-        self.gbx, self.p, self.split, self.n = gbx, p, split, nparts
-        self.geoms = [gbx.Geometry.part_of(p.dims, split, nparts, k) for k in range(nparts)]
-        self.U = [gbx.GaugeField(g, prec).upload(p.dbl) for g in self.geoms]
-        loc = gbx.part_split(p.dims, split, nparts, p.psi)
-        self.src = [gbx.SpinorField(g, p.Ls, p.nrhs, prec).upload(a) for g, a in zip(self.geoms, loc)]
-        self.dst = [gbx.SpinorField(g, p.Ls, p.nrhs, prec) for g in self.geoms]
+    def __init__(self, gbx, p, grid, prec):
+        self.gbx, self.p, self.grid = gbx, p, tuple(grid)
+        self.coords = gbx.grid_coords(grid)
+        self.geoms = {c: gbx.Geometry.part_of(p.dims, grid, c) for c in self.coords}
+        self.U = {c: gbx.GaugeField(g, prec).upload(p.dbl) for c, g in self.geoms.items()}
+        loc = gbx.part_split(p.dims, grid, p.psi)
+        self.src = {c: gbx.SpinorField(g, p.Ls, p.nrhs, prec).upload(loc[c]) for c, g in self.geoms.items()}
+        self.dst = {c: gbx.SpinorField(g, p.Ls, p.nrhs, prec) for c, g in self.geoms.items()}
 
     def exchange(self, proj=False, dag=False):
-        bufs = None
-        if proj:
-            nb = self.src[0].halo_bytes(proj=True)
-            bufs = [(self.gbx.DeviceBuffer(nb), self.gbx.DeviceBuffer(nb)) for _ in range(self.n)]
+        bufs = self.gbx.halo_buffers(self.src) if proj else None
         self.gbx.halo_exchange(self.src, proj=proj, dag=dag, bufs=bufs)
         return bufs
 
+    def each(self):
+        return [(self.U[c], self.dst[c], self.src[c]) for c in self.coords]
+
     def merged(self):
-        for g in self.geoms:
+        for g in self.geoms.values():
             g.sync()
-        return self.gbx.part_merge(self.p.dims, self.split, [d.download() for d in self.dst])
+        return self.gbx.part_merge(self.p.dims, self.grid, {c: d.download() for c, d in self.dst.items()})
 
 
-@pytest.mark.parametrize("dims,split,nparts", SPLITS)
+@pytest.mark.parametrize("dims,grid", GRIDS)
 @pytest.mark.parametrize("dag", [False, True])
 @pytest.mark.parametrize("prec,proj", [("f64", False), ("f64", True), ("f32", False), ("f32", True)])
-def test_parts_match_whole_lattice(gpu, gbx, problem, dims, split, nparts, dag, prec, proj):
+def test_parts_match_whole_lattice(gpu, gbx, problem, dims, grid, dag, prec, proj):
     p = problem(dims, 4)
-    parts = Parts(gbx, p, split, nparts, prec)
-    cfg = gbx.dhop_get_cfg(parts.U[0], parts.src[0])
+    parts = Parts(gbx, p, grid, prec)
+    c0 = parts.coords[0]
+    cfg = gbx.dhop_get_cfg(parts.U[c0], parts.src[c0])
     cfg.kind = 0
     want = whole_dhop(gbx, p, prec, dag, cfg=cfg)
     bufs = parts.exchange(proj=proj, dag=dag)
-    for U, d, s in zip(parts.U, parts.dst, parts.src):
+    for U, d, s in parts.each():
         gbx.dhop(U, d, s, dag=dag, cfg=cfg)
     got = parts.merged()
-    print(f"dims={dims} split={split} nparts={nparts} dag={dag} prec={prec} proj={proj} "
-          f"max|diff|={np.abs(got - want).max():.3e}")
+    print(f"dims={dims} grid={grid} dag={dag} prec={prec} proj={proj} max|diff|={np.abs(got - want).max():.3e}")
     assert np.array_equal(got, want)
     err = rel_l2(got[0], p.oracle_lex(dag=dag))
     assert err <= (1e-14 if prec == "f64" else 1e-6)
     del bufs
 
 
+@pytest.mark.parametrize("grid", [(1, 1, 1, 2), (1, 2, 1, 2)])
 @pytest.mark.parametrize("dag", [False, True])
-def test_interior_then_boundary(gpu, gbx, problem, dag):
+def test_interior_then_faces(gpu, gbx, problem, grid, dag):
     """gbx_dhop_range over [0, ninterior) before the exchange and over
     [ninterior, V) after it gives the full application."""
-    p = problem((4, 4, 4, 12), 4)
-    parts = Parts(gbx, p, 3, 2, "f64")
-    g0 = parts.geoms[0]
+    p = problem((4, 8, 4, 12), 4)
+    parts = Parts(gbx, p, grid, "f64")
+    g0 = parts.geoms[parts.coords[0]]
     assert 0 < g0.ninterior < g0.volume
-    for U, d, s in zip(parts.U, parts.dst, parts.src):
+    for U, d, s in parts.each():
         gbx.dhop_range(U, d, s, 0, d.geom.ninterior, dag=dag)  # ghosts not yet filled
     parts.exchange()
-    for U, d, s in zip(parts.U, parts.dst, parts.src):
+    for U, d, s in parts.each():
         gbx.dhop_range(U, d, s, d.geom.ninterior, d.geom.volume, dag=dag)
     got = parts.merged()
-    ref = Parts(gbx, p, 3, 2, "f64")
+    ref = Parts(gbx, p, grid, "f64")
     ref.exchange()
-    for U, d, s in zip(ref.U, ref.dst, ref.src):
+    for U, d, s in ref.each():
         gbx.dhop(U, d, s, dag=dag)
     assert np.array_equal(got, ref.merged())
     assert rel_l2(got[0], p.oracle_lex(dag=dag)) <= 1e-14
@@ -123,11 +126,11 @@ def test_range_on_whole_lattice(gpu, gbx, problem):
 def test_parts_mfma_multi_rhs(gpu, gbx, problem, prec):
     """The MFMA kernel (FP16 / BF16, 16 minor columns) on parts with whole-spinor halos."""
     p = problem((4, 4, 4, 8), 4, nrhs=4)
-    parts = Parts(gbx, p, 3, 2, prec)
+    parts = Parts(gbx, p, (1, 1, 2, 2), prec)
     cfg = gbx.kcfg(4, 256, kind=1)
     want = whole_dhop(gbx, p, prec, False, cfg=cfg)
     parts.exchange()
-    for U, d, s in zip(parts.U, parts.dst, parts.src):
+    for U, d, s in parts.each():
         gbx.dhop(U, d, s, cfg=cfg)
     got = parts.merged()
     assert np.array_equal(got, want)
@@ -136,64 +139,91 @@ def test_parts_mfma_multi_rhs(gpu, gbx, problem, prec):
 
 
 def test_projected_face_is_half_and_matches_projection(gpu, gbx, problem, orc):
-    """A spin-projected top face holds, for DhopDag's backward leg along T
-    (projector 1 - gamma_t: h0 = p0 - p2, h1 = p1 - p3), exactly those sums."""
+    """A spin-projected top face along T holds, for DhopDag's backward leg
+    (projector 1 - gamma_t: h0 = p0 - p2, h1 = p1 - p3), exactly those sums;
+    along Z for Dhop's forward leg from the bottom face (projector 1 - gamma_z:
+    h0 = p0 - i p2, h1 = p1 + i p3)."""
     p = problem((4, 4, 4, 8), 2)
-    parts = Parts(gbx, p, 3, 2, "f64")
-    x = parts.src[0]
-    assert x.halo_bytes(proj=True) * 2 == x.halo_bytes()
-    buf = gbx.DeviceBuffer(x.halo_bytes(proj=True))
-    x.halo_pack(1, buf.ptr, proj=True, dag=True)
-    x.geom.sync()
-    M, face = p.Ls * p.nrhs, x.geom.face
-    h = buf.download(np.float64).reshape(face, 6, M, 2)  # [f][complex (spin*3+colour)][m][re, im]
-    loc = gbx.part_split(p.dims, 3, 2, p.psi)[0][0]      # [V_local][Ls][4][3][2]
-    ld = (4, 4, 4, 4)
-    for f in range(face):
-        site = f + face * (ld[3] - 1)                      # local lexicographic index of face site f, t = 3
-        for s in range(p.Ls):
-            sp = loc[site, s]
-            want = np.concatenate([sp[0] - sp[2], sp[1] - sp[3]])  # [6][2]
-            assert np.array_equal(h[f, :, s, :], want)
+    parts = Parts(gbx, p, (1, 1, 2, 2), "f64")
+    c0 = parts.coords[0]
+    x = parts.src[c0]
+    loc = gbx.part_split(p.dims, (1, 1, 2, 2), p.psi)[c0][0]      # [V_local][Ls][4][3][2], local dims 4 4 2 4
+    ld = (4, 4, 2, 4)
+    M = p.Ls * p.nrhs
+
+    def cplx(a):
+        return a[..., 0] + 1j * a[..., 1]
+
+    for dim, face, dag in ((3, 1, True), (2, 0, False)):
+        assert x.halo_bytes(dim, proj=True) * 2 == x.halo_bytes(dim)
+        buf = gbx.DeviceBuffer(x.halo_bytes(dim, proj=True))
+        x.halo_pack(dim, face, buf.ptr, proj=True, dag=dag)
+        x.geom.sync()
+        F = x.geom.face[dim]
+        h = buf.download(np.float64).reshape(F, 6, M, 2)  # [f][complex (spin*3+colour)][m][re, im]
+        others = [e for e in range(4) if e != dim]
+        for f in range(F):
+            c = [0, 0, 0, 0]
+            r = f
+            for e in others:
+                c[e] = r % ld[e]
+                r //= ld[e]
+            c[dim] = ld[dim] - 1 if face == 1 else 0
+            site = c[0] + ld[0] * (c[1] + ld[1] * (c[2] + ld[2] * c[3]))
+            for s in range(p.Ls):
+                sp = cplx(loc[site, s])                      # [4][3] complex
+                if dim == 3:
+                    want = np.concatenate([sp[0] - sp[2], sp[1] - sp[3]])
+                else:
+                    want = np.concatenate([sp[0] - 1j * sp[2], sp[1] + 1j * sp[3]])
+                got = cplx(h[f, :, s, :])
+                assert np.array_equal(got, want), (dim, f, s)
 
 
 def test_halo_errors(gpu, gbx, problem):
     p = problem((4, 4, 4, 8), 2)
     whole = gbx.SpinorField(gbx.Geometry(p.dims), 2, 1, "f64")
     with pytest.raises(gbx.GbxError):
-        whole.halo_bytes()
-    parts = Parts(gbx, p, 3, 2, "f16")
-    buf = gbx.DeviceBuffer(parts.src[0].halo_bytes())
+        whole.halo_bytes(3)
+    parts = Parts(gbx, p, (1, 1, 1, 2), "f16")
+    c0 = parts.coords[0]
+    x = parts.src[c0]
+    buf = gbx.DeviceBuffer(x.halo_bytes(3))
     with pytest.raises(gbx.GbxError):
-        parts.src[0].halo_pack(0, buf.ptr, proj=True)
+        x.halo_pack(3, 0, buf.ptr, proj=True)   # projected 16-bit faces
     with pytest.raises(gbx.GbxError):
-        parts.src[0].halo_pack(2, buf.ptr)
+        x.halo_pack(3, 2, buf.ptr)              # face index
     with pytest.raises(gbx.GbxError):
-        parts.src[0].halo_pack(0, parts.src[0].halo_ghost(0))  # overlaps the field
+        x.halo_pack(2, 0, buf.ptr)              # dimension 2 is not cut
     with pytest.raises(gbx.GbxError):
-        parts.src[0].halo_unpack(1, parts.src[0].halo_ghost(0))  # overlaps the field
-    parts.src[0].halo_unpack(0, parts.src[0].halo_ghost(0))      # in place: nothing to copy
+        x.halo_pack(3, 0, x.halo_ghost(3, 0))   # overlaps the field
+    with pytest.raises(gbx.GbxError):
+        x.halo_unpack(3, 1, x.halo_ghost(3, 0))  # overlaps the field
+    x.halo_unpack(3, 0, x.halo_ghost(3, 0))      # in place: nothing to copy
     with pytest.raises(gbx.GbxError):  # the tiled kernel takes whole lattices only
-        gbx.dhop(parts.U[0], parts.dst[0], parts.src[0], cfg=gbx.kcfg(0, 256, kind=2, tile=gbx.pack_tile((2, 2, 2, 2))))
+        gbx.dhop(parts.U[c0], parts.dst[c0], x, cfg=gbx.kcfg(0, 256, kind=2, tile=gbx.pack_tile((2, 2, 2, 2))))
     with pytest.raises(gbx.GbxError):
-        gbx.Geometry.part_of(p.dims, 3, 3, 0)
+        gbx.Geometry.part_of(p.dims, (1, 1, 1, 3), (0, 0, 0, 0))
 
 
-@pytest.mark.parametrize("proj", [False, True])
-def test_overlap_timing_entry(gpu, gbx, problem, proj):
-    """The timed overlap schedule runs, and the interior it computes is the
-    true Dhop (the face sites are stand-in values by design)."""
-    p = problem((4, 4, 4, 12), 4)
-    parts = Parts(gbx, p, 3, 2, "f64")
+@pytest.mark.parametrize("grid,proj", [((1, 1, 1, 2), False), ((1, 1, 1, 2), True), ((1, 2, 1, 2), True)])
+def test_overlap_timing_entry(gpu, gbx, problem, grid, proj):
+    """The timed schedule runs, and the interior it computes is the true Dhop
+    (the face sites are stand-in values by design)."""
+    p = problem((4, 8, 4, 12), 4)
+    parts = Parts(gbx, p, grid, "f64")
     parts.exchange()
-    for U, d, s in zip(parts.U, parts.dst, parts.src):
+    for U, d, s in parts.each():
         gbx.dhop(U, d, s)
-    want = parts.dst[0].download()
-    g0 = parts.geoms[0]
-    ov, se = gbx.halo_overlap_time(parts.U[0], parts.dst[0], parts.src[0], proj=proj, nrep=3)
+    c0 = parts.coords[0]
+    want = parts.dst[c0].download()
+    ov, se = gbx.halo_overlap_time(parts.U[c0], parts.dst[c0], parts.src[c0], proj=proj, nrep=3)
     assert ov > 0 and se > 0
-    got = parts.dst[0].download()
-    # interior sites (local t = 1 .. 4 of 6) are unchanged
-    got_i = got[0].reshape(6, 4 * 4 * 4, -1)[1:5]
-    want_i = want[0].reshape(6, 4 * 4 * 4, -1)[1:5]
-    assert np.array_equal(got_i, want_i)
+    got = parts.dst[c0].download()
+    g0 = parts.geoms[c0]
+    ld = g0.dims
+    cut = [d for d in range(4) if grid[d] > 1]
+    for site in range(g0.volume):
+        c = [site % ld[0], site // ld[0] % ld[1], site // (ld[0] * ld[1]) % ld[2], site // (ld[0] * ld[1] * ld[2])]
+        if all(0 < c[d] < ld[d] - 1 for d in cut):
+            assert np.array_equal(got[0, site], want[0, site])
