@@ -490,20 +490,14 @@ def halo_section(gbx, tune, dbl, psi, ref64, nrep):
         us_int = host_timed(lambda: gbx.dhop_range(u0, y0, x0, 0, nint), g0, nrep)
         us_bnd = host_timed(lambda: gbx.dhop_range(u0, y0, x0, nint, vol), g0, nrep)
 
-        def pack_all(proj):
-            for d in cut:
-                for f in (0, 1):
-                    x0.halo_pack(d, f, bufs[(c0, d)][f].ptr if proj else y0.halo_ghost(d, f), proj=proj)
-
-        def unpack_all():
-            for d in cut:
-                for side in (0, 1):
-                    x0.halo_unpack(d, side, bufs[(c0, d)][1 - side].ptr, proj=True)
-
-        # (whole faces are packed into y0's own ghosts: a scratch target of the right size)
-        us_pack = host_timed(lambda: pack_all(False), g0, nrep)
-        us_pack_proj = host_timed(lambda: pack_all(True), g0, nrep)
-        us_unpack_proj = host_timed(unpack_all, g0, nrep)
+        # every face in one launch; whole faces are packed into y0's own
+        # ghosts (a scratch target of the right size)
+        scratch = {(d, f): y0.halo_ghost(d, f) for d in cut for f in (0, 1)}
+        sendb = {(d, f): bufs[(c0, d)][f].ptr for d in cut for f in (0, 1)}
+        recvb = {(d, s): bufs[(c0, d)][1 - s].ptr for d in cut for s in (0, 1)}
+        us_pack = host_timed(lambda: x0.halo_pack_all(scratch), g0, nrep)
+        us_pack_proj = host_timed(lambda: x0.halo_pack_all(sendb, proj=True), g0, nrep)
+        us_unpack_proj = host_timed(lambda: x0.halo_unpack_all(recvb, proj=True), g0, nrep)
         # one rank's schedule on this GPU (faces moved by a device copy)
         us_ov, us_se = gbx.halo_overlap_time(u0, y0, x0, proj=True, nrep=nrep)
         us_ov_w, us_se_w = gbx.halo_overlap_time(u0, y0, x0, proj=False, nrep=nrep)

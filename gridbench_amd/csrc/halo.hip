@@ -80,6 +80,82 @@ __global__ __launch_bounds__(256) void k_halo_unpack_proj(const uint4 *__restric
   }
 }
 
+// All faces of a part in one launch.  Job j covers work items
+// [start_j, start_j + n_j) of the flattened index; at most 8 jobs (two faces
+// of four dimensions), found by a scan.
+struct FaceJob {
+  const uint32_t *fs;  // face slots (pack) or unused (unpack)
+  const void *src;     // unpack: the received face
+  void *dst;           // pack: the face buffer; unpack: the ghost range
+  uint64_t start, n, face;
+  int K;               // projector (projected pack)
+};
+struct FaceJobs {
+  FaceJob j[8];
+  int n;
+};
+
+__device__ __forceinline__ int find_job(const FaceJobs &J, uint64_t i)
+{
+  int k = 0;
+  while (k + 1 < J.n && i >= J.j[k + 1].start) k++;
+  return k;
+}
+
+__global__ __launch_bounds__(256) void k_halo_pack_all(const uint4 *__restrict__ src, FaceJobs J, uint64_t units,
+                                                       uint64_t total)
+{
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+    const FaceJob &b = J.j[find_job(J, i)];
+    const uint64_t r = i - b.start, f = r / units, u = r - f * units;
+    ((uint4 *)b.dst)[r] = src[(uint64_t)b.fs[f] * units + u];
+  }
+}
+
+template <int P>
+__global__ __launch_bounds__(256) void k_halo_pack_all_proj(const void *__restrict__ src, FaceJobs J, uint32_t M,
+                                                            uint64_t total)
+{
+  typedef typename dev::Traits<P>::acc A;
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R, HCH = NCH / 2;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+    const FaceJob &b = J.j[find_job(J, i)];
+    const uint64_t r = i - b.start, f = r / M, m = r - f * M, q = b.fs[f];
+    A v[24];
+#pragma unroll
+    for (int k = 0; k < NCH; k++) dev::load_chunk<P, A>(src, (q * NCH + k) * M + m, v + k * R);
+    A hr[6], hi[6], h[12];
+    switch (b.K) {
+    case 0: dev::project<0>(v, hr, hi); break;
+    case 1: dev::project<1>(v, hr, hi); break;
+    case 2: dev::project<2>(v, hr, hi); break;
+    case 3: dev::project<3>(v, hr, hi); break;
+    case 4: dev::project<4>(v, hr, hi); break;
+    case 5: dev::project<5>(v, hr, hi); break;
+    case 6: dev::project<6>(v, hr, hi); break;
+    default: dev::project<7>(v, hr, hi); break;
+    }
+#pragma unroll
+    for (int j = 0; j < 6; j++) {
+      h[2 * j] = hr[j];
+      h[2 * j + 1] = hi[j];
+    }
+#pragma unroll
+    for (int k = 0; k < HCH; k++) dev::store_chunk<P, A>(b.dst, (f * HCH + k) * M + m, h + k * R, false);
+  }
+}
+
+__global__ __launch_bounds__(256) void k_halo_unpack_all_proj(FaceJobs J, uint32_t nch, uint32_t M, uint64_t total)
+{
+  const uint32_t hch = nch / 2;
+  const uint64_t per = (uint64_t)nch * M;
+  for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (uint64_t)gridDim.x * blockDim.x) {
+    const FaceJob &b = J.j[find_job(J, i)];
+    const uint64_t r = i - b.start, f = r / per, rr = r - f * per, k = rr / M, m = rr - k * M;
+    ((uint4 *)b.dst)[r] = k < hch ? ((const uint4 *)b.src)[(f * hch + k) * M + m] : make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
 template <int P>
 void launch_pack_proj(int K, dim3 grid, hipStream_t st, const void *src, const uint32_t *fs, void *dst, uint64_t face,
                       uint32_t M)
@@ -222,6 +298,107 @@ extern "C" int gbx_halo_unpack_proj(gbx_spinor x, int dim, int side, const void 
   return unpack_impl(x, dim, side, true, dbuf, x->geom->stream, "gbx_halo_unpack_proj");
 }
 
+// Every face of every cut dimension in one launch: bufs[2*d + face] for cut
+// dimensions d (entries of uncut dimensions are ignored).
+static int pack_all_impl(gbx_spinor_s *x, int dag, bool proj, void *const *bufs, hipStream_t st, const char *who)
+{
+  GBX_CHECK(x && bufs, GBX_ENULL, "%s: NULL argument", who);
+  GBX_CHECK(x->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
+  GBX_CHECK(!proj || x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "%s: projected faces of FP64 / FP32 fields only", who);
+  const gbx_geom_s *g = x->geom;
+  const uint64_t units = slot_units(x);
+  const uint64_t per = proj ? (uint64_t)x->M : units;  // work items per face site
+  FaceJobs J{};
+  uint64_t total = 0;
+  for (int d = 0; d < 4; d++) {
+    if (g->grid[d] == 1) continue;
+    const uint64_t F = g->face[d];
+    for (int f = 0; f < 2; f++) {
+      void *b = bufs[2 * d + f];
+      GBX_CHECK(b, GBX_ENULL, "%s: NULL buffer for dimension %d face %d", who, d, f);
+      GBX_CHECK(((uintptr_t)b & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
+      GBX_CHECK(disjoint(b, F * units * (proj ? 8 : 16), x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+      FaceJob &jb = J.j[J.n++];
+      jb.fs = g->face_slots + g->fs_base[d] + (uint64_t)f * F;
+      jb.dst = b;
+      jb.src = nullptr;
+      jb.start = total;
+      jb.n = F * per;
+      jb.face = F;
+      jb.K = face_projector(d, f, dag);
+      total += jb.n;
+    }
+  }
+  if (!proj) {
+    hipLaunchKernelGGL(k_halo_pack_all, dim3(grid_for(total)), dim3(256), 0, st, (const uint4 *)x->d, J, units, total);
+  } else if (x->prec == GBX_F64) {
+    hipLaunchKernelGGL((k_halo_pack_all_proj<GBX_F64>), dim3(grid_for(total)), dim3(256), 0, st, x->d, J, x->M, total);
+  } else {
+    hipLaunchKernelGGL((k_halo_pack_all_proj<GBX_F32>), dim3(grid_for(total)), dim3(256), 0, st, x->d, J, x->M, total);
+  }
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+// The ghosts of every cut dimension from the received faces, bufs[2*d + side]:
+// side 0 (minus) takes the previous part's top face, side 1 the next part's
+// bottom face.
+static int unpack_all_impl(gbx_spinor_s *x, bool proj, const void *const *bufs, hipStream_t st, const char *who)
+{
+  GBX_CHECK(x && bufs, GBX_ENULL, "%s: NULL argument", who);
+  GBX_CHECK(x->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
+  const gbx_geom_s *g = x->geom;
+  if (!proj) {
+    for (int d = 0; d < 4; d++)
+      if (g->grid[d] > 1)
+        for (int side = 0; side < 2; side++) GBX_TRY(unpack_impl(x, d, side, false, bufs[2 * d + side], st, who));
+    return GBX_OK;
+  }
+  GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "%s: FP64 / FP32 fields", who);
+  const uint64_t units = slot_units(x);
+  const uint32_t nch = (uint32_t)spinor_nchunk(x->prec);
+  FaceJobs J{};
+  uint64_t total = 0;
+  for (int d = 0; d < 4; d++) {
+    if (g->grid[d] == 1) continue;
+    const uint64_t F = g->face[d];
+    for (int side = 0; side < 2; side++) {
+      const void *b = bufs[2 * d + side];
+      GBX_CHECK(b, GBX_ENULL, "%s: NULL buffer for dimension %d side %d", who, d, side);
+      GBX_CHECK(((uintptr_t)b & 15) == 0, GBX_EINVAL, "%s: buffer not 16-byte aligned", who);
+      GBX_CHECK(disjoint(b, F * units * 8, x), GBX_EINVAL, "%s: buffer overlaps the field", who);
+      FaceJob &jb = J.j[J.n++];
+      jb.fs = nullptr;
+      jb.src = b;
+      jb.dst = (char *)x->d + (g->ghost_base[d] + (uint64_t)side * F) * units * 16;
+      jb.start = total;
+      jb.n = F * units;
+      jb.face = F;
+      jb.K = 0;
+      total += jb.n;
+    }
+  }
+  hipLaunchKernelGGL(k_halo_unpack_all_proj, dim3(grid_for(total)), dim3(256), 0, st, J, nch, x->M, total);
+  GBX_HIP(hipGetLastError());
+  return GBX_OK;
+}
+
+extern "C" int gbx_halo_pack_all(gbx_spinor x, int proj, int dag, void *const bufs[8])
+{
+  GBX_CHECK(proj == 0 || proj == 1, GBX_EINVAL, "gbx_halo_pack_all: proj=%d", proj);
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_pack_all: NULL field");
+  return pack_all_impl(x, dag, proj != 0, bufs, x->geom->stream, "gbx_halo_pack_all");
+}
+
+extern "C" int gbx_halo_unpack_all(gbx_spinor x, int proj, const void *const bufs[8])
+{
+  GBX_CHECK(proj == 0 || proj == 1, GBX_EINVAL, "gbx_halo_unpack_all: proj=%d", proj);
+  GBX_CHECK(x, GBX_ENULL, "gbx_halo_unpack_all: NULL field");
+  return unpack_all_impl(x, proj != 0, bufs, x->geom->stream, "gbx_halo_unpack_all");
+}
+
 // One rank's application on one GPU, timed two ways (HIP events on the
 // geometry's stream, nrep applications after one warm-up each):
 //  serial:  pack the faces of every cut dimension, move them, unpack, Dhop on
@@ -285,20 +462,21 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   // faces out (before anything else: the pack kernels need CUs, which the
   // interior kernel would otherwise hold), then moved and into the ghosts:
   // the top face feeds the minus side
-  auto pack = [&](hipStream_t st) -> int {
-    for (size_t c = 0; c < cut.size(); c++)
-      for (int f = 0; f < 2; f++) GBX_TRY(pack_impl(in, cut[c], f, dag, proj != 0, r.buf[4 * c + f], st, who));
-    return GBX_OK;
-  };
+  void *send[8] = {}, *recv[8] = {};
+  for (size_t c = 0; c < cut.size(); c++) {
+    send[2 * cut[c]] = r.buf[4 * c];          // bottom face
+    send[2 * cut[c] + 1] = r.buf[4 * c + 1];  // top face
+    recv[2 * cut[c]] = r.buf[4 * c + 3];      // minus side <- the previous part's top face
+    recv[2 * cut[c] + 1] = r.buf[4 * c + 2];  // plus side <- the next part's bottom face
+  }
+  auto pack = [&](hipStream_t st) -> int { return pack_all_impl(in, dag, proj != 0, send, st, who); };
   auto move_unpack = [&](hipStream_t st) -> int {
     for (size_t c = 0; c < cut.size(); c++) {
       void **b = &r.buf[4 * c];
       GBX_HIP(hipMemcpyAsync(b[2], b[0], fb[c], hipMemcpyDeviceToDevice, st));
       GBX_HIP(hipMemcpyAsync(b[3], b[1], fb[c], hipMemcpyDeviceToDevice, st));
-      GBX_TRY(unpack_impl(in, cut[c], 0, proj != 0, b[3], st, who));
-      GBX_TRY(unpack_impl(in, cut[c], 1, proj != 0, b[2], st, who));
     }
-    return GBX_OK;
+    return unpack_all_impl(in, proj != 0, recv, st, who);
   };
   auto serial = [&]() -> int {
     GBX_TRY(pack(ms));

@@ -370,6 +370,18 @@ class SpinorField:
             check(lib.gbx_halo_pack(self.h, int(dim), int(face), C.c_void_p(dptr)), "gbx_halo_pack")
         return self
 
+    def halo_pack_all(self, bufs, proj=False, dag=False):
+        """bufs: {(dim, face): device pointer} for every cut dimension."""
+        arr = (C.c_void_p * 8)(*[bufs.get((d, f)) for d in range(4) for f in (0, 1)])
+        check(lib.gbx_halo_pack_all(self.h, int(bool(proj)), int(bool(dag)), arr), "gbx_halo_pack_all")
+        return self
+
+    def halo_unpack_all(self, bufs, proj=False):
+        """bufs: {(dim, side): device pointer} for every cut dimension."""
+        arr = (C.c_void_p * 8)(*[bufs.get((d, f)) for d in range(4) for f in (0, 1)])
+        check(lib.gbx_halo_unpack_all(self.h, int(bool(proj)), arr), "gbx_halo_unpack_all")
+        return self
+
     def halo_unpack(self, dim, side, dptr, proj=False):
         fn = lib.gbx_halo_unpack_proj if proj else lib.gbx_halo_unpack
         check(fn(self.h, int(dim), int(side), C.c_void_p(dptr)), "gbx_halo_unpack_proj" if proj else "gbx_halo_unpack")
@@ -481,21 +493,18 @@ def halo_exchange(fields, proj=False, dag=False, bufs=None):
         return tuple(c)
 
     cut = [d for d in range(4) if grid[d] > 1]
-    for c, x in fields.items():
-        for d in cut:
-            if proj:
-                x.halo_pack(d, 1, bufs[(c, d)][1].ptr, proj=True, dag=dag)
-                x.halo_pack(d, 0, bufs[(c, d)][0].ptr, proj=True, dag=dag)
-            else:
-                x.halo_pack(d, 1, fields[step(c, d, 1)].halo_ghost(d, 0))
-                x.halo_pack(d, 0, fields[step(c, d, -1)].halo_ghost(d, 1))
+    for c, x in fields.items():  # one pack launch per part
+        if proj:
+            x.halo_pack_all({(d, f): bufs[(c, d)][f].ptr for d in cut for f in (0, 1)}, proj=True, dag=dag)
+        else:
+            x.halo_pack_all({(d, f): fields[step(c, d, -1 if f == 0 else 1)].halo_ghost(d, 1 - f)
+                             for d in cut for f in (0, 1)})
     for x in fields.values():
         x.geom.sync()
     if proj:
-        for c, x in fields.items():
-            for d in cut:
-                x.halo_unpack(d, 0, bufs[(step(c, d, -1), d)][1].ptr, proj=True)
-                x.halo_unpack(d, 1, bufs[(step(c, d, 1), d)][0].ptr, proj=True)
+        for c, x in fields.items():  # one unpack launch per part
+            x.halo_unpack_all({(d, side): bufs[(step(c, d, -1 if side == 0 else 1), d)][1 - side].ptr
+                               for d in cut for side in (0, 1)}, proj=True)
         for x in fields.values():
             x.geom.sync()
     return fields

@@ -345,6 +345,12 @@ int gbx_halo_unpack(gbx_spinor x, int dim, int side, const void *dbuf);
 /* is bit-identical to the one from whole-spinor ghosts.                   */
 int gbx_halo_pack_proj(gbx_spinor x, int dim, int face, int dag, void *dbuf);
 int gbx_halo_unpack_proj(gbx_spinor x, int dim, int side, const void *dbuf);
+/* Every face of every cut dimension in one launch (proj 0 / 1 as above):  */
+/* pack face f of dimension d into bufs[2*d + f]; unpack the ghosts of     */
+/* (d, side) from bufs[2*d + side].  Entries of uncut dimensions are       */
+/* ignored.                                                                */
+int gbx_halo_pack_all(gbx_spinor x, int proj, int dag, void *const bufs[8]);
+int gbx_halo_unpack_all(gbx_spinor x, int proj, const void *const bufs[8]);
 /* One rank's application, timed on this GPU (microseconds per application, */
 /* HIP events on the geometry's stream, nrep after one warm-up each):       */
 /*  serial:  pack the faces of every cut dimension, move each into a       */

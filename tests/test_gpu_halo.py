@@ -180,6 +180,46 @@ def test_projected_face_is_half_and_matches_projection(gpu, gbx, problem, orc):
                 assert np.array_equal(got, want), (dim, f, s)
 
 
+@pytest.mark.parametrize("proj", [False, True])
+@pytest.mark.parametrize("dag", [False, True])
+def test_pack_all_equals_face_by_face(gpu, gbx, problem, proj, dag):
+    """The one-launch pack / unpack of every face gives the bytes of the
+    face-by-face entries."""
+    grid = (1, 2, 2, 2)
+    p = problem((4, 4, 4, 6), 3)
+    parts = Parts(gbx, p, grid, "f64")
+    x = parts.src[parts.coords[3]]
+    cut = [1, 2, 3]
+    one = {(d, f): gbx.DeviceBuffer(x.halo_bytes(d, proj)) for d in cut for f in (0, 1)}
+    each = {(d, f): gbx.DeviceBuffer(x.halo_bytes(d, proj)) for d in cut for f in (0, 1)}
+    x.halo_pack_all({k: b.ptr for k, b in one.items()}, proj=proj, dag=dag)
+    for (d, f), b in each.items():
+        x.halo_pack(d, f, b.ptr, proj=proj, dag=dag)
+    x.geom.sync()
+    for k in one:
+        assert np.array_equal(one[k].download(), each[k].download()), k
+    # unpack: all at once into x, one by one into a copy, compare the ghost ranges
+    y = gbx.SpinorField(x.geom, p.Ls, p.nrhs, "f64").copy_from(x)
+    x.halo_unpack_all({(d, s): one[(d, 1 - s)].ptr for d in cut for s in (0, 1)}, proj=proj)
+    for d in cut:
+        for s in (0, 1):
+            y.halo_unpack(d, s, one[(d, 1 - s)].ptr, proj=proj)
+    x.geom.sync()
+    for d in cut:
+        for s in (0, 1):
+            assert np.array_equal(ghosts(x, d, s), ghosts(y, d, s)), (d, s)
+
+
+def ghosts(x, d, side):
+    """The bytes of the ghost range (d, side) of x."""
+    import ctypes as C
+    from gridbench_amd._lib import lib
+    nb = x.halo_bytes(d)
+    out = np.empty(nb, np.uint8)
+    assert lib.gbx_buffer_download(C.c_void_p(x.halo_ghost(d, side)), nb, out.ctypes.data_as(C.c_void_p)) == 0
+    return out
+
+
 def test_halo_errors(gpu, gbx, problem):
     p = problem((4, 4, 4, 8), 2)
     whole = gbx.SpinorField(gbx.Geometry(p.dims), 2, 1, "f64")
