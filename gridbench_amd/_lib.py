@@ -132,6 +132,8 @@ PROTOTYPES = {
     "gbx_halo_unpack_proj": (i32, [vp, i32, i32, vp]),
     "gbx_halo_overlap_time": (i32, [vp, vp, vp, i32, i32, i32, pdbl, pdbl]),
     "gbx_halo_pack_all": (i32, [vp, i32, i32, C.POINTER(vp)]),
+    "gbx_halo_ghost_proj": (i32, [vp, i32, i32, C.POINTER(vp)]),
+    "gbx_dhop_faces_projected": (i32, [vp, vp, vp, i32]),
     "gbx_halo_unpack_all": (i32, [vp, i32, C.POINTER(vp)]),
     "gbx_buffer_create": (i32, [u64, C.POINTER(vp)]),
     "gbx_buffer_destroy": (i32, [vp]),

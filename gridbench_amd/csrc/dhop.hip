@@ -11,6 +11,7 @@
 // the hardware deals workgroups round-robin over the 8 XCDs, so each XCD
 // then walks one contiguous eighth of the slot range and its L2 sees the
 // spatially compact neighbourhoods the slot order provides.
+#include <algorithm>
 #include <mutex>
 #include <unordered_map>
 
@@ -23,13 +24,15 @@ namespace {
 // and minor indices m0 .. m0+mc-1 (minor chunk L / nwg_sites): with mc < M
 // the lattice is swept once per minor chunk, so the 8-neighbour working set
 // is that of an mc-wide field (the links are re-read once per chunk).
-template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false>
+// GP: a domain part whose ghosts hold spin-projected halves from ghost0 on
+// (dhop_site_gp).
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false>
 __global__ void __launch_bounds__(PF ? 256 : 1024)
     k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
            void *__restrict__ out, const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mc,
            uint32_t nwg_sites, uint32_t cblk, uint32_t nchunk, uint32_t nwg, uint32_t per_xcd, int xcd, int nt,
            typename dev::Traits<P>::acc alpha,
-           typename dev::Traits<P>::acc beta)
+           typename dev::Traits<P>::acc beta, uint32_t ghost0)
 {
   typedef typename dev::Traits<P>::acc A;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -52,7 +55,8 @@ __global__ void __launch_bounds__(PF ? 256 : 1024)
   for (uint32_t w = threadIdx.x; w < items; w += blockDim.x) {
     const uint32_t sl = w / mcc, m = m0 + (w - sl * mcc);
     A rr[12], ri[12];
-    dev::dhop_site<P, DAG, PF, EXACT>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    if constexpr (GP) dev::dhop_site_gp<P, DAG, PF>(in, sN + sl * 8, M, m, ghost0, sU + sl * 8 * 18, rr, ri);
+    else dev::dhop_site<P, DAG, PF, EXACT>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
     dev::store_site<P, AXPBY>(out, center, q0 + sl, M, m, rr, ri, alpha, beta, nt != 0);
   }
 }
@@ -69,15 +73,32 @@ struct LaunchArgs {
   uint32_t nslot, M, spw, mc, nwg_sites, cblk, nchunk, nwg, per_xcd;
   int xcd, nt;
   double alpha, beta;
+  uint32_t ghost0;
 };
 
-template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false>
 void launch_t(const LaunchArgs &a)
 {
   typedef typename dev::Traits<P>::acc A;
-  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF, EXACT>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out, a.center,
-                     a.nslot, a.M, a.spw, a.mc, a.nwg_sites, a.cblk, a.nchunk, a.nwg, a.per_xcd, a.xcd, a.nt, (A)a.alpha,
-                     (A)a.beta);
+  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF, EXACT, GP>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out,
+                     a.center, a.nslot, a.M, a.spw, a.mc, a.nwg_sites, a.cblk, a.nchunk, a.nwg, a.per_xcd, a.xcd, a.nt,
+                     (A)a.alpha, (A)a.beta, a.ghost0);
+}
+
+// Projected ghosts (FP64 / FP32, no axpby).
+template <int P>
+void launch_gp(bool dag, bool c12, int pf, const LaunchArgs &a)
+{
+#define GBX_GP(D_, C_)                                        \
+  if (pf == 2) launch_t<P, D_, C_, false, 2, false, true>(a); \
+  else if (pf == 1) launch_t<P, D_, C_, false, 1, false, true>(a); \
+  else launch_t<P, D_, C_, false, 0, false, true>(a);
+  if (dag) {
+    if (c12) { GBX_GP(true, true) } else { GBX_GP(true, false) }
+  } else {
+    if (c12) { GBX_GP(false, true) } else { GBX_GP(false, false) }
+  }
+#undef GBX_GP
 }
 
 template <int P, bool DAG, bool C12>
@@ -274,7 +295,16 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   a.nt = c.nt;
   a.alpha = alpha;
   a.beta = beta;
+  a.ghost0 = (uint32_t)u->geom->vol;
   const int pf = (axpby && !c.axpby_pf) ? 0 : c.pf;
+  if (c.projected_ghosts) {
+    GBX_CHECK((u->prec == GBX_F64 || u->prec == GBX_F32) && !axpby && !exact && u->geom->nghost, GBX_EUNSUPPORTED,
+              "dhop: projected ghosts need a domain part, FP64 / FP32 fields and no axpby");
+    if (u->prec == GBX_F64) launch_gp<GBX_F64>(dag, u->comp12, pf, a);
+    else launch_gp<GBX_F32>(dag, u->comp12, pf, a);
+    GBX_HIP(hipGetLastError());
+    return GBX_OK;
+  }
   if (exact) {
     if (u->prec == GBX_F64) launch_exact<GBX_F64>(dag, pf, a);
     else launch_exact<GBX_F32>(dag, pf, a);
@@ -392,6 +422,23 @@ extern "C" int gbx_dhop_range(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
   r.begin = slot_begin;
   r.end = slot_end;
   return dhop_run(u, out, in, dag, dhop_range_cfg(u, in), u->geom->stream, r);
+}
+
+extern "C" int gbx_dhop_faces_projected(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag)
+{
+  GBX_TRY(check_operands(u, out, in));
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_dhop_faces_projected: dag must be 0 or 1");
+  const gbx_geom_s *g = in->geom;
+  GBX_CHECK(g->nghost != 0, GBX_EINVAL, "gbx_dhop_faces_projected: the field's geometry is not a domain part");
+  GBX_CHECK(in->prec == GBX_F64 || in->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "gbx_dhop_faces_projected: projected ghosts of FP64 / FP32 fields only");
+  const gbx_kcfg k = dhop_range_cfg(u, in);
+  dhop_cfg c = k.kind == 0 ? from_kcfg(k) : dhop_default_cfg(u, in);
+  c.projected_ghosts = true;
+  SlotRange r;
+  r.begin = g->ninterior;
+  r.end = g->vol;
+  return dhop_launch(u, out, in, dag, c, 0.0, 1.0, nullptr, g->stream, false, r);
 }
 
 extern "C" int gbx_dhop_xpay(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta)

@@ -249,6 +249,43 @@ __device__ __forceinline__ void leg_compute(const A *v, const A *U, A *rr, A *ri
   recon<K, FIRST>(ur, ui, rr, ri);
 }
 
+// Legs of a domain part whose ghosts hold projected halves.  Neighbour slot
+// n >= ghost0 is half-slot n - ghost0 of the ghost region (NCH/2 chunks per
+// half-slot, after the ghost0 whole slots): its chunks k and k + NCH/2 are
+// both read from half-slot chunk k (the same lines, so the second read hits
+// the cache), v[0..11] then holds the two-spinor, and the leg uses it in
+// place of its projection.  Other neighbours are gathered and projected as
+// in dhop_site; the arithmetic is the same.
+template <int P>
+__device__ __forceinline__ bool load_nb_gp(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
+                                           uint32_t ghost0, typename Traits<P>::acc *v)
+{
+  constexpr int NCH = Traits<P>::NCH, HCH = NCH / 2, R = Traits<P>::R;
+  const bool g = nb >= ghost0;
+  const uint64_t full = (uint64_t)nb * NCH * M + m;
+  const uint64_t half = (uint64_t)ghost0 * NCH * M + (uint64_t)(g ? nb - ghost0 : 0) * HCH * M + m;
+#pragma unroll
+  for (int k = 0; k < NCH; k++) {
+    const uint64_t t = g ? half + (uint64_t)(k % HCH) * M : full + (uint64_t)k * M;
+    load_chunk<P, typename Traits<P>::acc>(in, t, v + k * R);
+  }
+  return g;
+}
+
+template <int K, bool FIRST, int MUL, typename A>
+__device__ __forceinline__ void leg_compute_gp(const A *v, bool g, const A *U, A *rr, A *ri)
+{
+  A hr[6], hi[6], ur[6], ui[6];
+  project<K>(v, hr, hi);
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    hr[j] = g ? v[2 * j] : hr[j];
+    hi[j] = g ? v[2 * j + 1] : hi[j];
+  }
+  su3_mul<MUL>(U, hr, hi, ur, ui);
+  recon<K, FIRST>(ur, ui, rr, ri);
+}
+
 // Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
 // PF = 0: gather and compute leg by leg.  PF = 1: the gather of leg d+1 is
 // issued before leg d is computed (two gather buffers in registers), which
@@ -312,6 +349,69 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
     load_nb<P>(in, nb[7], M, m, v1);
     leg_compute<LegProj<DAG, 6>::value, false, CH>(v0, U + 6 * 18, rr, ri);
     leg_compute<LegProj<DAG, 7>::value, false, CH>(v1, U + 7 * 18, rr, ri);
+  }
+}
+
+// dhop_site on a domain part whose ghosts hold projected halves (load_nb_gp).
+template <int P, bool DAG, int PF>
+__device__ __forceinline__ void dhop_site_gp(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
+                                             uint32_t ghost0, const typename Traits<P>::acc *U,
+                                             typename Traits<P>::acc *rr, typename Traits<P>::acc *ri)
+{
+  typedef typename Traits<P>::acc A;
+  constexpr int CH = (P == GBX_F64 || P == GBX_F32) ? MUL_CHAIN : MUL_PLAIN;
+  if (PF == 0) {
+    A v[24];
+#define GBX_LEG(D, FIRST)                                                                  \
+    {                                                                                      \
+      const bool g = load_nb_gp<P>(in, nb[D], M, m, ghost0, v);                            \
+      leg_compute_gp<LegProj<DAG, D>::value, FIRST, CH>(v, g, U + D * 18, rr, ri);         \
+    }
+    GBX_LEG(0, true)
+    GBX_LEG(1, false)
+    GBX_LEG(2, false)
+    GBX_LEG(3, false)
+    GBX_LEG(4, false)
+    GBX_LEG(5, false)
+    GBX_LEG(6, false)
+    GBX_LEG(7, false)
+#undef GBX_LEG
+  } else if (PF == 2) {
+    A v0[24], v1[24], v2[24];
+    bool g0 = load_nb_gp<P>(in, nb[0], M, m, ghost0, v0);
+    bool g1 = load_nb_gp<P>(in, nb[1], M, m, ghost0, v1);
+    bool g2 = load_nb_gp<P>(in, nb[2], M, m, ghost0, v2);
+    leg_compute_gp<LegProj<DAG, 0>::value, true, CH>(v0, g0, U + 0 * 18, rr, ri);
+    g0 = load_nb_gp<P>(in, nb[3], M, m, ghost0, v0);
+    leg_compute_gp<LegProj<DAG, 1>::value, false, CH>(v1, g1, U + 1 * 18, rr, ri);
+    g1 = load_nb_gp<P>(in, nb[4], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 2>::value, false, CH>(v2, g2, U + 2 * 18, rr, ri);
+    g2 = load_nb_gp<P>(in, nb[5], M, m, ghost0, v2);
+    leg_compute_gp<LegProj<DAG, 3>::value, false, CH>(v0, g0, U + 3 * 18, rr, ri);
+    g0 = load_nb_gp<P>(in, nb[6], M, m, ghost0, v0);
+    leg_compute_gp<LegProj<DAG, 4>::value, false, CH>(v1, g1, U + 4 * 18, rr, ri);
+    g1 = load_nb_gp<P>(in, nb[7], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 5>::value, false, CH>(v2, g2, U + 5 * 18, rr, ri);
+    leg_compute_gp<LegProj<DAG, 6>::value, false, CH>(v0, g0, U + 6 * 18, rr, ri);
+    leg_compute_gp<LegProj<DAG, 7>::value, false, CH>(v1, g1, U + 7 * 18, rr, ri);
+  } else {
+    A v0[24], v1[24];
+    bool g0 = load_nb_gp<P>(in, nb[0], M, m, ghost0, v0);
+    bool g1 = load_nb_gp<P>(in, nb[1], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 0>::value, true, CH>(v0, g0, U + 0 * 18, rr, ri);
+    g0 = load_nb_gp<P>(in, nb[2], M, m, ghost0, v0);
+    leg_compute_gp<LegProj<DAG, 1>::value, false, CH>(v1, g1, U + 1 * 18, rr, ri);
+    g1 = load_nb_gp<P>(in, nb[3], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 2>::value, false, CH>(v0, g0, U + 2 * 18, rr, ri);
+    g0 = load_nb_gp<P>(in, nb[4], M, m, ghost0, v0);
+    leg_compute_gp<LegProj<DAG, 3>::value, false, CH>(v1, g1, U + 3 * 18, rr, ri);
+    g1 = load_nb_gp<P>(in, nb[5], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 4>::value, false, CH>(v0, g0, U + 4 * 18, rr, ri);
+    g0 = load_nb_gp<P>(in, nb[6], M, m, ghost0, v0);
+    leg_compute_gp<LegProj<DAG, 5>::value, false, CH>(v1, g1, U + 5 * 18, rr, ri);
+    g1 = load_nb_gp<P>(in, nb[7], M, m, ghost0, v1);
+    leg_compute_gp<LegProj<DAG, 6>::value, false, CH>(v0, g0, U + 6 * 18, rr, ri);
+    leg_compute_gp<LegProj<DAG, 7>::value, false, CH>(v1, g1, U + 7 * 18, rr, ri);
   }
 }
 

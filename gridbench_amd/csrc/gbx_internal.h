@@ -174,6 +174,9 @@ struct dhop_cfg {
   // that of the plain kernel, and the extra centre loads make the pipeline
   // slower there) unless the caller asked for a pf explicitly.
   bool axpby_pf = false;
+  // The field's ghosts hold spin-projected halves (a domain part's face
+  // slots after gbx_halo_ghost_proj receives).
+  bool projected_ghosts = false;
 };
 
 // Output slots begin .. end-1 (end clipped to the volume).  The VALU and MFMA

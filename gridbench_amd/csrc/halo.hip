@@ -212,6 +212,19 @@ extern "C" int gbx_halo_ghost(gbx_spinor x, int dim, int side, void **dptr)
   return GBX_OK;
 }
 
+extern "C" int gbx_halo_ghost_proj(gbx_spinor x, int dim, int side, void **dptr)
+{
+  GBX_CHECK(dptr, GBX_ENULL, "gbx_halo_ghost_proj: NULL dptr");
+  GBX_TRY(part_ok(x, dim, "gbx_halo_ghost_proj"));
+  GBX_CHECK(side == 0 || side == 1, GBX_EINVAL, "gbx_halo_ghost_proj: side=%d (0 = minus, 1 = plus)", side);
+  GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "gbx_halo_ghost_proj: FP64 / FP32 fields");
+  const gbx_geom_s *g = x->geom;
+  const uint64_t slot = slot_units(x) * 16;
+  // half-slots of NCH/2 chunks after the vol whole slots, in the order of the whole-spinor ghosts
+  *dptr = (char *)x->d + g->vol * slot + (g->ghost_base[dim] - g->vol + (uint64_t)side * g->face[dim]) * (slot / 2);
+  return GBX_OK;
+}
+
 // Projector of the leg that reads the face on the receiving part: the top
 // face feeds the next part's backward leg (direction dim + 4), the bottom
 // face the previous part's forward leg (direction dim); the non-dagger
@@ -401,17 +414,20 @@ extern "C" int gbx_halo_unpack_all(gbx_spinor x, int proj, const void *const buf
 
 // One rank's application on one GPU, timed two ways (HIP events on the
 // geometry's stream, nrep applications after one warm-up each):
-//  serial:  pack the faces of every cut dimension, move them, unpack, Dhop on
-//           every slot, all on one stream;
-//  overlap: pack, then move / unpack on a second stream while the interior
-//           slots run, then the face slots once the ghosts have landed.
+//  serial:  pack the faces of every cut dimension (one launch), move them,
+//           Dhop on the interior slots, then on the face slots, all on one
+//           stream;
+//  overlap: pack, then move on a second stream while the interior slots
+//           run, then the face slots once the faces have landed.
 // The move stands in for the transfer between GPUs: a device-to-device copy
-// of each packed face into a receive buffer on this GPU.  The received faces
-// are this part's own, so the face sites of `out` hold no meaningful values
-// afterwards; the interior sites hold Dhop(in).
+// of each packed face straight to where the face launch reads it (the ghost
+// slots, or for projected faces the half-slots gbx_halo_ghost_proj names),
+// so nothing is unpacked.  The received faces are this part's own, so the
+// face sites of `out` hold no meaningful values afterwards; the interior
+// sites hold Dhop(in).
 namespace {
 struct OverlapRes {
-  std::vector<void *> buf;  // per cut dimension: send bottom, send top, receive bottom, receive top
+  std::vector<void *> buf;  // per cut dimension: the packed bottom and top faces
   hipStream_t cs = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~OverlapRes()
@@ -441,16 +457,23 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   for (int d = 0; d < 4; d++)
     if (g->grid[d] > 1) cut.push_back(d);
   OverlapRes r;
-  r.buf.assign(4 * cut.size(), nullptr);
+  r.buf.assign(2 * cut.size(), nullptr);
   std::vector<uint64_t> fb(cut.size());
+  void *send[8] = {}, *recv[8] = {};
   for (size_t c = 0; c < cut.size(); c++) {
-    fb[c] = g->face[cut[c]] * slot_units(in) * (proj ? 8 : 16);
-    for (int k = 0; k < 4; k++)
-      if (hipMalloc(&r.buf[4 * c + k], fb[c]) != hipSuccess) {
+    const int d = cut[c];
+    fb[c] = g->face[d] * slot_units(in) * (proj ? 8 : 16);
+    for (int k = 0; k < 2; k++) {
+      if (hipMalloc(&r.buf[2 * c + k], fb[c]) != hipSuccess) {
         (void)hipGetLastError();
-        r.buf[4 * c + k] = nullptr;
+        r.buf[2 * c + k] = nullptr;
         return gbx_fail(GBX_ENOMEM, "%s: face buffers", who);
       }
+      send[2 * d + k] = r.buf[2 * c + k];  // k = 0 bottom face, 1 top face
+      // the receive lands in place: whole faces in the ghost slots, projected
+      // ones in the half-slots the face launch reads
+      GBX_TRY(proj ? gbx_halo_ghost_proj(in, d, k, &recv[2 * d + k]) : gbx_halo_ghost(in, d, k, &recv[2 * d + k]));
+    }
   }
   GBX_HIP(hipStreamCreate(&r.cs));
   for (hipEvent_t &e : r.ev) GBX_HIP(hipEventCreate(&e));
@@ -462,36 +485,35 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   // faces out (before anything else: the pack kernels need CUs, which the
   // interior kernel would otherwise hold), then moved and into the ghosts:
   // the top face feeds the minus side
-  void *send[8] = {}, *recv[8] = {};
-  for (size_t c = 0; c < cut.size(); c++) {
-    send[2 * cut[c]] = r.buf[4 * c];          // bottom face
-    send[2 * cut[c] + 1] = r.buf[4 * c + 1];  // top face
-    recv[2 * cut[c]] = r.buf[4 * c + 3];      // minus side <- the previous part's top face
-    recv[2 * cut[c] + 1] = r.buf[4 * c + 2];  // plus side <- the next part's bottom face
-  }
   auto pack = [&](hipStream_t st) -> int { return pack_all_impl(in, dag, proj != 0, send, st, who); };
-  auto move_unpack = [&](hipStream_t st) -> int {
-    for (size_t c = 0; c < cut.size(); c++) {
-      void **b = &r.buf[4 * c];
-      GBX_HIP(hipMemcpyAsync(b[2], b[0], fb[c], hipMemcpyDeviceToDevice, st));
-      GBX_HIP(hipMemcpyAsync(b[3], b[1], fb[c], hipMemcpyDeviceToDevice, st));
-    }
-    return unpack_all_impl(in, proj != 0, recv, st, who);
+  // the minus side receives the previous part's top face, the plus side the next part's bottom face
+  auto move = [&](hipStream_t st) -> int {
+    for (size_t c = 0; c < cut.size(); c++)
+      for (int side = 0; side < 2; side++) {
+        const int d = cut[c];
+        GBX_HIP(hipMemcpyAsync(recv[2 * d + side], send[2 * d + 1 - side], fb[c], hipMemcpyDeviceToDevice, st));
+      }
+    return GBX_OK;
+  };
+  // the face sites: projected ghosts are read as they are, no unpack
+  auto face_sites = [&]() -> int {
+    return proj ? gbx_dhop_faces_projected(u, out, in, dag) : dhop_run(u, out, in, dag, k, ms, faces);
   };
   auto serial = [&]() -> int {
     GBX_TRY(pack(ms));
-    GBX_TRY(move_unpack(ms));
-    return dhop_run(u, out, in, dag, k, ms);
+    GBX_TRY(move(ms));
+    GBX_TRY(dhop_run(u, out, in, dag, k, ms, interior));
+    return face_sites();
   };
   auto overlapped = [&]() -> int {
     GBX_TRY(pack(ms));
-    GBX_HIP(hipEventRecord(r.ev[2], ms));  // faces packed, previous face slots done
+    GBX_HIP(hipEventRecord(r.ev[2], ms));  // faces packed, previous face sites done
     GBX_HIP(hipStreamWaitEvent(r.cs, r.ev[2], 0));
-    GBX_TRY(move_unpack(r.cs));
+    GBX_TRY(move(r.cs));
     GBX_HIP(hipEventRecord(r.ev[3], r.cs));
     GBX_TRY(dhop_run(u, out, in, dag, k, ms, interior));
     GBX_HIP(hipStreamWaitEvent(ms, r.ev[3], 0));
-    return dhop_run(u, out, in, dag, k, ms, faces);
+    return face_sites();
   };
   auto timed = [&](const std::function<int()> &f, double *us) -> int {
     GBX_TRY(f());  // warm-up

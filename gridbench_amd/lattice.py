@@ -370,6 +370,11 @@ class SpinorField:
             check(lib.gbx_halo_pack(self.h, int(dim), int(face), C.c_void_p(dptr)), "gbx_halo_pack")
         return self
 
+    def halo_ghost_proj(self, dim, side):
+        p = C.c_void_p()
+        check(lib.gbx_halo_ghost_proj(self.h, int(dim), int(side), C.byref(p)), "gbx_halo_ghost_proj")
+        return p.value
+
     def halo_pack_all(self, bufs, proj=False, dag=False):
         """bufs: {(dim, face): device pointer} for every cut dimension."""
         arr = (C.c_void_p * 8)(*[bufs.get((d, f)) for d in range(4) for f in (0, 1)])
@@ -453,6 +458,12 @@ def dhop_range(U, out, inp, slot_begin, slot_end, dag=False):
     return out
 
 
+def dhop_faces_projected(U, out, inp, dag=False):
+    """Dhop on a part's face slots, its ghosts holding projected halves."""
+    check(lib.gbx_dhop_faces_projected(U.h, out.h, inp.h, int(bool(dag))), "gbx_dhop_faces_projected")
+    return out
+
+
 def halo_overlap_time(U, out, inp, dag=False, proj=True, nrep=20):
     """(us_overlap, us_serial) per application of one part: face exchange
     (faces moved on this GPU) overlapped with the interior slots, or run
@@ -475,6 +486,27 @@ def halo_buffers(fields, proj=True):
     grid = some.geom.grid
     return {(c, d): (DeviceBuffer(x.halo_bytes(d, proj)), DeviceBuffer(x.halo_bytes(d, proj)))
             for c, x in fields.items() for d in range(4) if grid[d] > 1}
+
+
+def halo_exchange_in_place(fields, dag=False):
+    """Projected faces packed straight into the neighbours' half-slots
+    (gbx_halo_ghost_proj), one launch per part: after it, every part's face
+    slots are gbx_dhop_faces_projected(dag).  Synchronises every part's
+    stream before returning."""
+    grid = next(iter(fields.values())).geom.grid
+    cut = [d for d in range(4) if grid[d] > 1]
+
+    def step(c, d, k):
+        c = list(c)
+        c[d] = (c[d] + k) % grid[d]
+        return tuple(c)
+
+    for c, x in fields.items():
+        x.halo_pack_all({(d, f): fields[step(c, d, -1 if f == 0 else 1)].halo_ghost_proj(d, 1 - f)
+                         for d in cut for f in (0, 1)}, proj=True, dag=dag)
+    for x in fields.values():
+        x.geom.sync()
+    return fields
 
 
 def halo_exchange(fields, proj=False, dag=False, bufs=None):
@@ -797,4 +829,5 @@ __all__ = [
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
     "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time", "grid_coords", "halo_buffers",
+    "dhop_faces_projected", "halo_exchange_in_place",
 ]

@@ -287,6 +287,12 @@ int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_
 /* gbx_dhop_xpay with an explicit VALU-kernel configuration (kind 0).     */
 int gbx_dhop_xpay_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor x, gbx_spinor in, int dag, double alpha, double beta,
                       const gbx_kcfg *cfg);
+/* Dhop (dag as packed) on the face slots ninterior .. V_local-1 of a      */
+/* domain part whose received faces are projected halves placed at         */
+/* gbx_halo_ghost_proj: those legs skip the projection and multiply the    */
+/* received two-spinor, bit-identical to the whole-spinor route.  FP64 /   */
+/* FP32 fields; VALU kernel.                                               */
+int gbx_dhop_faces_projected(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag);
 /* gbx_dhop on output slots slot_begin .. slot_end-1 only (device order,   */
 /* 0 <= begin <= end <= V; other output slots are not written).  On a     */
 /* domain part, [0, ninterior) needs no ghost: it can run while the halo   */
@@ -345,6 +351,11 @@ int gbx_halo_unpack(gbx_spinor x, int dim, int side, const void *dbuf);
 /* is bit-identical to the one from whole-spinor ghosts.                   */
 int gbx_halo_pack_proj(gbx_spinor x, int dim, int face, int dag, void *dbuf);
 int gbx_halo_unpack_proj(gbx_spinor x, int dim, int side, const void *dbuf);
+/* Where a projected face received for (dim, side) can land so that       */
+/* gbx_dhop_faces_projected reads it as it is (no unpack): half-slots of   */
+/* the ghost region, in the order of the whole-spinor ghosts.  A field's    */
+/* ghosts hold either whole spinors or projected halves, not both.         */
+int gbx_halo_ghost_proj(gbx_spinor x, int dim, int side, void **dptr);
 /* Every face of every cut dimension in one launch (proj 0 / 1 as above):  */
 /* pack face f of dimension d into bufs[2*d + f]; unpack the ghosts of     */
 /* (d, side) from bufs[2*d + side].  Entries of uncut dimensions are       */
@@ -353,12 +364,13 @@ int gbx_halo_pack_all(gbx_spinor x, int proj, int dag, void *const bufs[8]);
 int gbx_halo_unpack_all(gbx_spinor x, int proj, const void *const bufs[8]);
 /* One rank's application, timed on this GPU (microseconds per application, */
 /* HIP events on the geometry's stream, nrep after one warm-up each):       */
-/*  serial:  pack the faces of every cut dimension, move each into a       */
-/*           receive buffer (a device-to-device copy standing in for the    */
-/*           transfer between GPUs), unpack, Dhop on every slot, all on the */
-/*           geometry's stream;                                             */
-/*  overlap: pack, then move / unpack on a second stream while the interior */
-/*           slots run, then the face slots after an event.                 */
+/*  serial:  pack the faces of every cut dimension, move each straight to  */
+/*           where the face launch reads it (a device-to-device copy        */
+/*           standing in for the transfer between GPUs: ghost slots, or the */
+/*           projected half-slots), Dhop on the interior slots, then on the */
+/*           face slots, all on the geometry's stream;                      */
+/*  overlap: pack, then move on a second stream while the interior slots    */
+/*           run, then the face slots after an event.                       */
 /* The moved faces are the part's own, so afterwards out's face sites hold  */
 /* no meaningful values (its interior sites hold Dhop(in)), and the ghosts */
 /* of in are overwritten.  proj = 1: spin-projected faces (FP64 / FP32).   */

@@ -75,6 +75,28 @@ def test_parts_match_whole_lattice(gpu, gbx, problem, dims, grid, dag, prec, pro
     del bufs
 
 
+@pytest.mark.parametrize("dims,grid", [((4, 4, 4, 8), (1, 1, 1, 2)), ((4, 8, 4, 12), (1, 2, 1, 2)),
+                                       ((4, 6, 6, 6), (2, 1, 3, 2)), ((4, 4, 4, 4), (2, 2, 2, 2))])
+@pytest.mark.parametrize("dag", [False, True])
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+def test_faces_from_projected_halves_in_place(gpu, gbx, problem, dims, grid, dag, prec):
+    """Projected faces packed straight into the neighbours' half-slots and
+    read there by the face launch (no unpack): the whole lattice's Dhop to
+    the bit."""
+    p = problem(dims, 4)
+    parts = Parts(gbx, p, grid, prec)
+    c0 = parts.coords[0]
+    cfg = gbx.dhop_get_cfg(parts.U[c0], parts.src[c0])
+    want = whole_dhop(gbx, p, prec, dag, cfg=cfg if cfg.kind == 0 else None)
+    gbx.halo_exchange_in_place(parts.src, dag=dag)
+    for U, d, s in parts.each():
+        gbx.dhop_range(U, d, s, 0, d.geom.ninterior, dag=dag)
+        gbx.dhop_faces_projected(U, d, s, dag=dag)
+    got = parts.merged()
+    assert np.array_equal(got, want)
+    assert rel_l2(got[0], p.oracle_lex(dag=dag)) <= (1e-14 if prec == "f64" else 1e-6)
+
+
 @pytest.mark.parametrize("grid", [(1, 1, 1, 2), (1, 2, 1, 2)])
 @pytest.mark.parametrize("dag", [False, True])
 def test_interior_then_faces(gpu, gbx, problem, grid, dag):
@@ -244,6 +266,12 @@ def test_halo_errors(gpu, gbx, problem):
         gbx.dhop(parts.U[c0], parts.dst[c0], x, cfg=gbx.kcfg(0, 256, kind=2, tile=gbx.pack_tile((2, 2, 2, 2))))
     with pytest.raises(gbx.GbxError):
         gbx.Geometry.part_of(p.dims, (1, 1, 1, 3), (0, 0, 0, 0))
+    with pytest.raises(gbx.GbxError):  # projected ghosts of 16-bit fields
+        x.halo_ghost_proj(3, 0)
+    with pytest.raises(gbx.GbxError):
+        gbx.dhop_faces_projected(parts.U[c0], parts.dst[c0], x)
+    with pytest.raises(gbx.GbxError):  # not a part
+        gbx.dhop_faces_projected(gbx.GaugeField(whole.geom, "f64"), gbx.SpinorField(whole.geom, 2, 1, "f64"), whole)
 
 
 @pytest.mark.parametrize("grid,proj", [((1, 1, 1, 2), False), ((1, 1, 1, 2), True), ((1, 2, 1, 2), True)])
