@@ -216,19 +216,20 @@ extern "C" int gbx_batch_time(gbx_batch b, int nrep, double *usec)
   GBX_CHECK(b && usec, GBX_ENULL, "gbx_batch_time: NULL argument");
   GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_batch_time: nrep=%d", nrep);
   GBX_TRY(gbx_batch_run(b));  // warm-up
-  hipEvent_t e0, e1;
-  GBX_HIP(hipEventCreate(&e0));
-  GBX_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GBX_OK;
-  (void)hipEventRecord(e0, b->stream);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = gbx_fail(GBX_EHIP, "gbx_batch_time: hipEventCreate failed");
+  if (rc == GBX_OK && hipEventRecord(e0, b->stream) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: event record");
   for (int r = 0; r < nrep && rc == GBX_OK; r++)
     if (hipGraphLaunch(b->exec, b->stream) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: graph launch");
-  (void)hipEventRecord(e1, b->stream);
-  if (hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: kernel failed");
+  if (rc == GBX_OK && hipEventRecord(e1, b->stream) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: event record");
+  if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: kernel failed");
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gbx_batch_time: elapsed time");
+  if (rc != GBX_OK) (void)hipStreamSynchronize(b->stream);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc == GBX_OK) *usec = 1000.0 * ms / nrep;
   return rc;
 }

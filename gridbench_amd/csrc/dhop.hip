@@ -482,18 +482,19 @@ extern "C" int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag
   else gbx_dhop_get_cfg(u, in, &k);
   hipStream_t st = u->geom->stream;
   GBX_TRY(dhop_run(u, out, in, dag, k, st));  // warm-up
-  hipEvent_t e0, e1;
-  GBX_HIP(hipEventCreate(&e0));
-  GBX_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GBX_OK;
-  if (hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = gbx_fail(GBX_EHIP, "hipEventCreate failed");
+  if (rc == GBX_OK && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
   for (int r = 0; r < nrep && rc == GBX_OK; r++) rc = dhop_run(u, out, in, dag, k, st);
   if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
   if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventSynchronize failed");
   float ms = 0.f;
   if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (rc != GBX_OK) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc == GBX_OK) *usec = 1000.0 * (double)ms / nrep;
   return rc;
 }

@@ -165,10 +165,10 @@ template <> struct RefPrec<float> {
 // Time nrep launches of `launch` on stream st, reference convention.
 template <typename F> int timed_reps(int nrep, hipStream_t st, F launch, double *usec)
 {
-  hipEvent_t e0, e1;
-  GBX_HIP(hipEventCreate(&e0));
-  GBX_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GBX_OK;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = gbx_fail(GBX_EHIP, "hipEventCreate failed");
   for (int rep = 0; rep < nrep && rc == GBX_OK; rep++) {
     if (rep == (nrep > 1 ? 1 : 0) && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
     if (rc == GBX_OK) rc = launch();
@@ -177,9 +177,10 @@ template <typename F> int timed_reps(int nrep, hipStream_t st, F launch, double 
   if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "kernel failed: %s",
                                                                          hipGetErrorString(hipGetLastError()));
   float ms = 0.f;
-  if (rc == GBX_OK) (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
+  if (rc != GBX_OK) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc == GBX_OK && usec) *usec = 1000.0 * (double)ms;
   return rc;
 }

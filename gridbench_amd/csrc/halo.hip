@@ -428,10 +428,12 @@ extern "C" int gbx_halo_unpack_all(gbx_spinor x, int proj, const void *const buf
 namespace {
 struct OverlapRes {
   std::vector<void *> buf;  // per cut dimension: the packed bottom and top faces
-  hipStream_t cs = nullptr;
+  hipStream_t cs = nullptr, ms = nullptr;
   hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
   ~OverlapRes()
   {
+    // both streams may still use the buffers when a step failed
+    if (ms) (void)hipStreamSynchronize(ms);
     if (cs) (void)hipStreamSynchronize(cs);
     for (void *b : buf) (void)hipFree(b);
     for (hipEvent_t e : ev)
@@ -457,6 +459,7 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   for (int d = 0; d < 4; d++)
     if (g->grid[d] > 1) cut.push_back(d);
   OverlapRes r;
+  r.ms = g->stream;
   r.buf.assign(2 * cut.size(), nullptr);
   std::vector<uint64_t> fb(cut.size());
   void *send[8] = {}, *recv[8] = {};

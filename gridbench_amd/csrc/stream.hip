@@ -40,30 +40,31 @@ extern "C" int gbx_stream_copy_bw(uint64_t bytes, int nrep, double *gbps)
     return gbx_fail(GBX_ENOMEM, "gbx_stream_copy_bw: cannot allocate %llu bytes", (unsigned long long)bytes);
   }
   int rc = GBX_OK;
-  hipStream_t st;
-  hipEvent_t e0, e1;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   if (hipStreamCreate(&st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipStreamCreate failed");
+  if (rc == GBX_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = gbx_fail(GBX_EHIP, "hipEventCreate failed");
   if (rc == GBX_OK) {
-    (void)hipEventCreate(&e0);
-    (void)hipEventCreate(&e1);
-    (void)hipMemsetAsync(a, 0x3c, bytes, st);
     const uint64_t n = bytes / 16;
     int dev = 0, ncu = 256;
     (void)hipGetDevice(&dev);
     (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
     const dim3 grid((unsigned)(ncu * 16)), block(256);
-    hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);  // warm-up
-    (void)hipEventRecord(e0, st);
-    for (int r = 0; r < nrep; r++) hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);
-    (void)hipEventRecord(e1, st);
-    if (hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "copy kernel failed");
+    if (hipMemsetAsync(a, 0x3c, bytes, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipMemsetAsync failed");
+    if (rc == GBX_OK) hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);  // warm-up
+    if (rc == GBX_OK && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+    for (int r = 0; r < nrep && rc == GBX_OK; r++) hipLaunchKernelGGL(k_copy16, grid, block, 0, st, (const u4 *)a, (u4 *)b, n);
+    if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+    if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "copy kernel failed");
     float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, e0, e1);
+    if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
     if (rc == GBX_OK) *gbps = 2.0 * (double)bytes * nrep / (ms * 1e-3) / 1e9;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipStreamDestroy(st);
   }
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
   (void)hipFree(a);
   (void)hipFree(b);
   return rc;

@@ -24,18 +24,19 @@ int time_cfg(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, in
 {
   hipStream_t st = u->geom->stream;
   GBX_TRY(dhop_run(u, out, in, dag, k, st));
-  hipEvent_t e0, e1;
-  GBX_HIP(hipEventCreate(&e0));
-  GBX_HIP(hipEventCreate(&e1));
+  hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GBX_OK;
-  (void)hipEventRecord(e0, st);
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)
+    rc = gbx_fail(GBX_EHIP, "tune: hipEventCreate failed");
+  if (rc == GBX_OK && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "tune: event record");
   for (int r = 0; r < nrep && rc == GBX_OK; r++) rc = dhop_run(u, out, in, dag, k, st);
-  (void)hipEventRecord(e1, st);
-  if (hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "tune: kernel failed");
+  if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "tune: event record");
+  if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "tune: kernel failed");
   float ms = 0.f;
-  (void)hipEventElapsedTime(&ms, e0, e1);
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "tune: elapsed time");
+  if (rc != GBX_OK) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
   if (rc == GBX_OK) *usec = 1000.0 * ms / nrep;
   return rc;
 }
