@@ -38,6 +38,11 @@ Also reported, each with its own seeds:
              the faces moved on this GPU (serial, and overlapped with the
              interior on a second stream); the xGMI transfer time is
              modelled (bytes / link rate), not measured.
+  dropin     the reference entry dslash_kernel (gbx_dslash_kernel_f64 /
+             _f32) on host arrays in the reference layout with
+             TableGenerate's tables at 24^4 x Ls16, as bench.cc calls it:
+             its own device time (applications 1..nrep-1) and the whole
+             call's wall time.
 
 Launch configurations come from the autotuner; its results for this machine
 are cached in gridbench_amd/tuned_mi355x.jsonl (re-validated on every run).
@@ -440,6 +445,41 @@ def batch_section(gbx, nrep):
     return r
 
 
+def dropin_section(gbx, dbl, psi, ref64, nrep):
+    """The reference entry itself: dslash_kernel (gbx_dslash_kernel_f64 /
+    _f32, WilsonKernelsHand.h:17-29) called as bench.cc calls it, on host
+    arrays in the reference layout (Nsimd 1, where it is the lexicographic
+    layout) with TableGenerate's tables, 24^4 x Ls16.  Reported: the
+    entry's own device time of applications 1..nrep-1 (the reference's
+    timing convention) with the reference's rounding, and the wall time of
+    the whole call (table check, both PCIe copies, repacking)."""
+    V = int(np.prod(DIMS))
+    nbr, prm = gbx.table_generate(DIMS, 1, LS)
+    rows = []
+    for dt, name in ((np.float64, "f64"), (np.float32, "f32")):
+        U = np.ascontiguousarray(dbl, dtype=dt).reshape(-1)
+        inp = np.ascontiguousarray(psi, dtype=dt).reshape(-1)
+        out = np.zeros_like(inp)
+        gbx.dslash_kernel(2, U, out, inp, nbr, V, LS, prm, 1)  # device buffers allocated once
+        t0 = time.perf_counter()
+        us = gbx.dslash_kernel(nrep, U, out, inp, nbr, V, LS, prm, 1)
+        wall = time.perf_counter() - t0
+        path = gbx.dslash_last_path()
+        per = us / (nrep - 1)
+        err = float(np.linalg.norm(out.astype(np.float64) - ref64.reshape(-1)) / np.linalg.norm(ref64))
+        r = {"prec": name, "nsimd": 1, "nrep": nrep, "path": "SoA kernel" if path == 1 else "table-driven kernel",
+             "us_per_application_device": round(per, 2),
+             "gflops_device": round(1320.0 * V * LS / (per * 1e-6) / 1e9, 1),
+             "wall_s_whole_call": round(wall, 3),
+             "rel_l2_vs_device_resident_f64": float(f"{err:.3e}")}
+        log(f"dropin {r}")
+        rows.append(r)
+        del U, inp, out
+    return {"dims": list(DIMS), "Ls": LS, "rows": rows,
+            "note": "device time counts applications 1..nrep-1 of the reference-rounding kernel; the wall time "
+                    "includes the PCIe copies of U, in, the tables and out"}
+
+
 XGMI_GBPS = 153.6  # one MI355X xGMI link, AMD-published per-link figure (direction split not stated)
 
 
@@ -556,8 +596,9 @@ def main():
     ap.add_argument("--no-cg", action="store_true", help="skip the defect-correction CG")
     ap.add_argument("--no-batch", action="store_true", help="skip the uneven batch")
     ap.add_argument("--no-halo", action="store_true", help="skip the domain-part halo section")
+    ap.add_argument("--no-dropin", action="store_true", help="skip the reference-entry (dslash_kernel) section")
     ap.add_argument("--headline-only", action="store_true",
-                    help="= --no-variants --no-mixed --no-cg --no-batch --no-halo")
+                    help="= --no-variants --no-mixed --no-cg --no-batch --no-halo --no-dropin")
     ap.add_argument("--only", default="", help="comma-separated variant names to run (profiling)")
     ap.add_argument("--no-compare", action="store_true", help="skip the MFMA-vs-VALU form comparison (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=0)
@@ -570,7 +611,7 @@ def main():
               "face pack / unpack, is the 'halo' section; transfers between GPUs are not run).", file=sys.stderr)
         sys.exit(2)
     if args.headline_only:
-        args.no_variants = args.no_mixed = args.no_cg = args.no_batch = args.no_halo = True
+        args.no_variants = args.no_mixed = args.no_cg = args.no_batch = args.no_halo = args.no_dropin = True
 
     import gridbench_amd as gbx
     if gbx.device_count() < 1:
@@ -630,6 +671,8 @@ def main():
         extra["batch"] = batch_section(gbx, 20)
     if not args.no_halo:
         extra["halo"] = halo_section(gbx, tune, dbl, psi, ref64, 100)
+    if not args.no_dropin:
+        extra["dropin"] = dropin_section(gbx, dbl, psi, ref64, 20)
 
     cpu = ref_cpu = None
     if not args.no_cpu:
