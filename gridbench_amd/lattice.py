@@ -210,6 +210,15 @@ class Geometry:
         g.gauge_dims = dims
         return g
 
+    @classmethod
+    def rb_pair(cls, dims, block=None):
+        """(even, odd) half-lattice geometries of one red-black pair."""
+        dims = tuple(int(d) for d in dims)
+        he, ho = C.c_void_p(), C.c_void_p()
+        check(lib.gbx_geom_create_rb(ivec(dims), ivec(block) if block is not None else None, C.byref(he),
+                                     C.byref(ho)), "gbx_geom_create_rb")
+        return cls(dims, _handle=he), cls(dims, _handle=ho)
+
     def sync(self):
         check(lib.gbx_sync(self.h), "gbx_sync")
 
