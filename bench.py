@@ -204,7 +204,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             gbx.dhop(Uref, w, x64, dag=dag)
         y = gbx.SpinorField(g, Ls, nrhs, prec)
         cfg, _ = tune(U, y, x, dag=dag)
-        us = gbx.dhop_time(U, y, x, dag=dag, nrep=20)
+        us = gbx.dhop_time(U, y, x, dag=dag, nrep=100)
         gbx.dhop(U, y, x, dag=dag)
         if Ls == LS:
             got = y.download_rhs(0)
@@ -221,7 +221,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             for kind in ("valu", "mfma", "tile"):
                 try:
                     kc, _ = tune(U, y, x, dag=dag, kinds=kind)
-                    forms[kind] = {"us_per_launch": round(gbx.dhop_time(U, y, x, dag=dag, cfg=kc, nrep=20), 2),
+                    forms[kind] = {"us_per_launch": round(gbx.dhop_time(U, y, x, dag=dag, cfg=kc, nrep=100), 2),
                                    "cfg": kc.as_dict()}
                 except gbx.GbxError as e:
                     forms[kind] = {"unsupported": str(e)}
