@@ -157,6 +157,7 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
     snprintf(buf, sizeof buf, "_part%dx%dx%dx%d", g->grid[0], g->grid[1], g->grid[2], g->grid[3]);
     s += buf;
   }
+  if (g->nlevels == 0 && g->nghost == 0 && g->parity < 0) s += "_order";  // a caller-defined site order
   for (int l = 0; l < g->nlevels; l++) {
     snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
              g->levels[l * 4 + 3]);

@@ -478,6 +478,37 @@ extern "C" int gbx_geom_create_levels(const int dims[4], int nlevels, const int 
   return GBX_OK;
 }
 
+// A caller-defined site order: slot_of_site[site] for every lexicographic
+// site, a permutation of 0 .. V-1.  The kernels visit slots in order (and,
+// with the XCD-contiguous launch, give each XCD one eighth of the slots), so
+// the order decides which neighbours are still in an L2 when they are read.
+extern "C" int gbx_geom_create_order(const int dims[4], const uint32_t *slot_of_site, gbx_geom *out)
+{
+  GBX_CHECK(dims && slot_of_site && out, GBX_ENULL, "gbx_geom_create_order: NULL argument");
+  *out = nullptr;
+  const int one[4] = {1, 1, 1, 1};
+  uint64_t vol;
+  GBX_TRY(check_dims_levels(dims, 1, one, &vol));
+  std::vector<uint32_t> sos(slot_of_site, slot_of_site + vol), site_of(vol, UINT32_MAX);
+  for (uint64_t site = 0; site < vol; site++) {
+    GBX_CHECK(sos[site] < vol && site_of[sos[site]] == UINT32_MAX, GBX_EINVAL,
+              "gbx_geom_create_order: slot_of_site is not a permutation (site %llu)", (unsigned long long)site);
+    site_of[sos[site]] = (uint32_t)site;
+  }
+  GBX_TRY(gbx_require_device());
+  int device;
+  GBX_HIP(hipGetDevice(&device));
+  gbx_geom_s *g = geom_new(dims, 0, one, vol, device);
+  g->slot_of_site_h = std::move(sos);
+  g->site_of_slot_h = std::move(site_of);
+  std::vector<uint32_t> nbr(vol * 8);
+  for (uint64_t q = 0; q < vol; q++)
+    for (int mu = 0; mu < 8; mu++) nbr[q * 8 + mu] = g->slot_of_site_h[nbr_site(g->site_of_slot_h[q], mu, dims)];
+  GBX_TRY(geom_finish(g, nbr, nullptr));  // releases g on failure
+  *out = g;
+  return GBX_OK;
+}
+
 // Red-black pair: the even and odd sites of the lattice, each in the blocked
 // order of the full lattice restricted to its parity (so a half's
 // consecutive slots stay spatially compact), each half's neighbour table

@@ -211,6 +211,16 @@ class Geometry:
         return g
 
     @classmethod
+    def with_order(cls, dims, slot_of_site):
+        """Geometry whose slots follow slot_of_site[lexicographic site]."""
+        dims = tuple(int(d) for d in dims)
+        a = np.ascontiguousarray(slot_of_site, dtype=np.uint32)
+        assert a.size == int(np.prod(dims))
+        h = C.c_void_p()
+        check(lib.gbx_geom_create_order(ivec(dims), ptr(a, C.c_uint32), C.byref(h)), "gbx_geom_create_order")
+        return cls(dims, _handle=h)
+
+    @classmethod
     def rb_pair(cls, dims, block=None):
         """(even, odd) half-lattice geometries of one red-black pair."""
         dims = tuple(int(d) for d in dims)

@@ -164,6 +164,9 @@ int gbx_geom_create(const int dims[4], const int block[4], gbx_geom *out);
 /* innermost first; each level divides the next and the last divides dims.  */
 #define GBX_MAX_LEVELS 4
 int gbx_geom_create_levels(const int dims[4], int nlevels, const int *blocks, gbx_geom *out);
+/* Any site order: slot_of_site[site] (a permutation of 0 .. V-1) for every */
+/* lexicographic site.  Slots are what the kernels visit in order.          */
+int gbx_geom_create_order(const int dims[4], const uint32_t *slot_of_site, gbx_geom *out);
 /* Red-black (even-odd) pair: the sites of parity (x+y+z+t) mod 2 = 0 and 1  */
 /* as two half-lattice geometries (every dim even).  The 4D checkerboard of */
 /* the reference's unused red-black grids (TableGenerate.cc:26,28), with Ls */
