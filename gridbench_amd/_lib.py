@@ -97,6 +97,7 @@ PROTOTYPES = {
     "gbx_geom_create_levels": (i32, [pi32, i32, pi32, C.POINTER(vp)]),
     "gbx_geom_create_rb": (i32, [pi32, pi32, C.POINTER(vp), C.POINTER(vp)]),
     "gbx_geom_create_order": (i32, [pi32, pu32, C.POINTER(vp)]),
+    "gbx_geom_create_part_rb": (i32, [pi32, pi32, pi32, pi32, C.POINTER(vp), C.POINTER(vp)]),
     "gbx_geom_parity": (i32, [vp, pi32]),
     "gbx_geom_create_part": (i32, [pi32, pi32, pi32, pi32, C.POINTER(vp)]),
     "gbx_part_tables": (i32, [pi32, pi32, pi32, pi32, pu32, pu32, pu32, pu32, pu64]),

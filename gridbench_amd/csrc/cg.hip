@@ -420,6 +420,7 @@ int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_
   const gbx_geom_s *ge = u_hi[0]->geom, *go = u_hi[1]->geom;
   GBX_CHECK(ge->parity == 0 && go->parity == 1 && ge->pair_id == go->pair_id, GBX_EMISMATCH,
             "%s: u_hi must be the even then the odd half of one red-black pair", who);
+  GBX_CHECK(ge->nghost == 0, GBX_EINVAL, "%s: the solvers take a whole lattice's halves, not a domain part's", who);
   for (int p = 0; p < 2; p++) {
     const gbx_geom_s *g = p ? go : ge;
     GBX_CHECK(u_hi[p]->prec == GBX_F64, GBX_EINVAL, "%s: the outer gauge fields must be FP64", who);

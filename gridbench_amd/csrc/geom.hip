@@ -751,6 +751,162 @@ extern "C" int gbx_geom_create_part(const int dims[4], const int grid[4], const 
   return GBX_OK;
 }
 
+// Red-black halves of a domain part: the sites of one global parity of the
+// block.  Every local extent must be even, so a block's offsets are even and
+// local parity is global parity.  A half's faces hold the face sites of its
+// parity: face_d / 2 of them, index f / 2 for the face site f of the part
+// (the fastest dimension across a face has even extent, so consecutive
+// face indices alternate in parity).  Half p's neighbour table points into
+// half 1 - p: its local slots or its ghost slots, which hold the neighbouring
+// blocks' face sites of parity 1 - p.
+namespace {
+
+int part_rb_tables(const int dims[4], const int grid[4], const int coord[4], const int block[4], PartTables h[2])
+{
+  PartTables t;
+  GBX_TRY(part_tables(dims, grid, coord, block, t));
+  for (int d = 0; d < 4; d++)
+    GBX_CHECK(t.ld[d] % 2 == 0, GBX_EINVAL, "red-black domain part: local extent %d along %d must be even", t.ld[d], d);
+  const uint64_t vol = t.vol, half = vol / 2;
+  for (int p = 0; p < 2; p++) {
+    PartTables &a = h[p];
+    for (int d = 0; d < 4; d++) {
+      a.ld[d] = t.ld[d];
+      a.blk[d] = t.blk[d];
+    }
+    a.vol = half;
+    a.gvol = t.gvol;
+    a.slot_of_site.assign(half, 0);
+    a.site_of_slot.clear();
+    a.gsite.clear();
+    uint64_t gb = half, fb = 0;
+    for (int d = 0; d < 4; d++)
+      if (t.face[d]) {
+        a.face[d] = t.face[d] / 2;
+        a.ghost_base[d] = gb;
+        a.fs_base[d] = fb;
+        gb += 2 * a.face[d];
+        fb += 2 * a.face[d];
+      }
+    a.nghost = gb - half;
+    a.fslots.assign(fb, 0);
+    a.ninterior = 0;
+  }
+  // the part's order (interior first) restricted to each parity
+  for (uint64_t q = 0; q < vol; q++) {
+    const uint32_t site = t.site_of_slot[q];
+    const int p = site_parity(site, t.ld);
+    PartTables &a = h[p];
+    a.slot_of_site[half_index(site, t.ld)] = (uint32_t)a.site_of_slot.size();
+    a.site_of_slot.push_back(half_index(site, t.ld));
+    a.gsite.push_back(t.gsite[q]);
+    if (q < t.ninterior) a.ninterior++;
+  }
+  // face slots of each half: face site f of the part has parity-face index f / 2
+  for (int d = 0; d < 4; d++) {
+    if (!t.face[d]) continue;
+    for (int side = 0; side < 2; side++)
+      for (uint64_t f = 0; f < t.face[d]; f++) {
+        const uint32_t site = t.site_of_slot[t.fslots[t.fs_base[d] + side * t.face[d] + f]];
+        PartTables &a = h[site_parity(site, t.ld)];
+        a.fslots[a.fs_base[d] + side * a.face[d] + f / 2] = a.slot_of_site[half_index(site, t.ld)];
+      }
+  }
+  // neighbours: the part's table, mapped into the partner half's slots
+  for (int p = 0; p < 2; p++) {
+    PartTables &a = h[p], &b = h[1 - p];
+    a.nbr.assign(half * 8, 0);
+    for (uint64_t q = 0; q < vol; q++) {
+      const uint32_t site = t.site_of_slot[q];
+      if (site_parity(site, t.ld) != p) continue;
+      const uint32_t hq = a.slot_of_site[half_index(site, t.ld)];
+      for (int mu = 0; mu < 8; mu++) {
+        const uint32_t n = t.nbr[q * 8 + mu];
+        uint32_t m;
+        if (n < vol) {
+          m = b.slot_of_site[half_index(t.site_of_slot[n], t.ld)];
+        } else {
+          // ghost slot of the part: (dimension, side, face index f) -> the half's (f / 2)
+          int d = 0;
+          while (!(t.face[d] && n >= t.ghost_base[d] && n < t.ghost_base[d] + 2 * t.face[d])) d++;
+          const uint64_t r = n - t.ghost_base[d], side = r / t.face[d], f = r - side * t.face[d];
+          m = (uint32_t)(b.ghost_base[d] + side * b.face[d] + f / 2);
+        }
+        a.nbr[hq * 8 + mu] = m;
+      }
+    }
+  }
+  return GBX_OK;
+}
+
+}  // namespace
+
+extern "C" int gbx_geom_create_part_rb(const int dims[4], const int grid[4], const int coord[4], const int block[4],
+                                       gbx_geom *even, gbx_geom *odd)
+{
+  GBX_CHECK(even && odd, GBX_ENULL, "gbx_geom_create_part_rb: NULL out");
+  *even = *odd = nullptr;
+  PartTables h[2];
+  GBX_TRY(part_rb_tables(dims, grid, coord, block, h));
+  GBX_TRY(gbx_require_device());
+  int device;
+  GBX_HIP(hipGetDevice(&device));
+  uint64_t id;
+  {
+    std::lock_guard<std::mutex> lk(g_pair_mu);
+    id = ++g_next_pair_id;
+  }
+  gbx_geom_s *g[2];
+  for (int p = 0; p < 2; p++) {
+    g[p] = geom_new(h[p].ld, 1, h[p].blk, h[p].vol, device);
+    g[p]->gauge_vol = h[p].gvol;
+    g[p]->parity = p;
+    g[p]->pair_id = id;
+    for (int d = 0; d < 4; d++) {
+      g[p]->grid[d] = grid[d];
+      g[p]->coord[d] = coord[d];
+      g[p]->gdims[d] = dims[d];
+      g[p]->face[d] = h[p].face[d];
+      g[p]->ghost_base[d] = h[p].ghost_base[d];
+      g[p]->fs_base[d] = h[p].fs_base[d];
+    }
+    g[p]->nghost = h[p].nghost;
+    g[p]->ninterior = h[p].ninterior;
+    g[p]->slot_of_site_h = std::move(h[p].slot_of_site);
+    g[p]->site_of_slot_h = std::move(h[p].site_of_slot);
+  }
+  auto upload_faces = [&](int p) -> int {
+    if (hipMalloc(&g[p]->face_slots, h[p].fslots.size() * 4) != hipSuccess ||
+        hipMemcpy(g[p]->face_slots, h[p].fslots.data(), h[p].fslots.size() * 4, hipMemcpyHostToDevice) != hipSuccess) {
+      (void)hipGetLastError();
+      return gbx_fail(GBX_ENOMEM, "gbx_geom_create_part_rb: face table");
+    }
+    return GBX_OK;
+  };
+  int rc = geom_finish(g[0], h[0].nbr, &h[0].gsite);  // releases g[0] on failure
+  if (rc != GBX_OK) {
+    geom_release(g[1]);
+    return rc;
+  }
+  g[1]->stream_owner = g[0];
+  geom_retain(g[0]);
+  rc = geom_finish(g[1], h[1].nbr, &h[1].gsite);  // releases g[1] (and its hold on g[0]) on failure
+  if (rc != GBX_OK) {
+    geom_release(g[0]);
+    return rc;
+  }
+  rc = upload_faces(0);
+  if (rc == GBX_OK) rc = upload_faces(1);
+  if (rc != GBX_OK) {
+    geom_release(g[1]);  // and its hold on g[0]
+    geom_release(g[0]);
+    return rc;
+  }
+  *even = g[0];
+  *odd = g[1];
+  return GBX_OK;
+}
+
 extern "C" int gbx_geom_part_info(gbx_geom g, int grid[4], int coord[4], uint64_t face[4], uint64_t *ninterior)
 {
   GBX_CHECK(g && grid && coord && face && ninterior, GBX_ENULL, "gbx_geom_part_info: NULL argument");

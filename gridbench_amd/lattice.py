@@ -221,6 +221,18 @@ class Geometry:
         return cls(dims, _handle=h)
 
     @classmethod
+    def part_rb_pair(cls, dims, grid, coord, block=None):
+        """(even, odd) halves of block `coord` of the lattice `dims` cut by `grid`."""
+        dims, grid, coord = (tuple(int(v) for v in a) for a in (dims, grid, coord))
+        he, ho = C.c_void_p(), C.c_void_p()
+        check(lib.gbx_geom_create_part_rb(ivec(dims), ivec(grid), ivec(coord), ivec(block) if block is not None else None,
+                                          C.byref(he), C.byref(ho)), "gbx_geom_create_part_rb")
+        local = tuple(d // n for d, n in zip(dims, grid))
+        ge, go = cls(local, _handle=he), cls(local, _handle=ho)
+        ge.gauge_dims = go.gauge_dims = dims
+        return ge, go
+
+    @classmethod
     def rb_pair(cls, dims, block=None):
         """(even, odd) half-lattice geometries of one red-black pair."""
         dims = tuple(int(d) for d in dims)

@@ -198,6 +198,16 @@ int gbx_geom_parity(gbx_geom g, int *parity);
 /* Parts run the VALU and MFMA kernels (not the tiled kernel) and gbx_dhop,   */
 /* gbx_dhop_range, gbx_dhop_xpay; solvers take whole lattices.              */
 int gbx_geom_create_part(const int dims[4], const int grid[4], const int coord[4], const int block[4], gbx_geom *out);
+/* Red-black halves of a domain part (every local extent even): the sites   */
+/* of global parity 0 / 1 of the block, as gbx_geom_create_rb splits a whole */
+/* lattice (checkerboarded local lexicographic host arrays, index           */
+/* (x/2) + (lx/2)*(y + ly*(z + lz*t)) in local coordinates).  Each half's   */
+/* ghosts hold the neighbouring blocks' face sites of its parity: face_d/2   */
+/* per face, face site f of the block at index f/2.  gbx_dhop(u_e, out_e,    */
+/* in_o) is DhopEO on the block once in_o's ghosts are filled (the halo     */
+/* functions work on halves unchanged).                                     */
+int gbx_geom_create_part_rb(const int dims[4], const int grid[4], const int coord[4], const int block[4], gbx_geom *even,
+                            gbx_geom *odd);
 /* grid, coord, face (0 for uncut dimensions) and ninterior; grid = 1 1 1 1,  */
 /* ninterior = V for a geometry that is not a part.                          */
 int gbx_geom_part_info(gbx_geom g, int grid[4], int coord[4], uint64_t face[4], uint64_t *ninterior);

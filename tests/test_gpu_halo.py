@@ -295,3 +295,69 @@ def test_overlap_timing_entry(gpu, gbx, problem, grid, proj):
         c = [site % ld[0], site // ld[0] % ld[1], site // (ld[0] * ld[1]) % ld[2], site // (ld[0] * ld[1] * ld[2])]
         if all(0 < c[d] < ld[d] - 1 for d in cut):
             assert np.array_equal(got[0, site], want[0, site])
+
+
+@pytest.mark.parametrize("dims,grid", [((4, 4, 4, 8), (1, 1, 1, 2)), ((4, 4, 4, 4), (1, 2, 2, 2)),
+                                       ((8, 4, 4, 4), (2, 1, 1, 2))])
+@pytest.mark.parametrize("dag", [False, True])
+@pytest.mark.parametrize("proj", [False, True])
+def test_red_black_parts(gpu, gbx, problem, dims, grid, dag, proj):
+    """Red-black halves of domain parts: DhopEO and DhopOE on every block,
+    after the halves' faces are exchanged, equal the whole lattice's halves
+    (gbx_geom_create_rb) bit for bit."""
+    p = problem(dims, 4)
+    cfg = gbx.kcfg(4, 64)
+    ge, go = gbx.Geometry.rb_pair(dims)
+    Ue, Uo = gbx.GaugeField(ge, "f64").upload(p.dbl), gbx.GaugeField(go, "f64").upload(p.dbl)
+    pe, po = gbx.rb_split(dims, p.psi)
+    want = {}
+    for par, (U, gout, gin, src) in enumerate(((Ue, ge, go, po), (Uo, go, ge, pe))):
+        x = gbx.SpinorField(gin, p.Ls, 1, "f64").upload(src)
+        y = gbx.SpinorField(gout, p.Ls, 1, "f64")
+        gbx.dhop(U, y, x, dag=dag, cfg=cfg)
+        want[par] = y.download()
+    ld = tuple(d // n for d, n in zip(dims, grid))
+    loc = gbx.part_split(dims, grid, p.psi)
+    halves = {c: gbx.Geometry.part_rb_pair(dims, grid, c) for c in gbx.grid_coords(grid)}
+    for par in (0, 1):  # out on parity par, in on 1 - par
+        U, x, y = {}, {}, {}
+        for c, (he, ho) in halves.items():
+            hout, hin = (he, ho) if par == 0 else (ho, he)
+            U[c] = gbx.GaugeField(hout, "f64").upload(p.dbl)
+            x[c] = gbx.SpinorField(hin, p.Ls, 1, "f64").upload(gbx.rb_split(ld, loc[c])[1 - par])
+            y[c] = gbx.SpinorField(hout, p.Ls, 1, "f64")
+        if proj:
+            gbx.halo_exchange_in_place(x, dag=dag)
+            for c in x:
+                gbx.dhop_range(U[c], y[c], x[c], 0, y[c].geom.ninterior, dag=dag)
+                gbx.dhop_faces_projected(U[c], y[c], x[c], dag=dag)
+        else:
+            gbx.halo_exchange(x)
+            for c in x:
+                gbx.dhop(U[c], y[c], x[c], dag=dag, cfg=cfg)
+        for g in (h for pair in halves.values() for h in pair):
+            g.sync()
+        zero = np.zeros_like(gbx.rb_split(ld, loc[next(iter(loc))])[0])
+        full = {c: (gbx.rb_merge(ld, y[c].download(), zero) if par == 0 else gbx.rb_merge(ld, zero, y[c].download()))
+                for c in x}
+        got = gbx.rb_split(dims, gbx.part_merge(dims, grid, full))[par]
+        assert np.array_equal(got, want[par]), (par, np.abs(got - want[par]).max())
+
+
+def test_solvers_reject_domain_parts(gpu, gbx):
+    """The solvers run on whole lattices: a domain part's halves (whose ghosts
+    the solver would never refresh) and a part are refused."""
+    he, ho = gbx.Geometry.part_rb_pair((4, 4, 4, 8), (1, 1, 1, 2), (0, 0, 0, 0))
+    U = (gbx.GaugeField(he, "f64"), gbx.GaugeField(ho, "f64"))
+    x = (gbx.SpinorField(he, 2, 1, "f64"), gbx.SpinorField(ho, 2, 1, "f64"))
+    b = (gbx.SpinorField(he, 2, 1, "f64"), gbx.SpinorField(ho, 2, 1, "f64"))
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.cg_solve_eo(U, None, x, b, 0.1)
+    assert e.value.status == 1
+    with pytest.raises(gbx.GbxError):
+        gbx.dwf_solve_eo(U, None, x, b)
+    g = gbx.Geometry.part_of((4, 4, 4, 8), (1, 1, 1, 2), (0, 0, 0, 0))
+    Ug = gbx.GaugeField(g, "f64")
+    xg, bg = gbx.SpinorField(g, 2, 1, "f64"), gbx.SpinorField(g, 2, 1, "f64")
+    with pytest.raises(gbx.GbxError):
+        gbx.cg_solve(Ug, None, xg, bg, 0.1)
