@@ -361,3 +361,62 @@ def test_solvers_reject_domain_parts(gpu, gbx):
     xg, bg = gbx.SpinorField(g, 2, 1, "f64"), gbx.SpinorField(g, 2, 1, "f64")
     with pytest.raises(gbx.GbxError):
         gbx.cg_solve(Ug, None, xg, bg, 0.1)
+
+
+def test_cg_over_parts_matches_the_whole_lattice(gpu, gbx, problem):
+    """A solver over domain parts: CG on M^dag M x = b (M = 1 - kappa Dhop)
+    with every M and M^dag applied block by block after a face exchange,
+    and every inner product summed over the blocks, as ranks would run it.
+    Its solution solves the whole lattice's system to the tolerance, and
+    agrees with the library's whole-lattice solver."""
+    dims, grid, kappa = (4, 4, 4, 8), (1, 1, 2, 2), 0.12
+    p = problem(dims, 4)
+    b_lex = p.psi
+    parts = Parts(gbx, p, grid, "f64")            # src = b, one field per block
+    coords = parts.coords
+    mk = lambda: {c: gbx.SpinorField(parts.geoms[c], p.Ls, 1, "f64").zero() for c in coords}
+    b, x, r, q, t, Aq = parts.src, mk(), mk(), mk(), mk(), mk()
+
+    def apply_M(out, inp, dag):  # out = inp - kappa Dhop(inp), inp's faces exchanged first
+        gbx.halo_exchange(inp)
+        for c in coords:
+            gbx.dhop_xpay(parts.U[c], out[c], inp[c], inp[c], 1.0, -kappa, dag=dag)
+
+    def dot(u, v):
+        return sum(u[c].dot(v[c]).real for c in coords)
+
+    for c in coords:
+        r[c].copy_from(b[c])
+        q[c].copy_from(b[c])
+    rr, bb = dot(r, r), dot(b, b)
+    it = 0
+    while rr > 1e-24 * bb and it < 500:
+        apply_M(t, q, False)
+        apply_M(Aq, t, True)
+        alpha = rr / dot(q, Aq)
+        for c in coords:
+            x[c].axpy(alpha, q[c])
+            r[c].axpy(-alpha, Aq[c])
+        rr_new = dot(r, r)
+        for c in coords:
+            q[c].scale(rr_new / rr).axpy(1.0, r[c])
+        rr, it = rr_new, it + 1
+    assert rr <= 1e-24 * bb, it
+    for g in parts.geoms.values():
+        g.sync()
+    xs = gbx.part_merge(dims, grid, {c: x[c].download() for c in coords})
+    # the whole lattice: residual of the assembled solution, and the library's own solver
+    geom = gbx.Geometry(dims)
+    U = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    X = gbx.SpinorField(geom, p.Ls, 1, "f64").upload(xs)
+    B = gbx.SpinorField(geom, p.Ls, 1, "f64").upload(b_lex)
+    T = gbx.SpinorField(geom, p.Ls, 1, "f64")
+    R = gbx.SpinorField(geom, p.Ls, 1, "f64")
+    gbx.dhop_xpay(U, T, X, X, 1.0, -kappa)
+    gbx.dhop_xpay(U, R, T, T, 1.0, -kappa, dag=True)
+    R.axpy(-1.0, B)
+    assert np.sqrt(R.norm2() / B.norm2()) <= 1e-10
+    X2 = gbx.SpinorField(geom, p.Ls, 1, "f64").zero()
+    gbx.cg_solve(U, None, X2, B, kappa, tol=1e-12)
+    X2.axpy(-1.0, X)
+    assert np.sqrt(X2.norm2() / X.norm2()) <= 1e-9
