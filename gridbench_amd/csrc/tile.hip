@@ -132,19 +132,49 @@ __global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArg
   for (int i = threadIdx.x; i < T; i += blockDim.x) sQ[i] = a.tslots[(uint64_t)tile * T + i];
   __syncthreads();
   // The tile's spinors (minor chunk), links and neighbour slots into LDS.
+  // Each thread issues NB loads before it waits for any of them: a loop of
+  // load-then-store keeps one load per wavefront in flight, and the staging
+  // phase then runs at L2/HBM latency rather than bandwidth.  Indexes past
+  // the end are clamped to the last element (loaded, not stored).
+  constexpr int NB = 8;
+  const uint32_t nthr = blockDim.x;
   const V *gin = (const V *)a.in;
   const uint32_t nload = (uint32_t)T * NCH * mcc;
-  for (uint32_t w = threadIdx.x; w < nload; w += blockDim.x) {
-    const uint32_t ik = w / mcc, m = w - ik * mcc;
-    const uint32_t i = ik / NCH, k = ik - i * NCH;
-    sIn[(i * a.mc + m) * SW + k] = gin[((uint64_t)sQ[i] * NCH + k) * a.M + m0 + m];
+  for (uint32_t w0 = threadIdx.x; w0 < nload; w0 += NB * nthr) {
+    V tmp[NB];
+    uint32_t dst[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const uint32_t w = min(w0 + j * nthr, nload - 1);
+      const uint32_t ik = w / mcc, m = w - ik * mcc;
+      const uint32_t i = ik / NCH, k = ik - i * NCH;
+      tmp[j] = gin[((uint64_t)sQ[i] * NCH + k) * a.M + m0 + m];
+      dst[j] = (i * a.mc + m) * SW + k;
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++)
+      if (w0 + j * nthr < nload) sIn[dst[j]] = tmp[j];
   }
   typedef typename dev::Traits<P>::st S;
   const S *gU = (const S *)a.gauge;
-  for (uint32_t w = threadIdx.x; w < (uint32_t)T * 8 * RL; w += blockDim.x) {
-    const uint32_t i = w / (8 * RL), r = w - i * (8 * RL);
-    const uint32_t link = r / RL, k = r - link * RL;
-    sU[(i * 8 + link) * 18 + k] = (A)gU[(uint64_t)sQ[i] * 8 * RL + r];
+  const uint32_t nlink = (uint32_t)T * 8 * RL;
+  for (uint32_t w0 = threadIdx.x; w0 < nlink; w0 += NB * nthr) {
+    S tmp[NB];
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const uint32_t w = min(w0 + j * nthr, nlink - 1);
+      const uint32_t i = w / (8 * RL), r = w - i * (8 * RL);
+      tmp[j] = gU[(uint64_t)sQ[i] * 8 * RL + r];
+    }
+#pragma unroll
+    for (int j = 0; j < NB; j++) {
+      const uint32_t w = w0 + j * nthr;
+      if (w < nlink) {
+        const uint32_t i = w / (8 * RL), r = w - i * (8 * RL);
+        const uint32_t link = r / RL, k = r - link * RL;
+        sU[(i * 8 + link) * 18 + k] = (A)tmp[j];
+      }
+    }
   }
   for (uint32_t w = threadIdx.x; w < (uint32_t)T * 8; w += blockDim.x) {
     sN[w] = a.nbr[(uint64_t)sQ[w >> 3] * 8 + (w & 7)];

@@ -86,7 +86,7 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
       for (int mc : minor_chunks(M)) {
         const uint32_t mw = mc ? (uint32_t)mc : M;
         const uint64_t items = (uint64_t)e[0] * e[1] * e[2] * e[3] * mw;
-        for (int t : {256, 512, 1024}) {
+        for (int t : {256, 512, 768, 1024}) {
           if ((uint64_t)t > (items + 63) / 64 * 64 && t != 256) continue;
           gbx_kcfg k{0, t, 1, 1, 2, mc, 0, 0, 0, pack_tile(e)};
           if (tile_cfg_check(k, u, in) != GBX_OK) continue;
