@@ -191,7 +191,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
                 c4["dbl"] = gbx.gen_doubled_gauge(DIMS, C4_GAUGE_SEED)
                 c4["U64"] = gbx.GaugeField(g, "f64").upload(c4["dbl"])
             gauge, Uref = c4["dbl"], c4["U64"]
-        U = Uref if prec == "f64" else gbx.GaugeField(g, prec, comp12).upload(gauge)
+        U = Uref if (prec == "f64" and not comp12) else gbx.GaugeField(g, prec, comp12).upload(gauge)
         x = gbx.SpinorField(g, Ls, nrhs, prec)
         if Ls == LS:
             x.fill_gaussian(SPINOR_SEED + 1000)  # right-hand sides 1.. ; 0 is the headline spinor
@@ -241,6 +241,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             U.close()
 
     variant("f64_1rhs_dag", "f64", 1, dag=True)
+    variant("f64_1rhs_comp12", "f64", 1, comp12=True)
     variant("f32_1rhs", "f32", 1)
     variant("f32_16rhs", "f32", 16, compare=True)
     variant("f16_16rhs", "f16", 16, compare=True)
