@@ -124,6 +124,8 @@ PROTOTYPES = {
     "gbx_spinor_download_ref_f32": (i32, [vp, pflt, i32]),
     "gbx_spinor_copy": (i32, [vp, vp]),
     "gbx_spinor_fill_gaussian": (i32, [vp, u64]),
+    "gbx_spinor_set_basis": (i32, [vp, i32]),
+    "gbx_spinor_get_basis": (i32, [vp, pi32]),
     "gbx_dhop": (i32, [vp, vp, vp, i32]),
     "gbx_dhop_range": (i32, [vp, vp, vp, i32, u64, u64]),
     "gbx_halo_bytes": (i32, [vp, i32, i32, pu64]),

@@ -140,6 +140,7 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     GBX_CHECK(u[i]->prec == prec && in[i]->prec == prec && out[i]->prec == prec, GBX_EMISMATCH,
               "gbx_batch_create: lattice %d precision differs from lattice 0", i);
     GBX_CHECK(in[i]->M == out[i]->M && in[i]->d != out[i]->d, GBX_EMISMATCH, "gbx_batch_create: lattice %d in/out mismatch", i);
+    GBX_TRY(require_chiral("gbx_batch_create", in[i], out[i]));
     for (int j = 0; j < n; j++)
       GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
                 "gbx_batch_create: output %d aliases another lattice's field", i);

@@ -292,6 +292,7 @@ extern "C" int gbx_spinor_dot(gbx_spinor x, gbx_spinor y, double *re, double *im
   GBX_CHECK(re && im, GBX_ENULL, "gbx_spinor_dot: NULL result");
   GBX_TRY(same_shape(x, y, "gbx_spinor_dot"));
   GBX_CHECK(x->prec == y->prec, GBX_EMISMATCH, "gbx_spinor_dot: precisions differ");
+  GBX_CHECK(x->basis == y->basis, GBX_EMISMATCH, "gbx_spinor_dot: spin bases differ");
   return spinor_dot_on(x, y, re, im, x->geom->stream);
 }
 
@@ -299,6 +300,7 @@ extern "C" int gbx_spinor_axpy(double a, gbx_spinor x, gbx_spinor y)
 {
   GBX_TRY(same_shape(x, y, "gbx_spinor_axpy"));
   GBX_CHECK(x->prec == y->prec, GBX_EMISMATCH, "gbx_spinor_axpy: precisions differ");
+  GBX_CHECK(x->basis == y->basis, GBX_EMISMATCH, "gbx_spinor_axpy: spin bases differ");
   return spinor_axpy_on(a, x, y, x->geom->stream);
 }
 
@@ -323,6 +325,7 @@ extern "C" int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed)
   default: hipLaunchKernelGGL(k_fill_gaussian<GBX_BF16>, grid, block, 0, st, x->d, sos, npts, x->M, (uint32_t)x->Ls, x->geom->vol, key); break;
   }
   GBX_HIP(hipGetLastError());
+  x->basis = GBX_BASIS_CHIRAL;  // the values are those of a chiral-basis field
   return GBX_OK;
 }
 

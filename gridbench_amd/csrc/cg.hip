@@ -352,6 +352,7 @@ int solve(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, gbx_spinor_s *x, con
           gbx_cg_stats *stats)
 {
   GBX_TRY(check_args(u_hi, u_lo, x, b, prm));
+  GBX_TRY(require_chiral("cg", x, b));
   const auto t0 = std::chrono::steady_clock::now();
   gbx_cg_stats st{};
   gbx_geom_s *g = u_hi->geom;
@@ -421,6 +422,8 @@ int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_
   GBX_CHECK(ge->parity == 0 && go->parity == 1 && ge->pair_id == go->pair_id, GBX_EMISMATCH,
             "%s: u_hi must be the even then the odd half of one red-black pair", who);
   GBX_CHECK(ge->nghost == 0, GBX_EINVAL, "%s: the solvers take a whole lattice's halves, not a domain part's", who);
+  GBX_TRY(require_chiral(who, x[0], x[1]));
+  GBX_TRY(require_chiral(who, b[0], b[1]));
   for (int p = 0; p < 2; p++) {
     const gbx_geom_s *g = p ? go : ge;
     GBX_CHECK(u_hi[p]->prec == GBX_F64, GBX_EINVAL, "%s: the outer gauge fields must be FP64", who);

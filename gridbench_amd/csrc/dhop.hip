@@ -25,8 +25,8 @@ namespace {
 // the lattice is swept once per minor chunk, so the 8-neighbour working set
 // is that of an mc-wide field (the links are re-read once per chunk).
 // GP: a domain part whose ghosts hold spin-projected halves from ghost0 on
-// (dhop_site_gp).
-template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false>
+// (dhop_site_gp).  TD: fields in the GBX_BASIS_TDIAG spin basis.
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false, bool TD = false>
 __global__ void __launch_bounds__(PF ? 256 : 1024)
     k_dhop(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
            void *__restrict__ out, const void *__restrict__ center, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mc,
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(PF ? 256 : 1024)
     const uint32_t sl = w / mcc, m = m0 + (w - sl * mcc);
     A rr[12], ri[12];
     if constexpr (GP) dev::dhop_site_gp<P, DAG, PF>(in, sN + sl * 8, M, m, ghost0, sU + sl * 8 * 18, rr, ri);
-    else dev::dhop_site<P, DAG, PF, EXACT>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    else dev::dhop_site<P, DAG, PF, EXACT, TD>(in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
     dev::store_site<P, AXPBY>(out, center, q0 + sl, M, m, rr, ri, alpha, beta, nt != 0);
   }
 }
@@ -76,11 +76,11 @@ struct LaunchArgs {
   uint32_t ghost0;
 };
 
-template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false>
+template <int P, bool DAG, bool C12, bool AXPBY, int PF, bool EXACT = false, bool GP = false, bool TD = false>
 void launch_t(const LaunchArgs &a)
 {
   typedef typename dev::Traits<P>::acc A;
-  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF, EXACT, GP>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out,
+  hipLaunchKernelGGL((k_dhop<P, DAG, C12, AXPBY, PF, EXACT, GP, TD>), a.grid, a.block, a.lds, a.st, a.gauge, a.nbr, a.in, a.out,
                      a.center, a.nslot, a.M, a.spw, a.mc, a.nwg_sites, a.cblk, a.nchunk, a.nwg, a.per_xcd, a.xcd, a.nt,
                      (A)a.alpha, (A)a.beta, a.ghost0);
 }
@@ -101,25 +101,25 @@ void launch_gp(bool dag, bool c12, int pf, const LaunchArgs &a)
 #undef GBX_GP
 }
 
-template <int P, bool DAG, bool C12>
+template <int P, bool DAG, bool C12, bool TD = false>
 void launch_dc(bool axpby, int pf, const LaunchArgs &a)
 {
   if (axpby) {
-    if (pf == 2) launch_t<P, DAG, C12, true, 2>(a);
-    else if (pf == 1) launch_t<P, DAG, C12, true, 1>(a);
-    else launch_t<P, DAG, C12, true, 0>(a);
-  } else if (pf == 2) launch_t<P, DAG, C12, false, 2>(a);
-  else if (pf == 1) launch_t<P, DAG, C12, false, 1>(a);
-  else launch_t<P, DAG, C12, false, 0>(a);
+    if (pf == 2) launch_t<P, DAG, C12, true, 2, false, false, TD>(a);
+    else if (pf == 1) launch_t<P, DAG, C12, true, 1, false, false, TD>(a);
+    else launch_t<P, DAG, C12, true, 0, false, false, TD>(a);
+  } else if (pf == 2) launch_t<P, DAG, C12, false, 2, false, false, TD>(a);
+  else if (pf == 1) launch_t<P, DAG, C12, false, 1, false, false, TD>(a);
+  else launch_t<P, DAG, C12, false, 0, false, false, TD>(a);
 }
 
-template <int P>
+template <int P, bool TD = false>
 void launch_p(bool dag, bool c12, bool axpby, int pf, const LaunchArgs &a)
 {
-  if (!dag && !c12) launch_dc<P, false, false>(axpby, pf, a);
-  else if (!dag && c12) launch_dc<P, false, true>(axpby, pf, a);
-  else if (dag && !c12) launch_dc<P, true, false>(axpby, pf, a);
-  else launch_dc<P, true, true>(axpby, pf, a);
+  if (!dag && !c12) launch_dc<P, false, false, TD>(axpby, pf, a);
+  else if (!dag && c12) launch_dc<P, false, true, TD>(axpby, pf, a);
+  else if (dag && !c12) launch_dc<P, true, false, TD>(axpby, pf, a);
+  else launch_dc<P, true, true, TD>(axpby, pf, a);
 }
 
 // The reference's rounding (FP64 / FP32 storage, 18-real links, no axpby).
@@ -158,6 +158,7 @@ std::string dhop_signature(const gbx_gauge_s *u, const gbx_spinor_s *in)
     s += buf;
   }
   if (g->nlevels == 0 && g->nghost == 0 && g->parity < 0) s += "_order";  // a caller-defined site order
+  if (in->basis == GBX_BASIS_TDIAG) s += "_td";
   for (int l = 0; l < g->nlevels; l++) {
     snprintf(buf, sizeof buf, "_b%dx%dx%dx%d", g->levels[l * 4], g->levels[l * 4 + 1], g->levels[l * 4 + 2],
              g->levels[l * 4 + 3]);
@@ -256,14 +257,24 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   const bool axpby = !(alpha == 0.0 && beta == 1.0);
   GBX_CHECK(!exact || ((u->prec == GBX_F64 || u->prec == GBX_F32) && !u->comp12 && !axpby), GBX_EUNSUPPORTED,
             "dhop: the reference-rounding kernel takes FP64 / FP32 fields with 18-real links");
+  const bool td = in->basis == GBX_BASIS_TDIAG;
+  GBX_CHECK(!td || (!exact && !c.projected_ghosts), GBX_EUNSUPPORTED,
+            "dhop: the reference-rounding and projected-ghost forms take chiral-basis fields only");
   if (axpby) {
     GBX_CHECK(center && center->d, GBX_ENULL, "dhop_axpby: NULL center field");
     GBX_CHECK(center->d != out->d, GBX_EALIAS, "dhop_axpby: out aliases the center field");
     GBX_CHECK(center->prec == in->prec && center->M == in->M && center->geom == out->geom, GBX_EMISMATCH,
               "dhop_axpby: the added field must have the output's geometry, precision and shape");
+    GBX_CHECK(center->basis == in->basis, GBX_EMISMATCH, "dhop_axpby: the added field is in another spin basis");
   }
   const uint64_t q0 = range.begin, q1 = std::min<uint64_t>(range.end, u->geom->vol);
   GBX_CHECK(q0 <= q1, GBX_EINVAL, "dhop: slot range %llu .. %llu", (unsigned long long)q0, (unsigned long long)q1);
+  // The output takes the input's basis when it is written whole; a part of
+  // it only when it is in that basis already.
+  const bool whole = q0 == 0 && q1 == u->geom->vol;
+  GBX_CHECK(whole || out->basis == in->basis, GBX_EMISMATCH,
+            "dhop: a slot range of an output in another spin basis than the input");
+  out->basis = in->basis;
   if (q0 == q1) return GBX_OK;
   LaunchArgs a;
   a.nslot = (uint32_t)(q1 - q0);
@@ -309,6 +320,12 @@ int dhop_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   if (exact) {
     if (u->prec == GBX_F64) launch_exact<GBX_F64>(dag, pf, a);
     else launch_exact<GBX_F32>(dag, pf, a);
+    GBX_HIP(hipGetLastError());
+    return GBX_OK;
+  }
+  if (td) {
+    if (u->prec == GBX_F64) launch_p<GBX_F64, true>(dag, u->comp12, axpby, pf, a);
+    else launch_p<GBX_F32, true>(dag, u->comp12, axpby, pf, a);
     GBX_HIP(hipGetLastError());
     return GBX_OK;
   }

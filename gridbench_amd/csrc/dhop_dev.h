@@ -286,13 +286,74 @@ __device__ __forceinline__ void leg_compute_gp(const A *v, bool g, const A *U, A
   recon<K, FIRST>(ur, ui, rr, ri);
 }
 
+// GBX_BASIS_TDIAG fields (psi' = S psi, S = [[1, 1], [1, -1]] / sqrt(2) on
+// the spin pairs (0, 2), (1, 3)): there gamma_x,y,z are the chiral ones
+// negated, so an x, y or z leg uses the projector kind of the opposite sign,
+// and gamma_t = diag(1, 1, -1, -1), so the t legs' projectors 1 +/- gamma_t
+// keep twice the upper (kind PTP) or lower (PTM) two spins: such a leg
+// gathers NCH/2 chunks, multiplies them by the link and adds twice the
+// product to that half of the result.
+template <bool DAG, int D, bool TD> struct LegKind {
+  static constexpr int value = LegProj<DAG, D>::value ^ ((TD && (D & 3) != 3) ? 4 : 0);
+};
+template <bool DAG, int D, bool TD> struct LegHalf {
+  static constexpr bool value = TD && (D & 3) == 3;              // gathers half a spinor
+  static constexpr bool upper = LegProj<DAG, D>::value == PTP;  // which half
+};
+
+// Chunks of one spin half (v[0..11] = its 2 spins x 3 colours).
+template <int P, bool UPPER>
+__device__ __forceinline__ void load_nb_half(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
+                                             typename Traits<P>::acc *v)
+{
+  constexpr int NCH = Traits<P>::NCH, R = Traits<P>::R, K0 = UPPER ? 0 : NCH / 2;
+  const uint64_t base = (uint64_t)nb * NCH * M + m;
+#pragma unroll
+  for (int k = 0; k < NCH / 2; k++)
+    load_chunk<P, typename Traits<P>::acc>(in, base + (uint64_t)(K0 + k) * M, v + k * R);
+}
+
+template <bool UPPER, int MUL, typename A>
+__device__ __forceinline__ void leg_half(const A *v, const A *U, A *rr, A *ri)
+{
+  A hr[6], hi[6], ur[6], ui[6];
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    hr[j] = v[2 * j];
+    hi[j] = v[2 * j + 1];
+  }
+  su3_mul<MUL>(U, hr, hi, ur, ui);
+  constexpr int o = UPPER ? 0 : 6;
+#pragma unroll
+  for (int j = 0; j < 6; j++) {
+    rr[o + j] = dfma((A)2, ur[j], rr[o + j]);
+    ri[o + j] = dfma((A)2, ui[j], ri[o + j]);
+  }
+}
+
+template <int P, bool DAG, int D, bool TD>
+__device__ __forceinline__ void leg_load(const void *__restrict__ in, uint32_t nb, uint32_t M, uint32_t m,
+                                         typename Traits<P>::acc *v)
+{
+  if constexpr (LegHalf<DAG, D, TD>::value) load_nb_half<P, LegHalf<DAG, D, TD>::upper>(in, nb, M, m, v);
+  else load_nb<P>(in, nb, M, m, v);
+}
+
+template <bool DAG, int D, bool TD, bool FIRST, int MUL, typename A>
+__device__ __forceinline__ void leg_apply(const A *v, const A *U, A *rr, A *ri)
+{
+  static_assert(!(FIRST && LegHalf<DAG, D, TD>::value), "a half leg never comes first");
+  if constexpr (LegHalf<DAG, D, TD>::value) leg_half<LegHalf<DAG, D, TD>::upper, MUL>(v, U, rr, ri);
+  else leg_compute<LegKind<DAG, D, TD>::value, FIRST, MUL>(v, U, rr, ri);
+}
+
 // Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
 // PF = 0: gather and compute leg by leg.  PF = 1: the gather of leg d+1 is
 // issued before leg d is computed (two gather buffers in registers), which
 // keeps a second set of loads in flight per wavefront.  PF = 2: three
 // buffers, gathers two legs ahead.  EXACT: the reference's rounding in the
-// SU(3) products (FP64 / FP32).
-template <int P, bool DAG, int PF, bool EXACT = false>
+// SU(3) products (FP64 / FP32).  TD: GBX_BASIS_TDIAG fields (LegKind, LegHalf).
+template <int P, bool DAG, int PF, bool EXACT = false, bool TD = false>
 __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uint32_t *nb, uint32_t M, uint32_t m,
                                           const typename Traits<P>::acc *U, typename Traits<P>::acc *rr,
                                           typename Traits<P>::acc *ri)
@@ -302,8 +363,8 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
   if (PF == 0) {
     A v[24];
 #define GBX_LEG(D, FIRST)                                                  \
-    load_nb<P>(in, nb[D], M, m, v);                                        \
-    leg_compute<LegProj<DAG, D>::value, FIRST, CH>(v, U + D * 18, rr, ri);
+    leg_load<P, DAG, D, TD>(in, nb[D], M, m, v);                           \
+    leg_apply<DAG, D, TD, FIRST, CH>(v, U + D * 18, rr, ri);
     GBX_LEG(0, true)
     GBX_LEG(1, false)
     GBX_LEG(2, false)
@@ -315,40 +376,40 @@ __device__ __forceinline__ void dhop_site(const void *__restrict__ in, const uin
 #undef GBX_LEG
   } else if (PF == 2) {
     A v0[24], v1[24], v2[24];
-    load_nb<P>(in, nb[0], M, m, v0);
-    load_nb<P>(in, nb[1], M, m, v1);
-    load_nb<P>(in, nb[2], M, m, v2);
-    leg_compute<LegProj<DAG, 0>::value, true, CH>(v0, U + 0 * 18, rr, ri);
-    load_nb<P>(in, nb[3], M, m, v0);
-    leg_compute<LegProj<DAG, 1>::value, false, CH>(v1, U + 1 * 18, rr, ri);
-    load_nb<P>(in, nb[4], M, m, v1);
-    leg_compute<LegProj<DAG, 2>::value, false, CH>(v2, U + 2 * 18, rr, ri);
-    load_nb<P>(in, nb[5], M, m, v2);
-    leg_compute<LegProj<DAG, 3>::value, false, CH>(v0, U + 3 * 18, rr, ri);
-    load_nb<P>(in, nb[6], M, m, v0);
-    leg_compute<LegProj<DAG, 4>::value, false, CH>(v1, U + 4 * 18, rr, ri);
-    load_nb<P>(in, nb[7], M, m, v1);
-    leg_compute<LegProj<DAG, 5>::value, false, CH>(v2, U + 5 * 18, rr, ri);
-    leg_compute<LegProj<DAG, 6>::value, false, CH>(v0, U + 6 * 18, rr, ri);
-    leg_compute<LegProj<DAG, 7>::value, false, CH>(v1, U + 7 * 18, rr, ri);
+    leg_load<P, DAG, 0, TD>(in, nb[0], M, m, v0);
+    leg_load<P, DAG, 1, TD>(in, nb[1], M, m, v1);
+    leg_load<P, DAG, 2, TD>(in, nb[2], M, m, v2);
+    leg_apply<DAG, 0, TD, true, CH>(v0, U + 0 * 18, rr, ri);
+    leg_load<P, DAG, 3, TD>(in, nb[3], M, m, v0);
+    leg_apply<DAG, 1, TD, false, CH>(v1, U + 1 * 18, rr, ri);
+    leg_load<P, DAG, 4, TD>(in, nb[4], M, m, v1);
+    leg_apply<DAG, 2, TD, false, CH>(v2, U + 2 * 18, rr, ri);
+    leg_load<P, DAG, 5, TD>(in, nb[5], M, m, v2);
+    leg_apply<DAG, 3, TD, false, CH>(v0, U + 3 * 18, rr, ri);
+    leg_load<P, DAG, 6, TD>(in, nb[6], M, m, v0);
+    leg_apply<DAG, 4, TD, false, CH>(v1, U + 4 * 18, rr, ri);
+    leg_load<P, DAG, 7, TD>(in, nb[7], M, m, v1);
+    leg_apply<DAG, 5, TD, false, CH>(v2, U + 5 * 18, rr, ri);
+    leg_apply<DAG, 6, TD, false, CH>(v0, U + 6 * 18, rr, ri);
+    leg_apply<DAG, 7, TD, false, CH>(v1, U + 7 * 18, rr, ri);
   } else {
     A v0[24], v1[24];
-    load_nb<P>(in, nb[0], M, m, v0);
-    load_nb<P>(in, nb[1], M, m, v1);
-    leg_compute<LegProj<DAG, 0>::value, true, CH>(v0, U + 0 * 18, rr, ri);
-    load_nb<P>(in, nb[2], M, m, v0);
-    leg_compute<LegProj<DAG, 1>::value, false, CH>(v1, U + 1 * 18, rr, ri);
-    load_nb<P>(in, nb[3], M, m, v1);
-    leg_compute<LegProj<DAG, 2>::value, false, CH>(v0, U + 2 * 18, rr, ri);
-    load_nb<P>(in, nb[4], M, m, v0);
-    leg_compute<LegProj<DAG, 3>::value, false, CH>(v1, U + 3 * 18, rr, ri);
-    load_nb<P>(in, nb[5], M, m, v1);
-    leg_compute<LegProj<DAG, 4>::value, false, CH>(v0, U + 4 * 18, rr, ri);
-    load_nb<P>(in, nb[6], M, m, v0);
-    leg_compute<LegProj<DAG, 5>::value, false, CH>(v1, U + 5 * 18, rr, ri);
-    load_nb<P>(in, nb[7], M, m, v1);
-    leg_compute<LegProj<DAG, 6>::value, false, CH>(v0, U + 6 * 18, rr, ri);
-    leg_compute<LegProj<DAG, 7>::value, false, CH>(v1, U + 7 * 18, rr, ri);
+    leg_load<P, DAG, 0, TD>(in, nb[0], M, m, v0);
+    leg_load<P, DAG, 1, TD>(in, nb[1], M, m, v1);
+    leg_apply<DAG, 0, TD, true, CH>(v0, U + 0 * 18, rr, ri);
+    leg_load<P, DAG, 2, TD>(in, nb[2], M, m, v0);
+    leg_apply<DAG, 1, TD, false, CH>(v1, U + 1 * 18, rr, ri);
+    leg_load<P, DAG, 3, TD>(in, nb[3], M, m, v1);
+    leg_apply<DAG, 2, TD, false, CH>(v0, U + 2 * 18, rr, ri);
+    leg_load<P, DAG, 4, TD>(in, nb[4], M, m, v0);
+    leg_apply<DAG, 3, TD, false, CH>(v1, U + 3 * 18, rr, ri);
+    leg_load<P, DAG, 5, TD>(in, nb[5], M, m, v1);
+    leg_apply<DAG, 4, TD, false, CH>(v0, U + 4 * 18, rr, ri);
+    leg_load<P, DAG, 6, TD>(in, nb[6], M, m, v0);
+    leg_apply<DAG, 5, TD, false, CH>(v1, U + 5 * 18, rr, ri);
+    leg_load<P, DAG, 7, TD>(in, nb[7], M, m, v1);
+    leg_apply<DAG, 6, TD, false, CH>(v0, U + 6 * 18, rr, ri);
+    leg_apply<DAG, 7, TD, false, CH>(v1, U + 7 * 18, rr, ri);
   }
 }
 

@@ -147,6 +147,7 @@ int dwf_s5_launch(gbx_spinor_s *out, const gbx_spinor_s *x, const gbx_spinor_s *
                   double alpha, double beta, hipStream_t st)
 {
   GBX_CHECK(out && x && out->d && x->d, GBX_ENULL, "dwf_mooee: NULL field");
+  GBX_TRY(require_chiral("dwf_mooee", x, z));
   GBX_TRY(check_params(p, x->Ls));
   GBX_CHECK(out->geom == x->geom && out->Ls == x->Ls && out->nrhs == x->nrhs && out->prec == x->prec, GBX_EMISMATCH,
             "dwf_mooee: out and in differ in geometry, shape or precision");
@@ -185,6 +186,7 @@ extern "C" int gbx_dwf_apply(gbx_gauge u, gbx_spinor out, gbx_spinor in, const g
   GBX_CHECK(u && out && in, GBX_ENULL, "gbx_dwf_apply: NULL argument");
   GBX_CHECK(whole_lattice(u->geom), GBX_EINVAL, "gbx_dwf_apply: whole lattices only (red-black halves: gbx_dwf_solve_eo)");
   GBX_TRY(check_params(p, in->Ls));
+  GBX_TRY(require_chiral("gbx_dwf_apply", in));
   GBX_TRY(gbx_dhop(u, out, in, dag));
   return dwf_s5_launch(out, in, out, p, 0, dag, 1.0, -0.5, u->geom->stream);
 }

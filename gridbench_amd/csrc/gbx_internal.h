@@ -136,7 +136,13 @@ struct gbx_spinor_s {
   uint32_t M;             // Ls * nrhs
   void *d = nullptr;
   size_t bytes = 0;
+  int basis = GBX_BASIS_CHIRAL;  // spin basis of the stored values
 };
+
+// GBX_EUNSUPPORTED unless every given field stores chiral-basis values
+// (entries that have no GBX_BASIS_TDIAG form).
+int require_chiral(const char *what, const gbx_spinor_s *a, const gbx_spinor_s *b = nullptr,
+                   const gbx_spinor_s *c = nullptr);
 
 void geom_retain(gbx_geom_s *g);
 void geom_release(gbx_geom_s *g);

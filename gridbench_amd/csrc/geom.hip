@@ -1097,16 +1097,32 @@ static int spinor_xfer_lex(gbx_spinor x, int r0, int nr, double *lex, bool up)
   return GBX_OK;
 }
 
+// Host arrays hold chiral-basis values: a whole-field upload leaves the
+// field in that basis, partial uploads and downloads need it there.
 extern "C" int gbx_spinor_upload(gbx_spinor x, const double *lex)
 {
-  return spinor_xfer_lex(x, 0, x ? x->nrhs : 1, (double *)lex, true);
+  GBX_TRY(spinor_xfer_lex(x, 0, x ? x->nrhs : 1, (double *)lex, true));
+  x->basis = GBX_BASIS_CHIRAL;
+  return GBX_OK;
 }
 
-extern "C" int gbx_spinor_download(gbx_spinor x, double *lex) { return spinor_xfer_lex(x, 0, x ? x->nrhs : 1, lex, false); }
+extern "C" int gbx_spinor_download(gbx_spinor x, double *lex)
+{
+  GBX_TRY(require_chiral("gbx_spinor_download", x));
+  return spinor_xfer_lex(x, 0, x ? x->nrhs : 1, lex, false);
+}
 
-extern "C" int gbx_spinor_upload_rhs(gbx_spinor x, int r, const double *lex) { return spinor_xfer_lex(x, r, 1, (double *)lex, true); }
+extern "C" int gbx_spinor_upload_rhs(gbx_spinor x, int r, const double *lex)
+{
+  GBX_TRY(require_chiral("gbx_spinor_upload_rhs", x));
+  return spinor_xfer_lex(x, r, 1, (double *)lex, true);
+}
 
-extern "C" int gbx_spinor_download_rhs(gbx_spinor x, int r, double *lex) { return spinor_xfer_lex(x, r, 1, lex, false); }
+extern "C" int gbx_spinor_download_rhs(gbx_spinor x, int r, double *lex)
+{
+  GBX_TRY(require_chiral("gbx_spinor_download_rhs", x));
+  return spinor_xfer_lex(x, r, 1, lex, false);
+}
 
 template <typename T> static int spinor_upload_ref(gbx_spinor x, const T *ref, int nsimd)
 {
@@ -1150,10 +1166,28 @@ template <typename T> static int spinor_download_ref(gbx_spinor x, T *ref, int n
   return GBX_OK;
 }
 
-extern "C" int gbx_spinor_upload_ref_f64(gbx_spinor x, const double *ref, int nsimd) { return spinor_upload_ref(x, ref, nsimd); }
-extern "C" int gbx_spinor_download_ref_f64(gbx_spinor x, double *ref, int nsimd) { return spinor_download_ref(x, ref, nsimd); }
-extern "C" int gbx_spinor_upload_ref_f32(gbx_spinor x, const float *ref, int nsimd) { return spinor_upload_ref(x, ref, nsimd); }
-extern "C" int gbx_spinor_download_ref_f32(gbx_spinor x, float *ref, int nsimd) { return spinor_download_ref(x, ref, nsimd); }
+extern "C" int gbx_spinor_upload_ref_f64(gbx_spinor x, const double *ref, int nsimd)
+{
+  GBX_TRY(spinor_upload_ref(x, ref, nsimd));
+  x->basis = GBX_BASIS_CHIRAL;
+  return GBX_OK;
+}
+extern "C" int gbx_spinor_download_ref_f64(gbx_spinor x, double *ref, int nsimd)
+{
+  GBX_TRY(require_chiral("gbx_spinor_download_ref_f64", x));
+  return spinor_download_ref(x, ref, nsimd);
+}
+extern "C" int gbx_spinor_upload_ref_f32(gbx_spinor x, const float *ref, int nsimd)
+{
+  GBX_TRY(spinor_upload_ref(x, ref, nsimd));
+  x->basis = GBX_BASIS_CHIRAL;
+  return GBX_OK;
+}
+extern "C" int gbx_spinor_download_ref_f32(gbx_spinor x, float *ref, int nsimd)
+{
+  GBX_TRY(require_chiral("gbx_spinor_download_ref_f32", x));
+  return spinor_download_ref(x, ref, nsimd);
+}
 
 extern "C" int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src)
 {
@@ -1163,13 +1197,19 @@ extern "C" int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src)
             "gbx_spinor_copy: different geometry or shape");
   gbx_geom_s *g = dst->geom;
   if (dst == src) return GBX_OK;
+  // dst takes src's spin basis, which 16-bit fields do not have but chiral
+  GBX_CHECK(src->basis == GBX_BASIS_CHIRAL || dst->prec == GBX_F64 || dst->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "gbx_spinor_copy: 16-bit fields hold chiral-basis values only");
   if (dst->prec == src->prec) {
     GBX_HIP(hipMemcpyAsync(dst->d, src->d, dst->bytes, hipMemcpyDeviceToDevice, g->stream));
+    dst->basis = src->basis;
     return GBX_OK;
   }
   // The chunked layouts of different precisions place reals differently:
   // convert point by point (all 24 reals of one (slot, m) per thread).
-  return spinor_convert_scaled(dst, src, 1.0, g->stream);
+  GBX_TRY(spinor_convert_scaled(dst, src, 1.0, g->stream));
+  dst->basis = src->basis;
+  return GBX_OK;
 }
 
 // Reference-layout arrays already in device memory <-> device objects,

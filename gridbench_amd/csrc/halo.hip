@@ -218,6 +218,7 @@ extern "C" int gbx_halo_ghost_proj(gbx_spinor x, int dim, int side, void **dptr)
   GBX_TRY(part_ok(x, dim, "gbx_halo_ghost_proj"));
   GBX_CHECK(side == 0 || side == 1, GBX_EINVAL, "gbx_halo_ghost_proj: side=%d (0 = minus, 1 = plus)", side);
   GBX_CHECK(x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED, "gbx_halo_ghost_proj: FP64 / FP32 fields");
+  GBX_TRY(require_chiral("gbx_halo_ghost_proj", x));
   const gbx_geom_s *g = x->geom;
   const uint64_t slot = slot_units(x) * 16;
   // half-slots of NCH/2 chunks after the vol whole slots, in the order of the whole-spinor ghosts
@@ -242,6 +243,7 @@ static int pack_impl(gbx_spinor_s *x, int dim, int face, int dag, bool proj, voi
   GBX_TRY(part_ok(x, dim, who));
   GBX_CHECK(face == 0 || face == 1, GBX_EINVAL, "%s: face=%d (0 = bottom, 1 = top)", who, face);
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
+  if (proj) GBX_TRY(require_chiral(who, x));
   GBX_CHECK(!proj || x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
             "%s: projected faces of FP64 / FP32 fields only (16-bit storage is projected in FP32 by the kernels: a "
             "projected FP32 face is no smaller than the full 16-bit one, use gbx_halo_pack)", who);
@@ -267,6 +269,7 @@ static int pack_impl(gbx_spinor_s *x, int dim, int face, int dag, bool proj, voi
 static int unpack_impl(gbx_spinor_s *x, int dim, int side, bool proj, const void *dbuf, hipStream_t st, const char *who)
 {
   GBX_CHECK(dbuf, GBX_ENULL, "%s: NULL buffer", who);
+  if (proj) GBX_TRY(require_chiral(who, x));
   void *ghost;
   GBX_TRY(gbx_halo_ghost(x, dim, side, &ghost));
   const gbx_geom_s *g = x->geom;
@@ -318,6 +321,7 @@ static int pack_all_impl(gbx_spinor_s *x, int dag, bool proj, void *const *bufs,
   GBX_CHECK(x && bufs, GBX_ENULL, "%s: NULL argument", who);
   GBX_CHECK(x->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "%s: dag=%d", who, dag);
+  if (proj) GBX_TRY(require_chiral(who, x));
   GBX_CHECK(!proj || x->prec == GBX_F64 || x->prec == GBX_F32, GBX_EUNSUPPORTED,
             "%s: projected faces of FP64 / FP32 fields only", who);
   const gbx_geom_s *g = x->geom;
@@ -362,6 +366,7 @@ static int unpack_all_impl(gbx_spinor_s *x, bool proj, const void *const *bufs, 
 {
   GBX_CHECK(x && bufs, GBX_ENULL, "%s: NULL argument", who);
   GBX_CHECK(x->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
+  if (proj) GBX_TRY(require_chiral(who, x));
   const gbx_geom_s *g = x->geom;
   if (!proj) {
     for (int d = 0; d < 4; d++)
@@ -454,6 +459,7 @@ extern "C" int gbx_halo_overlap_time(gbx_gauge u, gbx_spinor out, gbx_spinor in,
   GBX_CHECK(in->geom->nghost != 0, GBX_EINVAL, "%s: the field's geometry is not a domain part", who);
   GBX_CHECK(!proj || in->prec == GBX_F64 || in->prec == GBX_F32, GBX_EUNSUPPORTED,
             "%s: projected faces of FP64 / FP32 fields only", who);
+  if (proj) GBX_TRY(require_chiral(who, in));
   const gbx_geom_s *g = in->geom;
   std::vector<int> cut;
   for (int d = 0; d < 4; d++)

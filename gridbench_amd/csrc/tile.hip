@@ -265,6 +265,7 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
 {
   const gbx_geom_s *g = u->geom;
   GBX_CHECK(whole_lattice(g), GBX_EUNSUPPORTED, "tile kernel: whole lattices only");
+  GBX_TRY(require_chiral("tile kernel", in));
   const TileShape s = unpack_tile(k.tile);
   for (int d = 0; d < 4; d++)
     GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,
@@ -282,6 +283,7 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
 int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st)
 {
   GBX_TRY(tile_cfg_check(k, u, in));
+  out->basis = in->basis;
   gbx_geom_s *g = u->geom;
   const TileShape s = unpack_tile(k.tile);
   TileArgs a;

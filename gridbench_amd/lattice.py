@@ -353,6 +353,18 @@ class SpinorField:
         check(lib.gbx_spinor_fill_gaussian(self.h, int(seed)), "gbx_spinor_fill_gaussian")
         return self
 
+    def set_basis(self, basis):
+        """Convert the stored values to spin basis BASIS_CHIRAL or BASIS_TDIAG
+        (gamma_t diagonal: the VALU kernel's t legs gather half a spinor)."""
+        check(lib.gbx_spinor_set_basis(self.h, int(basis)), "gbx_spinor_set_basis")
+        return self
+
+    @property
+    def basis(self):
+        b = C.c_int()
+        check(lib.gbx_spinor_get_basis(self.h, C.byref(b)), "gbx_spinor_get_basis")
+        return b.value
+
     def norm2(self):
         n = C.c_double()
         check(lib.gbx_spinor_norm2(self.h, C.byref(n)), "gbx_spinor_norm2")
@@ -851,8 +863,11 @@ def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
     return 48 * bs + 8 * RU * bu / (Ls * nrhs)
 
 
+# Spin bases of stored spinor values (include/gbx.h).
+BASIS_CHIRAL, BASIS_TDIAG = 0, 1
+
 __all__ = [
-    "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
+    "BASIS_CHIRAL", "BASIS_TDIAG", "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
     "dhop", "dhop_axpby",

@@ -258,6 +258,26 @@ int gbx_spinor_copy(gbx_spinor dst, gbx_spinor src);
 /* the host; the values are not the host generator's bits.                  */
 int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 
+/* Spin basis of a field's stored values.  GBX_BASIS_CHIRAL (every field's */
+/* basis at creation) is the reference's (WilsonKernelsHandCpu.h:94-303).  */
+/* GBX_BASIS_TDIAG is psi' = S psi with S = [[1, 1], [1, -1]] / sqrt(2) on  */
+/* (upper, lower) spin pairs: there gamma_t = diag(1, 1, -1, -1) and        */
+/* gamma_x,y,z are the chiral ones negated, so (1 -/+ gamma_t) keep one     */
+/* spin half and the VALU kernel's +-t legs gather half of each spinor.     */
+/* Dhop is basis-covariant (out takes the input's basis); the result, set  */
+/* back to the chiral basis, agrees with the chiral kernel's to rounding.   */
+/* No counterpart in the reference (its kernel stores chiral spinors).      */
+/* TDIAG fields: FP64 / FP32; accepted by gbx_dhop*, gbx_tune* (VALU form), */
+/* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
+/* halo routines; every other entry returns GBX_EUNSUPPORTED for them      */
+/* (upload / download included: set the basis back first).                 */
+#define GBX_BASIS_CHIRAL 0
+#define GBX_BASIS_TDIAG 1
+/* Convert the stored values in place (S is its own inverse: a round trip   */
+/* returns the values to within a few ulp, not bit for bit).               */
+int gbx_spinor_set_basis(gbx_spinor x, int basis);
+int gbx_spinor_get_basis(gbx_spinor x, int *basis);
+
 /* ---------------------------------------------------------------------- */
 /* The hopping term on device-resident fields: out = Dhop in (dag = 0) or  */
 /* Dhop^dagger in (dag = 1).  Same operator as the reference kernel        */
