@@ -179,7 +179,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
     out = []
     c4 = {}  # the 24^4 x Ls12 x 16-RHS configuration: its own gauge field (seed 401), spinors seed 402
 
-    def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False, compare=False):
+    def variant(name, prec, nrhs, Ls=LS, dag=False, comp12=False, compare=False, tdiag=False):
         if only and name not in only:
             return
         g = geom
@@ -202,10 +202,14 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
             x64 = gbx.SpinorField(g, Ls, nrhs, "f64").copy_from(x)
             w = gbx.SpinorField(g, Ls, nrhs, "f64")
             gbx.dhop(Uref, w, x64, dag=dag)
+        if tdiag:  # the spin basis with diagonal gamma_t, set before any timing
+            x.set_basis(gbx.BASIS_TDIAG)
         y = gbx.SpinorField(g, Ls, nrhs, prec)
         cfg, _ = tune(U, y, x, dag=dag)
         us = gbx.dhop_time(U, y, x, dag=dag, nrep=100)
         gbx.dhop(U, y, x, dag=dag)
+        if tdiag:
+            y.set_basis(gbx.BASIS_CHIRAL)
         if Ls == LS:
             got = y.download_rhs(0)
             err = float(np.linalg.norm((got - want).ravel()) / np.linalg.norm(want.ravel()))
@@ -227,6 +231,7 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
                     forms[kind] = {"unsupported": str(e)}
         b = gbx.bytes_per_site(prec, prec, Ls, nrhs, comp12) * sites5 * nrhs
         v = {"name": name, "prec": prec, "nrhs": nrhs, "Ls": Ls, "dag": dag, "comp12": comp12,
+             "spin_basis": "tdiag" if tdiag else "chiral",
              "us_per_launch": round(us, 2), "gflops": round(1320.0 * sites5 * nrhs / (us * 1e-6) / 1e9, 1),
              "hbm_gbps_alg": round(b / (us * 1e-6) / 1e9, 1),
              "per_rhs_speedup_vs_f64_1rhs": round(us_f64 * (Ls / LS) / (us / nrhs), 3),
@@ -240,14 +245,16 @@ def variants_section(gbx, tune, geom, dbl, U64, psi, ref64, ref64_dag, us_f64, o
         if U is not Uref:
             U.close()
 
-    variant("f64_1rhs_dag", "f64", 1, dag=True)
-    variant("f64_1rhs_comp12", "f64", 1, comp12=True)
-    variant("f32_1rhs", "f32", 1)
-    variant("f32_16rhs", "f32", 16, compare=True)
+    variant("f64_1rhs_dag", "f64", 1, dag=True, tdiag=True)
+    variant("f64_1rhs_comp12", "f64", 1, comp12=True, tdiag=True)
+    variant("f32_1rhs", "f32", 1, tdiag=True)
+    variant("f32_1rhs_chiral", "f32", 1)
+    variant("f32_16rhs", "f32", 16, compare=True, tdiag=True)
+    variant("f32_16rhs_chiral", "f32", 16)
     variant("f16_16rhs", "f16", 16, compare=True)
     variant("bf16_16rhs", "bf16", 16, compare=True)
     variant("f16_16rhs_comp12", "f16", 16, comp12=True)
-    variant("f32_16rhs_ls12", "f32", 16, Ls=12, compare=True)
+    variant("f32_16rhs_ls12", "f32", 16, Ls=12, compare=True, tdiag=True)
     variant("bf16_16rhs_ls12", "bf16", 16, Ls=12, compare=True)
     return out
 
@@ -630,27 +637,44 @@ def main():
     U64 = gbx.GaugeField(geom, "f64").upload(dbl)
     x64 = gbx.SpinorField(geom, LS, 1, "f64").upload_rhs(0, psi)
     y64 = gbx.SpinorField(geom, LS, 1, "f64")
+    # The headline field: the same spinor stored in the spin basis with
+    # diagonal gamma_t (converted once, before the timed region), so that the
+    # kernel's +-t legs gather half a spinor.  Its result, set back to the
+    # chiral basis, is checked against the chiral kernel's below.
+    xh = gbx.SpinorField(geom, LS, 1, "f64").copy_from(x64).set_basis(gbx.BASIS_TDIAG)
+    yh = gbx.SpinorField(geom, LS, 1, "f64")
 
-    best, best_us = tune(U64, y64, x64)
+    best, best_us = tune(U64, yh, xh)
     log(f"FP64 1-RHS config: {best.as_dict()} {best_us:.1f} us")
 
     # ------------------------------------------------ timed headline region
     for _ in range(args.warmup):
-        gbx.dhop(U64, y64, x64)
+        gbx.dhop(U64, yh, xh)
     geom.sync()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        gbx.dhop(U64, y64, x64)
+        gbx.dhop(U64, yh, xh)
     geom.sync()
     t1 = time.perf_counter()
     ms = (t1 - t0) * 1e3 / args.steps
     gflops = 1320.0 * sites5 / (ms * 1e-3) / 1e9
     # kernel duration from HIP events on the kernel's own stream
-    us_ev = gbx.dhop_time(U64, y64, x64, nrep=args.steps)
+    us_ev = gbx.dhop_time(U64, yh, xh, nrep=args.steps)
     bytes_alg = gbx.bytes_per_site("f64", "f64", LS, 1) * sites5
     achieved = bytes_alg / (us_ev * 1e-6) / 1e9
     log(f"FP64 1-RHS: {ms:.3f} ms/step wall, {us_ev:.1f} us/launch events, {gflops:.1f} GFLOP/s, {achieved:.0f} GB/s")
+    gbx.dhop(U64, yh, xh)
+    yh.set_basis(gbx.BASIS_CHIRAL)
+    got_h = yh.download_rhs(0)
+    for f in (xh, yh):
+        f.close()
+    # the same application on the chiral-basis field: reference for every later section
+    best_chiral, _ = tune(U64, y64, x64)
+    us_chiral = gbx.dhop_time(U64, y64, x64, nrep=min(args.steps, 200))
+    gbx.dhop(U64, y64, x64)
     ref64 = y64.download_rhs(0)
+    err_h = float(np.linalg.norm((got_h - ref64).ravel()) / np.linalg.norm(ref64.ravel()))
+    log(f"FP64 1-RHS chiral-basis field: {us_chiral:.1f} us/launch; headline result vs it: rel L2 {err_h:.2e}")
 
     stream_gbps = gbx.stream_copy_bw(4 << 30, 20)
     log(f"STREAM copy: {stream_gbps:.0f} GB/s")
@@ -698,7 +722,11 @@ def main():
         "dtype": "f64",
         "data": "synthetic (seeded SU(3) links, Gaussian spinor; gauge seed 201, spinor seed 202)",
         "config": {"workload": "DWF Dhop (Hand DhopSite, non-dagger) 24^4 x Ls16 FP64 1-RHS", "dims": list(DIMS),
-                   "Ls": LS, "nrhs": 1, "flop_per_5d_site": 1320, "kernel_cfg": best.as_dict()},
+                   "Ls": LS, "nrhs": 1, "flop_per_5d_site": 1320, "kernel_cfg": best.as_dict(),
+                   "spin_basis": "tdiag (gamma_t diagonal; converted before the timed region)"},
+        "chiral_basis": {"us_per_launch_events": round(us_chiral, 2), "kernel_cfg": best_chiral.as_dict(),
+                         "gflops": round(1320.0 * sites5 / (us_chiral * 1e-6) / 1e9, 1),
+                         "headline_rel_l2_vs_chiral_kernel": float(f"{err_h:.3e}")},
         "hbm_gbps_alg": round(achieved, 1),
         "achievable_hbm_gbps_stream_copy": round(stream_gbps, 1),
         "frac_of_achievable_hbm": round(achieved / stream_gbps, 3),

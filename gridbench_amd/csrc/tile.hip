@@ -7,7 +7,8 @@
 // indices.  It loads the tile's own spinors once into LDS; a leg whose
 // neighbour lies inside the box reads LDS, only legs that leave the box
 // gather from L2.  For a 2^4 tile half the legs stay inside, so each spinor
-// line is requested 1 + 4 times instead of 8.
+// line is requested 1 + 4 times instead of 8.  GBX_BASIS_TDIAG fields: the
+// t legs read half of each spinor, from LDS or memory (dev::LegHalf).
 //
 // Tiles are visited in the geometry's slot order (sorted by their smallest
 // slot), so consecutive workgroups -- and, with xcd = 1, each XCD's share --
@@ -94,6 +95,18 @@ int tile_table(gbx_geom_s *g, int packed, const uint32_t **tab, uint32_t *ntiles
   return GBX_OK;
 }
 
+// A leg's chunks from the LDS copy of the box: all NCH, or the spin half a
+// GBX_BASIS_TDIAG t leg reads (dev::LegHalf).
+template <int P, bool DAG, int D, bool TD, typename A, typename V>
+__device__ __forceinline__ void lds_leg_load(const V *p, A *v)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  constexpr bool half = dev::LegHalf<DAG, D, TD>::value;
+  constexpr int K0 = (half && !dev::LegHalf<DAG, D, TD>::upper) ? NCH / 2 : 0, N = half ? NCH / 2 : NCH;
+#pragma unroll
+  for (int k = 0; k < N; k++) dev::load_chunk<P, A>(p, K0 + k, v + k * R);
+}
+
 struct TileArgs {
   const void *gauge;
   const uint32_t *nbr, *tslots;
@@ -104,7 +117,7 @@ struct TileArgs {
   int xcd, nt;
 };
 
-template <int P, bool DAG, bool C12>
+template <int P, bool DAG, bool C12, bool TD = false>
 __global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArgs a)
 {
   typedef typename dev::Traits<P>::acc A;
@@ -212,12 +225,11 @@ __global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArg
     {                                                                                                                  \
       const int ln = sL[i * 8 + (D)];                                                                                  \
       if (ln >= 0) {                                                                                                   \
-        const V *p = sIn + ((uint32_t)ln * a.mc + ml) * SW;                                                            \
-        _Pragma("unroll") for (int k = 0; k < NCH; k++) dev::load_chunk<P, A>(p, k, v + k * dev::Traits<P>::R);        \
-        dev::leg_compute<dev::LegProj<DAG, D>::value, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                    \
+        lds_leg_load<P, DAG, D, TD>(sIn + ((uint32_t)ln * a.mc + ml) * SW, v);                                         \
+        dev::leg_apply<DAG, D, TD, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                                       \
       } else {                                                                                                         \
-        dev::load_nb<P>(a.in, sN[i * 8 + D], a.M, m, v);                                                               \
-        dev::leg_compute<dev::LegProj<DAG, D>::value, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                    \
+        dev::leg_load<P, DAG, D, TD>(a.in, sN[i * 8 + D], a.M, m, v);                                                  \
+        dev::leg_apply<DAG, D, TD, FIRST, CH>(v, sU + (i * 8 + D) * 18, rr, ri);                                       \
       }                                                                                                                \
     }
     GBX_TLEG(0, true)
@@ -233,13 +245,13 @@ __global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArg
   }
 }
 
-template <int P>
+template <int P, bool TD = false>
 void launch_p(bool dag, bool c12, dim3 grid, dim3 block, size_t lds, hipStream_t st, const TileArgs &a)
 {
-  if (!dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, false, false>), grid, block, lds, st, a);
-  else if (!dag && c12) hipLaunchKernelGGL((k_dhop_tile<P, false, true>), grid, block, lds, st, a);
-  else if (dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, true, false>), grid, block, lds, st, a);
-  else hipLaunchKernelGGL((k_dhop_tile<P, true, true>), grid, block, lds, st, a);
+  if (!dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, false, false, TD>), grid, block, lds, st, a);
+  else if (!dag && c12) hipLaunchKernelGGL((k_dhop_tile<P, false, true, TD>), grid, block, lds, st, a);
+  else if (dag && !c12) hipLaunchKernelGGL((k_dhop_tile<P, true, false, TD>), grid, block, lds, st, a);
+  else hipLaunchKernelGGL((k_dhop_tile<P, true, true, TD>), grid, block, lds, st, a);
 }
 
 size_t tile_lds(int prec, int T, int mc)
@@ -265,7 +277,6 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
 {
   const gbx_geom_s *g = u->geom;
   GBX_CHECK(whole_lattice(g), GBX_EUNSUPPORTED, "tile kernel: whole lattices only");
-  GBX_TRY(require_chiral("tile kernel", in));
   const TileShape s = unpack_tile(k.tile);
   for (int d = 0; d < 4; d++)
     GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,
@@ -313,6 +324,12 @@ int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
     if (pad > lds) lds = pad;
   }
   const dim3 grid(k.xcd ? a.per_xcd * 8 : a.nwg), block((unsigned)k.threads);
+  if (in->basis == GBX_BASIS_TDIAG) {  // FP64 / FP32 only (gbx_spinor_set_basis)
+    if (u->prec == GBX_F64) launch_p<GBX_F64, true>(dag, u->comp12, grid, block, lds, st, a);
+    else launch_p<GBX_F32, true>(dag, u->comp12, grid, block, lds, st, a);
+    GBX_HIP(hipGetLastError());
+    return GBX_OK;
+  }
   switch (u->prec) {
   case GBX_F64: launch_p<GBX_F64>(dag, u->comp12, grid, block, lds, st, a); break;
   case GBX_F32: launch_p<GBX_F32>(dag, u->comp12, grid, block, lds, st, a); break;

@@ -267,7 +267,8 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* Dhop is basis-covariant (out takes the input's basis); the result, set  */
 /* back to the chiral basis, agrees with the chiral kernel's to rounding.   */
 /* No counterpart in the reference (its kernel stores chiral spinors).      */
-/* TDIAG fields: FP64 / FP32; accepted by gbx_dhop*, gbx_tune* (VALU form), */
+/* TDIAG fields: FP64 / FP32; accepted by gbx_dhop* and gbx_tune* (VALU   */
+/* and LDS-tiled forms),                                                   */
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
 /* halo routines; every other entry returns GBX_EUNSUPPORTED for them      */
 /* (upload / download included: set the basis back first).                 */

@@ -131,16 +131,34 @@ def test_domain_parts_with_whole_faces(gpu, gbx, problem, dims, grid):
     assert rel_l2(got[0], p.oracle_lex()) <= 1e-14
 
 
-def test_tuner_searches_the_valu_form(gpu, gbx, problem):
+@pytest.mark.parametrize("prec", ["f64", "f32"])
+@pytest.mark.parametrize("dag", [False, True])
+@pytest.mark.parametrize("tile,mc", [((2, 2, 2, 2), 0), ((4, 2, 2, 2), 4), ((3, 3, 3, 1), 2), ((2, 2, 2, 4), 0)])
+def test_tiled_kernel_bitexact_vs_valu(gpu, gbx, problem, prec, dag, tile, mc):
+    """The tiled kernel's t legs read half a spinor from LDS or memory; same
+    arithmetic in the same order as the VALU kernel on TDIAG fields."""
+    p = problem((12, 6, 6, 4), 4, nrhs=2)
+    geom, U, x, y = fields(gbx, p, prec)
+    x.set_basis(gbx.BASIS_TDIAG)
+    gbx.dhop(U, y, x, dag=dag, cfg=gbx.kcfg(0, 256, 1, 1, kind=2, mc=mc, tile=tile))
+    got = chiral_download(gbx, y)
+    gbx.dhop(U, y, x, dag=dag, cfg=gbx.kcfg(1, 64, 0, 0))
+    assert np.array_equal(got, chiral_download(gbx, y))
+    assert rel_l2(got[1], p.oracle_lex(dag=dag, rhs=1)) <= TOL[prec]
+
+
+def test_tuner_on_tdiag_fields(gpu, gbx, problem):
     p = problem((4, 4, 4, 8), 4, nrhs=2)
     geom, U, x, y = fields(gbx, p, "f32")
     x.set_basis(gbx.BASIS_TDIAG)
     best, us = gbx.tune(U, y, x)
-    assert best.kind == 0 and us > 0
+    assert best.kind in (0, 2) and us > 0
     gbx.dhop(U, y, x)
     assert rel_l2(chiral_download(gbx, y)[1], p.oracle_lex(rhs=1)) <= 1e-6
+    best, us = gbx.tune(U, y, x, kinds="tile")
+    assert best.kind == 2
     with pytest.raises(gbx.GbxError) as e:
-        gbx.tune(U, y, x.set_basis(gbx.BASIS_TDIAG), kinds="tile")
+        gbx.tune(U, y, x, kinds="mfma")  # the matrix-core form takes 16-bit (hence chiral) fields
     assert e.value.status == EUNSUPPORTED
 
 
@@ -158,7 +176,6 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     assert status(lambda: x.download()) == EUNSUPPORTED
     assert status(lambda: x.download_rhs(0)) == EUNSUPPORTED
     assert status(lambda: x.upload_rhs(0, host[0])) == EUNSUPPORTED
-    assert status(lambda: gbx.dhop(U, y, x, cfg=gbx.kcfg(0, 256, 1, 1, kind=2, tile=(2, 2, 2, 2)))) == EUNSUPPORTED
     assert status(lambda: gbx.dwf_mooee(y, x)) == EUNSUPPORTED
     assert status(lambda: gbx.dwf_apply(U, y, x)) == EUNSUPPORTED
     assert status(lambda: gbx.cg_solve(U, None, y, x, 0.12)) == EUNSUPPORTED
