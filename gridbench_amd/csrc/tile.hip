@@ -95,6 +95,11 @@ int tile_table(gbx_geom_s *g, int packed, const uint32_t **tab, uint32_t *ntiles
   return GBX_OK;
 }
 
+// Launch bound: FP64 holds 24 accumulation doubles per item, and the
+// TDIAG form of FP32 needs more than the 128 VGPRs that 1024-thread
+// workgroups leave (at 1024 it spills).
+constexpr int tile_max_threads(int prec, bool td) { return prec == GBX_F64 ? 512 : td ? 768 : 1024; }
+
 // A leg's chunks from the LDS copy of the box: all NCH, or the spin half a
 // GBX_BASIS_TDIAG t leg reads (dev::LegHalf).
 template <int P, bool DAG, int D, bool TD, typename A, typename V>
@@ -118,7 +123,7 @@ struct TileArgs {
 };
 
 template <int P, bool DAG, bool C12, bool TD = false>
-__global__ void __launch_bounds__(P == GBX_F64 ? 512 : 1024) k_dhop_tile(TileArgs a)
+__global__ void __launch_bounds__(tile_max_threads(P, TD)) k_dhop_tile(TileArgs a)
 {
   typedef typename dev::Traits<P>::acc A;
   typedef typename dev::Traits<P>::vec V;
@@ -282,7 +287,7 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
     GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,
               s.e[d], g->dims[d]);
   const uint32_t mc = (k.mc <= 0 || (uint32_t)k.mc > in->M) ? in->M : (uint32_t)k.mc;
-  const int tmax = u->prec == GBX_F64 ? 512 : 1024;  // the kernel's launch bound
+  const int tmax = tile_max_threads(u->prec, in->basis == GBX_BASIS_TDIAG);  // the kernel's launch bound
   GBX_CHECK(k.threads >= 64 && k.threads <= tmax && k.threads % 64 == 0, GBX_EINVAL, "tile kernel: threads=%d (max %d)",
             k.threads, tmax);
   GBX_CHECK(k.wgpc >= 0 && k.wgpc <= 64, GBX_EINVAL, "tile kernel: wgpc=%d", k.wgpc);
