@@ -525,7 +525,9 @@ int gbx_dwf_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor 
 /* ---------------------------------------------------------------------- */
 /* Batched application over many independent lattices (each its own      */
 /* geometry, gauge field, Ls and nrhs; FP32/FP16/BF16 storage) in one      */
-/* persistent launch captured in a HIP graph.                              */
+/* persistent launch captured in a HIP graph.  The fields are read as      */
+/* chiral-basis values: creation refuses TDIAG fields, and they must not   */
+/* be set to that basis while the batch is in use.                         */
 /* ---------------------------------------------------------------------- */
 typedef struct gbx_batch_s *gbx_batch;
 int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag,
