@@ -352,7 +352,11 @@ int solve(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, gbx_spinor_s *x, con
           gbx_cg_stats *stats)
 {
   GBX_TRY(check_args(u_hi, u_lo, x, b, prm));
-  GBX_TRY(require_chiral("cg", x, b));
+  // TDIAG fields: the solve runs in that basis (S is orthogonal, so norms
+  // and residuals are the chiral ones); FP64 or FP32 inner solves only.
+  GBX_CHECK(x->basis == b->basis, GBX_EMISMATCH, "cg: x and b are in different spin bases");
+  GBX_CHECK(b->basis == GBX_BASIS_CHIRAL || !u_lo || u_lo->prec == GBX_F32, GBX_EUNSUPPORTED,
+            "cg: TDIAG fields need an FP64 or FP32 inner gauge field");
   const auto t0 = std::chrono::steady_clock::now();
   gbx_cg_stats st{};
   gbx_geom_s *g = u_hi->geom;
@@ -371,6 +375,7 @@ int solve(const gbx_gauge_s *u_hi, const gbx_gauge_s *u_lo, gbx_spinor_s *x, con
   GBX_TRY(make(g, Ls, plo, p));
   GBX_TRY(make(g, Ls, plo, Ap));
   GBX_TRY(make(g, Ls, plo, tl));
+  for (Tmp *f : {&r, &t, &y, &e, &ri, &p, &Ap, &tl}) f->s->basis = b->basis;
   const gbx_gauge_s *uin = u_lo ? u_lo : u_hi;
   const NormalOp op_hi{false, u_hi, nullptr, prm->kappa, t.s, nullptr};
   const NormalOp op_lo{false, uin, nullptr, prm->kappa, tl.s, nullptr};

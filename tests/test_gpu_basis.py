@@ -178,7 +178,14 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     assert status(lambda: x.upload_rhs(0, host[0])) == EUNSUPPORTED
     assert status(lambda: gbx.dwf_mooee(y, x)) == EUNSUPPORTED
     assert status(lambda: gbx.dwf_apply(U, y, x)) == EUNSUPPORTED
-    assert status(lambda: gbx.cg_solve(U, None, y, x, 0.12)) == EUNSUPPORTED
+    assert status(lambda: gbx.cg_solve(U, None, y, x, 0.12)) == EMISMATCH  # x chiral, b TDIAG
+    ge, go = gbx.Geometry.rb_pair(p.dims)
+    Ue, Uo = gbx.GaugeField(ge, "f64").upload(p.dbl), gbx.GaugeField(go, "f64").upload(p.dbl)
+    even, odd = gbx.rb_split(p.dims, p.psi)
+    be = gbx.SpinorField(ge, p.Ls, 1, "f64").upload(even).set_basis(gbx.BASIS_TDIAG)
+    bo = gbx.SpinorField(go, p.Ls, 1, "f64").upload(odd).set_basis(gbx.BASIS_TDIAG)
+    xe, xo = gbx.SpinorField(ge, p.Ls, 1, "f64"), gbx.SpinorField(go, p.Ls, 1, "f64")
+    assert status(lambda: gbx.cg_solve_eo((Ue, Uo), None, (xe, xo), (be, bo), 0.12)) == EUNSUPPORTED
     assert status(lambda: gbx.Batch([U], [y], [x])) == EUNSUPPORTED
     assert status(lambda: y.axpy(1.0, x)) == EMISMATCH  # y chiral, x TDIAG
     assert status(lambda: x.set_basis(7)) == EINVAL
@@ -200,3 +207,31 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     xp.set_basis(gbx.BASIS_TDIAG)
     buf = gbx.DeviceBuffer(xp.halo_bytes(3, proj=True))
     assert status(lambda: xp.halo_pack(3, 1, buf.ptr, proj=True)) == EUNSUPPORTED
+
+
+def test_cg_on_tdiag_fields(gpu, gbx, problem):
+    """Defect-correction CG (FP64 outer, FP32 inner, and FP64 alone) on
+    TDIAG fields: the solution, set back to the chiral basis, is the chiral
+    solve's; 16-bit inner fields cannot hold the basis."""
+    p = problem((4, 4, 4, 8), 4)
+    geom = gbx.Geometry(p.dims)
+    U64 = gbx.GaugeField(geom, "f64").upload(p.dbl)
+    U32 = gbx.GaugeField(geom, "f32").upload(p.dbl)
+    U16 = gbx.GaugeField(geom, "f16").upload(p.dbl)
+    b = gbx.SpinorField(geom, p.Ls, 1, "f64").upload(p.psi)
+    x = gbx.SpinorField(geom, p.Ls, 1, "f64")
+    x.zero()
+    st = gbx.cg_solve(U64, U32, x, b, 0.12, tol=1e-10)
+    want = x.download()
+    b.set_basis(gbx.BASIS_TDIAG)
+    for lo in (U32, None):
+        x.zero()
+        x.set_basis(gbx.BASIS_TDIAG)
+        st = gbx.cg_solve(U64, lo, x, b, 0.12, tol=1e-10)
+        assert st["true_resid"] <= 1e-10
+        assert x.basis == gbx.BASIS_TDIAG
+        assert rel_l2(chiral_download(gbx, x), want) <= 1e-8
+    x.set_basis(gbx.BASIS_TDIAG)
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.cg_solve(U64, U16, x, b, 0.12)
+    assert e.value.status == EUNSUPPORTED
