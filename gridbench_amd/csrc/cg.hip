@@ -427,8 +427,9 @@ int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_
   GBX_CHECK(ge->parity == 0 && go->parity == 1 && ge->pair_id == go->pair_id, GBX_EMISMATCH,
             "%s: u_hi must be the even then the odd half of one red-black pair", who);
   GBX_CHECK(ge->nghost == 0, GBX_EINVAL, "%s: the solvers take a whole lattice's halves, not a domain part's", who);
-  GBX_TRY(require_chiral(who, x[0], x[1]));
-  GBX_TRY(require_chiral(who, b[0], b[1]));
+  const int basis = b[0]->basis;
+  GBX_CHECK(x[0]->basis == basis && x[1]->basis == basis && b[1]->basis == basis, GBX_EMISMATCH,
+            "%s: x and b are in different spin bases", who);
   for (int p = 0; p < 2; p++) {
     const gbx_geom_s *g = p ? go : ge;
     GBX_CHECK(u_hi[p]->prec == GBX_F64, GBX_EINVAL, "%s: the outer gauge fields must be FP64", who);
@@ -445,6 +446,10 @@ int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_
                 "%s: inner gauge fields must share one of FP32, FP16, BF16", who);
     }
   }
+  // TDIAG fields: the Wilson operator only (the DWF terms' chirality
+  // projectors are written for chiral values), FP64 or FP32 inner solves.
+  GBX_CHECK(basis == GBX_BASIS_CHIRAL || (kind == OP_WILSON && (!u_lo || u_lo[0]->prec == GBX_F32)), GBX_EUNSUPPORTED,
+            "%s: TDIAG fields: the Wilson operator with an FP64 or FP32 inner solve only", who);
   if (kind == OP_DWF) {
     GBX_CHECK(prm->tol > 0 && prm->inner_tol > 0 && prm->inner_tol < 1 && prm->max_outer >= 1 && prm->max_inner >= 1,
               GBX_EINVAL, "%s: bad tolerances or iteration limits", who);
@@ -476,6 +481,7 @@ int solve_schur(int kind, const gbx_dwf_params *dwf, const gbx_gauge_s *const u_
   GBX_TRY(make(g, Ls, plo, tl));
   GBX_TRY(make(g, Ls, plo, tel));
   GBX_TRY(make(g1, Ls, plo, tol_));
+  for (Tmp *f : {&bp, &r, &y, &t, &te, &to, &ro, &e, &ri, &p, &Ap, &tl, &tel, &tol_}) f->s->basis = basis;
   NormalOp op_hi{true, u_hi[0], u_hi[1], kappa, t.s, to.s};
   NormalOp op_lo = u_lo ? NormalOp{true, u_lo[0], u_lo[1], kappa, tl.s, tol_.s} : NormalOp{true, u_hi[0], u_hi[1], kappa, tl.s,
                                                                                            tol_.s};

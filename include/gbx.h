@@ -268,7 +268,8 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* back to the chiral basis, agrees with the chiral kernel's to rounding.   */
 /* No counterpart in the reference (its kernel stores chiral spinors).      */
 /* TDIAG fields: FP64 / FP32; accepted by gbx_dhop* and gbx_tune* (VALU   */
-/* and LDS-tiled forms), gbx_cg_solve (FP32 inner) and gbx_cg_solve_f64,   */
+/* and LDS-tiled forms), gbx_cg_solve (FP32 inner), gbx_cg_solve_f64 and   */
+/* gbx_cg_solve_eo (FP32 inner or none),                                    */
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
 /* halo routines; every other entry returns GBX_EUNSUPPORTED for them      */
 /* (upload / download included: set the basis back first).                 */

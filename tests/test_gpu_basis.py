@@ -185,7 +185,10 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     be = gbx.SpinorField(ge, p.Ls, 1, "f64").upload(even).set_basis(gbx.BASIS_TDIAG)
     bo = gbx.SpinorField(go, p.Ls, 1, "f64").upload(odd).set_basis(gbx.BASIS_TDIAG)
     xe, xo = gbx.SpinorField(ge, p.Ls, 1, "f64"), gbx.SpinorField(go, p.Ls, 1, "f64")
-    assert status(lambda: gbx.cg_solve_eo((Ue, Uo), None, (xe, xo), (be, bo), 0.12)) == EUNSUPPORTED
+    assert status(lambda: gbx.cg_solve_eo((Ue, Uo), None, (xe, xo), (be, bo), 0.12)) == EMISMATCH  # x chiral
+    xe.set_basis(gbx.BASIS_TDIAG)
+    xo.set_basis(gbx.BASIS_TDIAG)
+    assert status(lambda: gbx.dwf_solve_eo((Ue, Uo), None, (xe, xo), (be, bo))) == EUNSUPPORTED
     assert status(lambda: gbx.Batch([U], [y], [x])) == EUNSUPPORTED
     assert status(lambda: y.axpy(1.0, x)) == EMISMATCH  # y chiral, x TDIAG
     assert status(lambda: x.set_basis(7)) == EINVAL
@@ -235,3 +238,30 @@ def test_cg_on_tdiag_fields(gpu, gbx, problem):
     with pytest.raises(gbx.GbxError) as e:
         gbx.cg_solve(U64, U16, x, b, 0.12)
     assert e.value.status == EUNSUPPORTED
+
+
+def test_even_odd_wilson_solve_on_tdiag_fields(gpu, gbx, problem):
+    """Schur-complement solve of M x = b (Wilson) on TDIAG halves, FP32 inner
+    and FP64 alone: the solution, set back to the chiral basis, is the
+    chiral solve's."""
+    p = problem((4, 4, 4, 8), 4)
+    ge, go = gbx.Geometry.rb_pair(p.dims)
+    U = [gbx.GaugeField(g, "f64").upload(p.dbl) for g in (ge, go)]
+    U32 = [gbx.GaugeField(g, "f32").upload(p.dbl) for g in (ge, go)]
+    even, odd = gbx.rb_split(p.dims, p.psi)
+    b = [gbx.SpinorField(ge, p.Ls, 1, "f64").upload(even), gbx.SpinorField(go, p.Ls, 1, "f64").upload(odd)]
+    x = [gbx.SpinorField(g, p.Ls, 1, "f64") for g in (ge, go)]
+    for f in x:
+        f.zero()
+    gbx.cg_solve_eo(U, U32, x, b, 0.12, tol=1e-10)
+    want = [f.download() for f in x]
+    for f in b:
+        f.set_basis(gbx.BASIS_TDIAG)
+    for lo in (U32, None):
+        for f in x:
+            f.zero()
+            f.set_basis(gbx.BASIS_TDIAG)
+        st = gbx.cg_solve_eo(U, lo, x, b, 0.12, tol=1e-10)
+        assert st["true_resid"] <= 1e-10
+        for f, w in zip(x, want):
+            assert rel_l2(chiral_download(gbx, f), w) <= 1e-8
