@@ -271,7 +271,9 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* and LDS-tiled forms), gbx_cg_solve (FP32 inner), gbx_cg_solve_f64 and   */
 /* gbx_cg_solve_eo (FP32 inner or none),                                    */
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
-/* halo routines; every other entry returns GBX_EUNSUPPORTED for them      */
+/* halo routines (a face is copied as stored: sending and receiving parts  */
+/* must be in one basis); every other entry returns GBX_EUNSUPPORTED for   */
+/* them                                                                     */
 /* (upload / download included: set the basis back first).                 */
 #define GBX_BASIS_CHIRAL 0
 #define GBX_BASIS_TDIAG 1
