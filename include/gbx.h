@@ -273,8 +273,9 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
 /* halo routines (a face is copied as stored: sending and receiving parts  */
 /* must be in one basis); every other entry returns GBX_EUNSUPPORTED for   */
-/* them                                                                     */
-/* (upload / download included: set the basis back first).                 */
+/* them, downloads and one-RHS uploads included (set the basis back        */
+/* first).  A whole upload or gbx_spinor_fill_gaussian writes chiral       */
+/* values and leaves the field in GBX_BASIS_CHIRAL.                        */
 #define GBX_BASIS_CHIRAL 0
 #define GBX_BASIS_TDIAG 1
 /* Convert the stored values in place (S is its own inverse: a round trip   */
