@@ -31,7 +31,7 @@ constexpr int kThreads = 256;
 
 __global__ void k_reset(uint32_t *c) { c[0] = 0u; }
 
-template <int P, bool DAG, int PF>
+template <int P, bool DAG, int PF, bool TD = false>
 __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
                                                          uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw,
                                                          int nt)
@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
     for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
       const uint32_t sl = w / M, m = w - sl * M;
       A rr[12], ri[12];
-      dev::dhop_site<P, DAG, PF>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+      dev::dhop_site<P, DAG, PF, false, TD>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
       dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
     }
     __syncthreads();  // LDS is reused by the next item; s_next is written
@@ -68,6 +68,7 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
 
 struct gbx_batch_s {
   int prec = 0, dag = 0, n = 0;
+  bool td = false;  // every field in the GBX_BASIS_TDIAG spin basis
   uint32_t nitems = 0, max_spw = 1;
   int grid = 0;
   hipStream_t stream = nullptr;
@@ -99,6 +100,17 @@ void launch(gbx_batch_s *b, hipStream_t st, int nt)
 {
   const size_t lds = lds_for(P, b->max_spw);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, b->d_counter);
+  if constexpr (P == GBX_F64 || P == GBX_F32) {  // the precisions a TDIAG field can have
+    if (b->td) {
+      if (b->dag)
+        hipLaunchKernelGGL((k_dhop_batch<P, true, 1, true>), dim3(b->grid), dim3(kThreads), lds, st,
+                           (const LatDesc *)b->d_lat, (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+      else
+        hipLaunchKernelGGL((k_dhop_batch<P, false, 1, true>), dim3(b->grid), dim3(kThreads), lds, st,
+                           (const LatDesc *)b->d_lat, (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+      return;
+    }
+  }
   if (b->dag)
     hipLaunchKernelGGL((k_dhop_batch<P, true, 1>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
                        (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
@@ -140,7 +152,7 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     GBX_CHECK(u[i]->prec == prec && in[i]->prec == prec && out[i]->prec == prec, GBX_EMISMATCH,
               "gbx_batch_create: lattice %d precision differs from lattice 0", i);
     GBX_CHECK(in[i]->M == out[i]->M && in[i]->d != out[i]->d, GBX_EMISMATCH, "gbx_batch_create: lattice %d in/out mismatch", i);
-    GBX_TRY(require_chiral("gbx_batch_create", in[i], out[i]));
+    GBX_CHECK(in[i]->basis == in[0]->basis, GBX_EMISMATCH, "gbx_batch_create: lattice %d is in another spin basis", i);
     for (int j = 0; j < n; j++)
       GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
                 "gbx_batch_create: output %d aliases another lattice's field", i);
@@ -160,6 +172,7 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   gbx_batch_s *b = new gbx_batch_s();
   b->prec = prec;
   b->dag = dag;
+  b->td = in[0]->basis == GBX_BASIS_TDIAG;
   b->n = n;
   b->nitems = (uint32_t)items.size();
   b->max_spw = max_spw;
@@ -193,6 +206,7 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: end capture"));
   if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
     return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: instantiate"));
+  for (int i = 0; i < n; i++) out[i]->basis = in[i]->basis;  // what every run writes
   *bp = b;
   return GBX_OK;
 }

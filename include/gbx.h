@@ -268,8 +268,8 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* back to the chiral basis, agrees with the chiral kernel's to rounding.   */
 /* No counterpart in the reference (its kernel stores chiral spinors).      */
 /* TDIAG fields: FP64 / FP32; accepted by gbx_dhop* and gbx_tune* (VALU   */
-/* and LDS-tiled forms), gbx_cg_solve (FP32 inner), gbx_cg_solve_f64 and   */
-/* gbx_cg_solve_eo (FP32 inner or none),                                    */
+/* and LDS-tiled forms), gbx_batch_*, gbx_cg_solve (FP32 inner),           */
+/* gbx_cg_solve_f64 and gbx_cg_solve_eo (FP32 inner or none),              */
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
 /* halo routines (a face is copied as stored: sending and receiving parts  */
 /* must be in one basis); every other entry returns GBX_EUNSUPPORTED for   */
@@ -529,9 +529,9 @@ int gbx_dwf_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor 
 /* ---------------------------------------------------------------------- */
 /* Batched application over many independent lattices (each its own      */
 /* geometry, gauge field, Ls and nrhs; FP32/FP16/BF16 storage) in one      */
-/* persistent launch captured in a HIP graph.  The fields are read as      */
-/* chiral-basis values: creation refuses TDIAG fields, and they must not   */
-/* be set to that basis while the batch is in use.                         */
+/* persistent launch captured in a HIP graph.  The inputs share one spin  */
+/* basis (GBX_BASIS_TDIAG for FP32 batches too), the outputs take it, and  */
+/* no field's basis may change while the batch is in use.                  */
 /* ---------------------------------------------------------------------- */
 typedef struct gbx_batch_s *gbx_batch;
 int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag,

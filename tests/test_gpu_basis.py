@@ -167,6 +167,8 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     geom, U, x, y = fields(gbx, p, "f64")
     x.set_basis(gbx.BASIS_TDIAG)
     host = np.empty_like(p.psi)
+    z0 = gbx.SpinorField(geom, p.Ls, 1, "f64")
+    z1 = gbx.SpinorField(geom, p.Ls, 1, "f64").upload(p.psi)
 
     def status(fn):
         with pytest.raises(gbx.GbxError) as e:
@@ -189,7 +191,7 @@ def test_entries_without_a_tdiag_form_refuse(gpu, gbx, problem):
     xe.set_basis(gbx.BASIS_TDIAG)
     xo.set_basis(gbx.BASIS_TDIAG)
     assert status(lambda: gbx.dwf_solve_eo((Ue, Uo), None, (xe, xo), (be, bo))) == EUNSUPPORTED
-    assert status(lambda: gbx.Batch([U], [y], [x])) == EUNSUPPORTED
+    assert status(lambda: gbx.Batch([U, U], [y, z0], [x, z1])) == EMISMATCH  # inputs in two bases
     assert status(lambda: y.axpy(1.0, x)) == EMISMATCH  # y chiral, x TDIAG
     assert status(lambda: x.set_basis(7)) == EINVAL
     h = gbx.SpinorField(geom, p.Ls, 1, "f16")
@@ -265,3 +267,25 @@ def test_even_odd_wilson_solve_on_tdiag_fields(gpu, gbx, problem):
         assert st["true_resid"] <= 1e-10
         for f, w in zip(x, want):
             assert rel_l2(chiral_download(gbx, f), w) <= 1e-8
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+@pytest.mark.parametrize("dag", [False, True])
+def test_batch_on_tdiag_fields(gpu, gbx, prec, dag):
+    """The persistent batch launch on TDIAG inputs: each lattice's output is
+    its own single-lattice TDIAG application bit for bit (same arithmetic)."""
+    lattices = [((4, 4, 4, 4), 4, 1), ((6, 4, 2, 8), 3, 2), ((2, 4, 4, 2), 5, 3)]
+    objs = []
+    for i, (dims, Ls, nrhs) in enumerate(lattices):
+        g = gbx.Geometry(dims)
+        U = gbx.GaugeField(g, prec).upload(gbx.gen_doubled_gauge(dims, 7000 + 2 * i))
+        x = gbx.SpinorField(g, Ls, nrhs, prec).upload(gbx.gen_spinor(dims, Ls, nrhs, 7001 + 2 * i))
+        x.set_basis(gbx.BASIS_TDIAG)
+        objs.append((g, U, x, gbx.SpinorField(g, Ls, nrhs, prec)))
+    b = gbx.Batch([o[1] for o in objs], [o[3] for o in objs], [o[2] for o in objs], dag=dag)
+    b.run()
+    for g, U, x, y in objs:
+        assert y.basis == gbx.BASIS_TDIAG
+        ref = gbx.SpinorField(g, x.Ls, x.nrhs, prec)
+        gbx.dhop(U, ref, x, dag=dag, cfg=gbx.kcfg(1, 64, 0, 0))
+        assert np.array_equal(chiral_download(gbx, ref), chiral_download(gbx, y))
