@@ -82,10 +82,20 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
-def traffic_from_profiles():
-    """HBM bytes per launch of the FP64 kernel, from the committed rocprofv3
-    PMC summary (TCC read/write requests, copy-calibrated)."""
-    path = os.path.join(ROOT, "profiles", "dhop_f64_24x24x24x24_ls16_pmc.md")
+# Headline check: the TDIAG result, set back to the chiral basis, against the
+# chiral kernel (itself pinned to the oracle at <= 1e-14 by the tests).
+HEADLINE_CHECK_TOL = 1e-13
+# rocprofv3 PMC summaries (TCC read/write requests, copy-calibrated) of the
+# kernels behind the headline and the 16-RHS rows.
+PROFILE_OF = {"f64_1rhs": "dhop_f64_24x24x24x24_ls16_pmc.md",
+              "f32_16rhs": "dhop_tile_f32_24x24x24x24_ls16_n16_stats_pmc.md",
+              "f16_16rhs": "dhop_mfma_f16_bf16_24x24x24x24_n16_stats_pmc.md"}
+
+
+def traffic_from_profiles(row="f64_1rhs"):
+    """HBM bytes per launch of a row's kernel, from its committed rocprofv3
+    PMC summary (the `traffic_bytes_per_launch:` line), or None."""
+    path = os.path.join(ROOT, "profiles", PROFILE_OF[row])
     try:
         with open(path) as f:
             for line in f:
@@ -96,27 +106,70 @@ def traffic_from_profiles():
     return None
 
 
-def cpu_baseline(dbl, psi, threads, budget_s=12.0):
-    """The oracle (plain-C restatement of the reference CPU Hand kernel) with
-    OpenMP over 4D sites on this host, on the same 24^4 x 16 FP64 inputs."""
+def cpu_baseline(dbl, psi, threads, budget_s=8.0):
+    """The CPU Hand kernel on this host: the oracle's C restatement of the
+    reference kernel (WilsonKernelsHandCpu.h:385-451 semantics) built for
+    timing (-O3, AVX2 or AVX-512 as the host allows, multiply-adds fused),
+    OpenMP over 4D sites, Nsimd 1, on the same 24^4 x 16 FP64 inputs, rep 0
+    untimed as the reference times.  Also: a thread sweep at 24^4 x 16 and the
+    reference's CPU configuration 8^4 x Ls8 at its nrep = 1000 (bench.cc:146),
+    one thread and all threads.  About 25 s of CPU time in all."""
     from oracle import oracle as orc
     V = int(np.prod(DIMS))
     nbr, prm = orc.tables(DIMS, 1, LS)
     U = orc.pack_gauge(DIMS, 1, dbl)
     inp = orc.pack_spinor(DIMS, 1, LS, psi)
-    orc.hand(U, inp, nbr, prm, V, LS, 1, nthreads=threads)  # rep 0, untimed (reference convention)
-    t0 = time.perf_counter()
-    n = 0
-    while True:
-        orc.hand(U, inp, nbr, prm, V, LS, 1, nthreads=threads)
-        n += 1
-        if time.perf_counter() - t0 >= budget_s or n >= 50:
+    out = orc.aligned_empty(inp.size, np.float64)
+
+    def per_app(nthr, budget, nmax):
+        orc.port_hand(U, inp, nbr, prm, V, LS, nthr, out)  # rep 0, untimed
+        t0 = time.perf_counter()
+        n = 0
+        while True:
+            orc.port_hand(U, inp, nbr, prm, V, LS, nthr, out)
+            n += 1
+            if time.perf_counter() - t0 >= budget or n >= nmax:
+                break
+        return (time.perf_counter() - t0) / n, n
+
+    flop = 1320.0 * V * LS
+    dt, n = per_app(threads, budget_s, 50)
+    # the timing build against the oracle (fused multiply-adds round differently)
+    want = orc.hand(U, inp, nbr, prm, V, LS, 1, nthreads=threads)
+    port_err = float(np.linalg.norm(out - want) / np.linalg.norm(want))
+    del want
+    sweep = []
+    for t in (1, 2, 4, 8, 16, 32, 64):
+        if t > threads:
             break
-    dt = (time.perf_counter() - t0) / n
-    gflops = 1320.0 * V * LS / dt / 1e9
-    return {"value": round(gflops, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"24^4 x Ls16 FP64 1-RHS, {n} full applications after 1 untimed (oracle, OpenMP "
-                      f"{threads} threads, -O3 -march=x86-64-v3)", "ms_per_application": round(dt * 1e3, 2),
+        d, k = (dt, n) if t == threads else per_app(t, 2.0, 20)
+        sweep.append({"threads": t, "gflops": round(flop / d / 1e9, 2), "applications": k})
+    if threads not in [r["threads"] for r in sweep]:
+        sweep.append({"threads": threads, "gflops": round(flop / dt / 1e9, 2), "applications": n})
+    # 8^4 x Ls8 (the reference's CPU configuration), nrep = 1000 with rep 0
+    # untimed, like dslash_kernel_cpu's own timer (WilsonKernelsHandCpu.h:407-450)
+    d1, Ls1 = (8, 8, 8, 8), 8
+    V1 = int(np.prod(d1))
+    nbr1, prm1 = orc.tables(d1, 1, Ls1)
+    import gridbench_amd as gbx
+    U1 = orc.pack_gauge(d1, 1, gbx.gen_doubled_gauge(d1, 101))
+    in1 = orc.pack_spinor(d1, 1, Ls1, gbx.gen_spinor(d1, Ls1, 1, 102)[0])
+    out1 = orc.aligned_empty(in1.size, np.float64)
+    c1 = []
+    for t in sorted({1, threads}):
+        orc.port_hand(U1, in1, nbr1, prm1, V1, Ls1, t, out1)
+        t0 = time.perf_counter()
+        for _ in range(999):
+            orc.port_hand(U1, in1, nbr1, prm1, V1, Ls1, t, out1)
+        d = (time.perf_counter() - t0) / 999
+        c1.append({"threads": t, "gflops": round(1320.0 * V1 * Ls1 / d / 1e9, 2), "us_per_application": round(d * 1e6, 1)})
+    build = orc.port_lib()[1]
+    return {"value": round(flop / dt / 1e9, 3), "unit": "GFLOP/s", "cores": threads, "kind": "port",
+            "sample": f"24^4 x Ls16 FP64 1-RHS, {n} full applications after 1 untimed (the oracle's C restatement "
+                      f"built for timing: {build}, -O3, fused multiply-adds; OpenMP {threads} threads)",
+            "ms_per_application": round(dt * 1e3, 2), "rel_l2_vs_oracle": float(f"{port_err:.3e}"),
+            "thread_sweep_24x24x24x24_ls16": sweep,
+            "c1_8x8x8x8_ls8_nrep1000": c1,
             "host": host_info()}
 
 
@@ -691,7 +744,10 @@ def main():
     log(f"FP64 1-RHS chiral-basis field: {us_chiral:.1f} us/launch; headline result vs it: rel L2 {err_h:.2e}")
 
     stream_gbps = gbx.stream_copy_bw(4 << 30, 20)
-    log(f"STREAM copy: {stream_gbps:.0f} GB/s")
+    read3_gbps = gbx.stream_bw(6 << 30, 3, 20)
+    read0_gbps = gbx.stream_bw(4 << 30, 0, 20)
+    achievable = max(stream_gbps, read3_gbps, read0_gbps)
+    log(f"HBM probes: copy {stream_gbps:.0f}, 3:1 read:write {read3_gbps:.0f}, read-only {read0_gbps:.0f} GB/s")
 
     extra = {}
     if not args.no_variants:
@@ -722,6 +778,32 @@ def main():
         ref_cpu = cpu_reference_build(dbl, psi)
         log(f"reference build on the CPU: {ref_cpu}")
 
+    # the multi-RHS rows of the metric, compact (the full rows are in "variants")
+    rows16 = {}
+    for v in extra.get("variants", []):
+        if v["name"] in ("f32_16rhs", "f16_16rhs"):
+            t = traffic_from_profiles(v["name"])
+            alg = gbx.bytes_per_site(v["prec"], v["prec"], v["Ls"], v["nrhs"], v["comp12"]) * sites5 * v["nrhs"]
+            rows16[v["name"]] = {"us": v["us_per_launch"], "gflops": v["gflops"], "hbm_gbps_alg": v["hbm_gbps_alg"],
+                                 "frac": round(v["hbm_gbps_alg"] / HBM_PEAK_GBPS, 4),
+                                 "per_rhs_speedup": v["per_rhs_speedup_vs_f64_1rhs"], "rel_l2": v["rel_l2_vs_f64"],
+                                 "traffic_ratio": round(t / alg, 3) if t else None,
+                                 "kind": {0: "valu", 1: "mfma", 2: "tile"}.get(v["cfg"]["kind"], "?")}
+    dropin_us = None
+    for r in extra.get("dropin", {}).get("rows", []):
+        if r["prec"] == "f64":
+            dropin_us = r["us_per_application_device"]
+    cfg_extra = {"chiral_basis_us_per_launch": round(us_chiral, 2),
+                 "chiral_basis_gflops": round(1320.0 * sites5 / (us_chiral * 1e-6) / 1e9, 1),
+                 "dropin_f64_us_per_application": dropin_us}
+    roof_extra = {}
+    for name, r in rows16.items():
+        for k in ("us", "gflops", "per_rhs_speedup", "rel_l2", "kind"):
+            cfg_extra[f"{name}_{k}"] = r[k]
+        roof_extra[f"{name}_frac"] = r["frac"]
+        roof_extra[f"{name}_traffic_ratio"] = r["traffic_ratio"]
+    t64 = traffic_from_profiles()
+
     result = {
         "metric": "DWF Dhop GFLOP/s + HBM GB/s @24^4xLs16: FP64 1-RHS vs FP32/FP16 16-RHS",
         "value": round(gflops, 2),
@@ -738,21 +820,31 @@ def main():
         "config": {"workload": "DWF Dhop (Hand DhopSite, non-dagger) 24^4 x Ls16 FP64 1-RHS", "dims": list(DIMS),
                    "Ls": LS, "nrhs": 1, "flop_per_5d_site": 1320, "kernel_cfg": best.as_dict(),
                    "spin_basis": "tdiag (gamma_t diagonal; converted before the timed region)",
-                   "traversal_block": list(FP64_BLOCK)},
+                   "traversal_block": list(FP64_BLOCK), **cfg_extra},
         "chiral_basis": {"us_per_launch_events": round(us_chiral, 2), "kernel_cfg": best_chiral.as_dict(),
                          "gflops": round(1320.0 * sites5 / (us_chiral * 1e-6) / 1e9, 1),
                          "headline_rel_l2_vs_chiral_kernel": float(f"{err_h:.3e}")},
         "hbm_gbps_alg": round(achieved, 1),
-        "achievable_hbm_gbps_stream_copy": round(stream_gbps, 1),
-        "frac_of_achievable_hbm": round(achieved / stream_gbps, 3),
+        "achievable_hbm_gbps": {"copy": round(stream_gbps, 1), "read3_write1": round(read3_gbps, 1),
+                                "read_only": round(read0_gbps, 1), "max": round(achievable, 1)},
+        "frac_of_achievable_hbm": round(achieved / achievable, 3),
+        "headline_check": {"rel_l2_vs_chiral_kernel": float(f"{err_h:.3e}"), "tol": HEADLINE_CHECK_TOL,
+                           "passed": err_h <= HEADLINE_CHECK_TOL},
+        "multi_rhs_16": rows16,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": traffic_from_profiles(),
-                     "algorithmic_bytes_per_launch": bytes_alg, "us_per_launch_events": round(us_ev, 2)},
+                     "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": t64,
+                     "traffic_ratio": round(t64 / bytes_alg, 3) if t64 else None,
+                     "algorithmic_bytes_per_launch": bytes_alg, "us_per_launch_events": round(us_ev, 2),
+                     "frac_of_achievable": round(achieved / achievable, 3), **roof_extra},
         "cpu_baseline": cpu,
         "cpu_reference_build": ref_cpu,
         **extra,
     }
     print(json.dumps(result), flush=True)
+    if err_h > HEADLINE_CHECK_TOL:
+        print(f"bench.py: the headline result differs from the chiral kernel's by {err_h:.3e} (rel L2) "
+              f"> {HEADLINE_CHECK_TOL:g}", file=sys.stderr)
+        sys.exit(4)
 
 
 if __name__ == "__main__":

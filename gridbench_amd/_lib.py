@@ -173,6 +173,7 @@ PROTOTYPES = {
     "gbx_batch_time": (i32, [vp, i32, pdbl]),
     "gbx_batch_info": (i32, [vp, pu64, pi32]),
     "gbx_stream_copy_bw": (i32, [u64, i32, pdbl]),
+    "gbx_stream_bw": (i32, [u64, i32, i32, pdbl]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

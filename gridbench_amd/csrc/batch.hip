@@ -78,6 +78,11 @@ struct gbx_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<gbx_geom_s *> geoms;
+  // The fields the graph reads and writes (they must outlive the batch).
+  // Every run checks that the inputs are still in the spin basis the graph
+  // was captured for and stamps that basis on the outputs it writes.
+  std::vector<gbx_spinor_s *> in, out;
+  int basis = GBX_BASIS_CHIRAL;
   ~gbx_batch_s()
   {
     if (stream) (void)hipStreamSynchronize(stream);
@@ -147,11 +152,10 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   uint32_t max_spw = 1;
   for (int i = 0; i < n; i++) {
     GBX_CHECK(u[i] && out[i] && in[i], GBX_ENULL, "gbx_batch_create: NULL handle at %d", i);
-    GBX_CHECK(u[i]->geom == out[i]->geom && dhop_input_geom_ok(out[i]->geom, in[i]->geom), GBX_EMISMATCH,
-              "gbx_batch_create: lattice %d mixes geometries (a red-black half reads its partner half)", i);
-    GBX_CHECK(u[i]->prec == prec && in[i]->prec == prec && out[i]->prec == prec, GBX_EMISMATCH,
-              "gbx_batch_create: lattice %d precision differs from lattice 0", i);
-    GBX_CHECK(in[i]->M == out[i]->M && in[i]->d != out[i]->d, GBX_EMISMATCH, "gbx_batch_create: lattice %d in/out mismatch", i);
+    // the single-lattice entries' checks: allocated fields, geometry, equal
+    // Ls and nrhs, one precision, no aliasing
+    GBX_TRY(dhop_check_operands(u[i], out[i], in[i]));
+    GBX_CHECK(u[i]->prec == prec, GBX_EMISMATCH, "gbx_batch_create: lattice %d precision differs from lattice 0", i);
     GBX_CHECK(in[i]->basis == in[0]->basis, GBX_EMISMATCH, "gbx_batch_create: lattice %d is in another spin basis", i);
     for (int j = 0; j < n; j++)
       GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
@@ -173,6 +177,9 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   b->prec = prec;
   b->dag = dag;
   b->td = in[0]->basis == GBX_BASIS_TDIAG;
+  b->basis = in[0]->basis;
+  b->in.assign(in, in + n);
+  b->out.assign(out, out + n);
   b->n = n;
   b->nitems = (uint32_t)items.size();
   b->max_spw = max_spw;
@@ -217,11 +224,28 @@ extern "C" int gbx_batch_destroy(gbx_batch b)
   return GBX_OK;
 }
 
+// The captured graph applies the kernel of the inputs' spin basis at capture
+// time: refuse to run it on inputs converted since.
+static int batch_check_basis(const gbx_batch_s *b, const char *what)
+{
+  for (int i = 0; i < b->n; i++)
+    GBX_CHECK(b->in[i]->basis == b->basis, GBX_EMISMATCH,
+              "%s: input %d is in another spin basis than when the batch was created", what, i);
+  return GBX_OK;
+}
+
+static void batch_stamp_outputs(gbx_batch_s *b)
+{
+  for (gbx_spinor_s *o : b->out) o->basis = b->basis;
+}
+
 extern "C" int gbx_batch_run(gbx_batch b)
 {
   GBX_CHECK(b, GBX_ENULL, "gbx_batch_run: NULL batch");
+  GBX_TRY(batch_check_basis(b, "gbx_batch_run"));
   for (gbx_geom_s *g : b->geoms) GBX_HIP(hipStreamSynchronize(g->stream));
   GBX_HIP(hipGraphLaunch(b->exec, b->stream));
+  batch_stamp_outputs(b);
   GBX_HIP(hipStreamSynchronize(b->stream));
   return GBX_OK;
 }
@@ -230,7 +254,7 @@ extern "C" int gbx_batch_time(gbx_batch b, int nrep, double *usec)
 {
   GBX_CHECK(b && usec, GBX_ENULL, "gbx_batch_time: NULL argument");
   GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_batch_time: nrep=%d", nrep);
-  GBX_TRY(gbx_batch_run(b));  // warm-up
+  GBX_TRY(gbx_batch_run(b));  // warm-up (checks the inputs' basis, stamps the outputs')
   hipEvent_t e0 = nullptr, e1 = nullptr;
   int rc = GBX_OK;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess)

@@ -853,6 +853,14 @@ def stream_copy_bw(nbytes=2 << 30, nrep=20):
     return g.value
 
 
+def stream_bw(read_bytes=4 << 30, mix=3, nrep=20):
+    """HBM bandwidth (GB/s, read + write) of a read-weighted stream: one
+    16-byte write per `mix` 16-byte reads, or reads only with mix = 0."""
+    g = C.c_double()
+    check(lib.gbx_stream_bw(int(read_bytes), int(mix), int(nrep), C.byref(g)), "gbx_stream_bw")
+    return g.value
+
+
 # ------------------------------------------------------------ work models
 def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
     """Algorithmic HBM bytes per (5D site, right-hand side): in + out spinor
@@ -871,7 +879,7 @@ __all__ = [
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
     "dhop", "dhop_axpby",
-    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "bytes_per_site", "dhop_xpay",
+    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "stream_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
     "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time", "grid_coords", "halo_buffers",

@@ -549,6 +549,12 @@ int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups);
 /* write bytes.                                                            */
 /* ---------------------------------------------------------------------- */
 int gbx_stream_copy_bw(uint64_t bytes, int nrep, double *gbps);
+/* Read-weighted variant: reads `read_bytes` (a multiple of 16*mix) and     */
+/* writes one 16-byte element per `mix` read (mix = 1, 2, 3, 4 or 8), or,  */
+/* with mix = 0, only reads (one 16-byte write per thread).  GB/s counts   */
+/* read + write bytes.  A kernel that reads several times what it writes  */
+/* (Dhop reads about 2.7 B of HBM per byte written) is bounded by these.  */
+int gbx_stream_bw(uint64_t read_bytes, int mix, int nrep, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -175,42 +175,3 @@ int oracle_pack_gauge_f64(const int dims[4], int nsimd, const double *lex, doubl
     }
   return 0;
 }
-
-/* Build the doubled field U[x][mu] = U_mu(x), U[x][mu+4] = U_mu(x - mu)^dagger
- * from the 4 forward links per site, lexicographic [V][4][3][3][2] ->
- * [V][8][3][3][2]. */
-void oracle_double_gauge_f64(const int dims[4], const double *links, double *dbl)
-{
-  const uint64_t V = (uint64_t)dims[0] * dims[1] * dims[2] * dims[3];
-  for (uint64_t x = 0; x < V; x++) {
-    int c[4];
-    uint64_t r = x;
-    for (int d = 0; d < 4; d++) { c[d] = (int)(r % (uint64_t)dims[d]); r /= (uint64_t)dims[d]; }
-    for (int mu = 0; mu < 4; mu++) {
-      int b[4];
-      memcpy(b, c, sizeof b);
-      b[mu] = (c[mu] - 1 + dims[mu]) % dims[mu];
-      uint64_t xb = 0;
-      for (int d = 3; d >= 0; d--) xb = xb * (uint64_t)dims[d] + (uint64_t)b[d];
-      for (int i = 0; i < 3; i++)
-        for (int j = 0; j < 3; j++) {
-          const double *f = links + ((x * 4 + (uint64_t)mu) * 9 + (uint64_t)(i * 3 + j)) * 2;
-          double *o = dbl + ((x * 8 + (uint64_t)mu) * 9 + (uint64_t)(i * 3 + j)) * 2;
-          o[0] = f[0];
-          o[1] = f[1];
-          /* backward link: conjugate transpose of U_mu(x - mu) */
-          const double *g = links + ((xb * 4 + (uint64_t)mu) * 9 + (uint64_t)(j * 3 + i)) * 2;
-          double *ob = dbl + ((x * 8 + (uint64_t)mu + 4) * 9 + (uint64_t)(i * 3 + j)) * 2;
-          ob[0] = g[0];
-          ob[1] = -g[1];
-        }
-    }
-  }
-}
-
-/* Plain FP64 -> FP32 conversion (round to nearest), as bench.cc:123,135 does
- * when it fills the single-precision arrays from the double data. */
-void oracle_to_f32(const double *src, float *dst, uint64_t n)
-{
-  for (uint64_t i = 0; i < n; i++) dst[i] = (float)src[i];
-}

@@ -3,6 +3,8 @@
 ctypes access to
   * liboracle.so: the plain-C restatement of the reference Hand kernel
     (hand.c / hand_impl.h) and its table/layout helpers;
+  * libport_v{3,4}.so: the same source built for timing (multiply-adds may
+    be fused), for bench.py's CPU baseline only;
   * _ref/libref_w{16,32,64,128}.so (when built in this checkout): the
     reference's own dslash_kernel_cpu (WilsonKernelsHandCpu.h:385-451) and
     the scalar dslash_simple.h kernel, compiled from the reference sources by
@@ -46,10 +48,54 @@ def lib():
             getattr(L, n).argtypes = [P(C.c_int), C.c_int, C.c_uint64, P(C.c_double), P(C.c_double)]
         L.oracle_pack_gauge_f64.restype = C.c_int
         L.oracle_pack_gauge_f64.argtypes = [P(C.c_int), C.c_int, P(C.c_double), P(C.c_double)]
-        L.oracle_double_gauge_f64.restype = None
-        L.oracle_double_gauge_f64.argtypes = [P(C.c_int), P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
+
+
+_port = None
+
+
+def host_supports_avx512():
+    """x86-64-v4 (AVX-512 F/BW/CD/DQ/VL) on this host, from /proc/cpuinfo."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("flags"):
+                    fl = set(line.split(":", 1)[1].split())
+                    return {"avx512f", "avx512bw", "avx512cd", "avx512dq", "avx512vl"} <= fl
+    except OSError:
+        pass
+    return False
+
+
+def port_lib():
+    """The restatement built for timing (libport_v4.so on AVX-512 hosts, else
+    libport_v3.so): same source, multiply-adds may be fused.  Returns
+    (library, name).  bench.py's CPU baseline only; never the checker."""
+    global _port
+    if _port is None:
+        name = "libport_v4.so" if host_supports_avx512() else "libport_v3.so"
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            raise ImportError(f"{path} missing: run `make -C oracle`")
+        L = C.CDLL(path)
+        P = C.POINTER
+        f = L.oracle_dhop_hand_f64
+        f.restype = None
+        f.argtypes = [C.c_int, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_uint64), P(C.c_uint8),
+                      C.c_uint64, C.c_uint64, C.c_int]
+        _port = (L, name)
+    return _port
+
+
+def port_hand(U, inp, nbr, prm, nsite, Ls, nthreads, out=None):
+    """One FP64 application of the timing build (Nsimd 1); returns the output."""
+    L, _ = port_lib()
+    if out is None:
+        out = aligned_empty(inp.size, np.float64)
+    L.oracle_dhop_hand_f64(1, 0, _p(U, C.c_double), _p(out, C.c_double), _p(inp, C.c_double), _p(nbr, C.c_uint64),
+                           _p(prm, C.c_uint8), int(nsite), int(Ls), int(nthreads))
+    return out
 
 
 def _p(a, T):
@@ -127,13 +173,6 @@ def pack_gauge(dims, nsimd, lex_doubled):
     out = aligned_empty(lex.size, np.float64)
     if lib().oracle_pack_gauge_f64(_dims(dims), nsimd, _p(lex, C.c_double), _p(out, C.c_double)):
         raise ValueError((dims, nsimd))
-    return out
-
-
-def double_gauge(dims, links):
-    links = np.ascontiguousarray(links, dtype=np.float64).reshape(-1)
-    out = np.empty(links.size * 2, dtype=np.float64)
-    lib().oracle_double_gauge_f64(_dims(dims), _p(links, C.c_double), _p(out, C.c_double))
     return out
 
 

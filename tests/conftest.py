@@ -38,6 +38,41 @@ def rel_l2(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
+def round_storage(x, prec):
+    """x rounded to nearest (ties to even) in the storage precision, as float64."""
+    x = np.asarray(x, dtype=np.float64)
+    if prec == "f64":
+        return x
+    if prec == "f16":
+        return x.astype(np.float16).astype(np.float64)
+    f = x.astype(np.float32)
+    if prec == "f32":
+        return f.astype(np.float64)
+    b = f.view(np.uint32).astype(np.uint64)  # bf16: round the float32's low 16 bits away
+    b = (((b + 0x7FFF + ((b >> 16) & 1)) >> 16) << 16).astype(np.uint32)
+    return b.view(np.float32).astype(np.float64)
+
+
+# Dhop's error with 16-bit storage and FP32 arithmetic comes from rounding the
+# spinor, the links and the output once each: about sqrt(3) = 1.7 x the
+# relative rounding error of storing the input field.  The bound allows 2.5 x;
+# a kernel that also rounded its sums to 16 bits would exceed it.
+ROUNDING_FACTOR_16 = 2.5
+
+
+def storage_tol(prec, x):
+    """Relative L2 bound of Dhop against the FP64 oracle for a field x (the
+    exact input values) stored in prec: 1e-14 FP64, 1e-6 FP32, and for FP16 /
+    BF16 ROUNDING_FACTOR_16 x the relative error of storing x itself
+    (measured on up to 2M of its values)."""
+    if prec == "f64":
+        return 1e-14
+    if prec == "f32":
+        return 1e-6
+    s = np.asarray(x, dtype=np.float64).reshape(-1)[:1 << 21]
+    return ROUNDING_FACTOR_16 * rel_l2(round_storage(s, prec), s)
+
+
 class Problem:
     """Seeded inputs of one lattice in both layouts, plus oracle helpers."""
 

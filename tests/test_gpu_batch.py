@@ -79,3 +79,46 @@ def test_batch_argument_errors(gpu, gbx):
     with pytest.raises(gbx.GbxError) as e:  # an output aliasing another lattice's input
         gbx.Batch([objs[0][1], objs[0][1]], [objs[0][3], objs[0][2]], [objs[0][2], objs[0][2]])
     assert e.value.status in (3, 9)
+
+
+def test_batch_rechecks_the_spin_basis_on_every_run(gpu, gbx):
+    """A batch captured on TDIAG fields: reading an output (converted to the
+    chiral basis in place) and running again restamps the output, so the next
+    read is right; an input converted back to chiral is refused (the graph
+    holds the TDIAG kernel) until it is converted again."""
+    objs = build(gbx, "f32", LATTICES[:3])
+    for o in objs:
+        o[2].set_basis(gbx.BASIS_TDIAG)
+    b = gbx.Batch([o[1] for o in objs], [o[3] for o in objs], [o[2] for o in objs])
+    want = []
+    for g, U, x, y, dbl, psi in objs:
+        ref = gbx.SpinorField(g, x.Ls, x.nrhs, "f32")
+        gbx.dhop(U, ref, x, cfg=gbx.kcfg(1, 64, 0, 0))
+        want.append(ref.set_basis(gbx.BASIS_CHIRAL).download())
+    for _ in range(2):
+        b.run()
+        for o, w in zip(objs, want):
+            assert o[3].basis == gbx.BASIS_TDIAG
+            assert np.array_equal(o[3].set_basis(gbx.BASIS_CHIRAL).download(), w)
+    objs[1][2].set_basis(gbx.BASIS_CHIRAL)
+    with pytest.raises(gbx.GbxError) as e:
+        b.run()
+    assert e.value.status == 9  # GBX_EMISMATCH
+    with pytest.raises(gbx.GbxError) as e:
+        b.time(1)
+    assert e.value.status == 9
+    objs[1][2].set_basis(gbx.BASIS_TDIAG)
+    b.run()
+    assert np.array_equal(objs[1][3].set_basis(gbx.BASIS_CHIRAL).download(), want[1])
+
+
+def test_batch_checks_each_lattice_like_the_single_entries(gpu, gbx):
+    """Ls and nrhs must match between a lattice's input and output (equal
+    Ls * nrhs is not enough), and fields must be allocated."""
+    objs = build(gbx, "f32", LATTICES[:1])
+    g, U, x, y, dbl, psi = objs[0]
+    assert (x.Ls, x.nrhs) == (4, 1)
+    bad = gbx.SpinorField(g, 2, 2, "f32")  # Ls * nrhs = 4 as well
+    with pytest.raises(gbx.GbxError) as e:
+        gbx.Batch([U], [bad], [x])
+    assert e.value.status == 9

@@ -4,17 +4,19 @@ of the reference Hand kernel, WilsonKernelsHandCpu.h:385-451).
 Tolerances (relative L2 over the whole output, vs the FP64 oracle):
   FP64 storage           <= 1e-14   (FMA contraction on the GPU only)
   FP32 storage           <= 1e-6
-  FP16 storage           <= 2e-3    (10-bit mantissa inputs, FP32 arithmetic)
-  BF16 storage           <= 1.6e-2  (7-bit mantissa inputs, FP32 arithmetic)
+  FP16 / BF16 storage    <= 2.5 x the relative error of storing the input
+                            field in that format (conftest.storage_tol: one
+                            rounding each of spinor, links and output gives
+                            about 1.7 x; 16-bit sums would exceed 2.5 x)
 """
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, storage_tol
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f64": 1e-14, "f32": 1e-6, "f16": 2e-3, "bf16": 1.6e-2}
+TOL = {"f64": 1e-14, "f32": 1e-6}
 
 
 def run_dhop(gbx, p, prec, dag=False, comp12=False, block=None, cfg=None, psi=None):
@@ -46,8 +48,9 @@ def test_low_precision_vs_f64_oracle(gpu, gbx, problem, prec, comp12, dag):
     got = run_dhop(gbx, p, prec, dag=dag, comp12=comp12)[0]
     want = p.oracle_lex(dag=dag)
     err = rel_l2(got, want)
-    print(f"prec={prec} comp12={comp12} dag={dag} rel_l2_vs_f64_oracle={err:.3e}")
-    assert err <= TOL[prec]
+    tol = storage_tol(prec, p.psi)
+    print(f"prec={prec} comp12={comp12} dag={dag} rel_l2_vs_f64_oracle={err:.3e} bound={tol:.3e}")
+    assert err <= tol
 
 
 def test_f32_vs_f32_oracle(gpu, gbx, problem):

@@ -10,7 +10,7 @@ checked with the oracle's M = 1 - kappa Dhop on the merged lattice.
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, storage_tol
 
 pytestmark = pytest.mark.gpu
 
@@ -73,8 +73,8 @@ def test_halves_reproduce_whole_lattice_kernel(gpu, gbx, problem, prec):
 
 @pytest.mark.parametrize("prec", ["f16", "bf16"])
 def test_mfma_and_compressed_links_on_halves(gpu, gbx, problem, prec):
-    tol = {"f16": 2e-3, "bf16": 1.6e-2}[prec]
     p = problem((4, 4, 4, 8), 8, nrhs=2)
+    tol = storage_tol(prec, p.psi)
     for comp12 in (False, True):
         ge, go, Ue, Uo = halves(gbx, p, prec, comp12)
         pe, po = gbx.rb_split(p.dims, p.psi)
@@ -83,7 +83,7 @@ def test_mfma_and_compressed_links_on_halves(gpu, gbx, problem, prec):
         gbx.dhop(Ue, ye, xo, cfg=gbx.kcfg(2, 128, 1, 1, kind=1, pf=1))
         mf = ye.download()
         gbx.dhop(Ue, ye, xo, cfg=gbx.kcfg(4, 64, 1, 0))
-        assert rel_l2(mf, ye.download()) <= (5e-4 if prec == "f16" else 4e-3)
+        assert rel_l2(mf, ye.download()) <= tol
         for r in range(2):
             want = gbx.rb_split(p.dims, p.oracle_lex(rhs=r, psi=masked(gbx, p, 1)[r])[None])[0][0]
             assert rel_l2(mf[r], want) <= tol

@@ -8,11 +8,10 @@ and the limit itself.
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, storage_tol
 
 pytestmark = pytest.mark.gpu
 
-TOL = {"f64": 1e-14, "f32": 1e-6, "f16": 2e-3, "bf16": 1.6e-2}
 
 
 def apply(gbx, p, prec, cfg=None, dag=False):
@@ -31,7 +30,7 @@ def test_largest_minor_extent(gpu, gbx, problem, prec):
     for r in (0, 31, 63):
         err = rel_l2(got[r], p.oracle_lex(rhs=r))
         print(f"prec={prec} rhs={r} rel_l2={err:.3e}")
-        assert err <= TOL[prec]
+        assert err <= storage_tol(prec, p.psi)
 
 
 def test_ragged_minor_extent(gpu, gbx, problem):
@@ -47,7 +46,7 @@ def test_mfma_largest_minor_extent(gpu, gbx, problem, prec):
     p = problem((2, 2, 2, 4), 64, nrhs=64)
     got = apply(gbx, p, prec, cfg=gbx.kcfg(1, 256, kind=1))
     for r in (0, 63):
-        assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= TOL[prec]
+        assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= storage_tol(prec, p.psi)
 
 
 @pytest.mark.parametrize("dims", [(1, 2, 4, 4), (4, 1, 1, 4), (2, 2, 2, 2), (1, 1, 1, 1)])

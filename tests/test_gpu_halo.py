@@ -12,7 +12,7 @@ application is the reference operator (WilsonKernelsHandCpu.h:426-446).
 import numpy as np
 import pytest
 
-from conftest import rel_l2
+from conftest import rel_l2, storage_tol
 
 pytestmark = pytest.mark.gpu
 
@@ -157,7 +157,7 @@ def test_parts_mfma_multi_rhs(gpu, gbx, problem, prec):
     got = parts.merged()
     assert np.array_equal(got, want)
     for r in range(p.nrhs):
-        assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= (2e-3 if prec == "f16" else 1.6e-2)
+        assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= storage_tol(prec, p.psi)
 
 
 def test_projected_face_is_half_and_matches_projection(gpu, gbx, problem, orc):
