@@ -530,14 +530,17 @@ int gbx_dwf_solve_eo(const gbx_gauge u_hi[2], const gbx_gauge *u_lo, gbx_spinor 
 /* Batched application over many independent lattices (each its own      */
 /* geometry, gauge field, Ls and nrhs; FP32/FP16/BF16 storage) in one      */
 /* persistent launch captured in a HIP graph.  The inputs share one spin  */
-/* basis (GBX_BASIS_TDIAG for FP32 batches too), the outputs take it, and  */
-/* no field's basis may change while the batch is in use.                  */
+/* basis (GBX_BASIS_TDIAG for FP64 / FP32 batches too) and the outputs     */
+/* take it.  The fields must outlive the batch.                            */
 /* ---------------------------------------------------------------------- */
 typedef struct gbx_batch_s *gbx_batch;
 int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag,
                      gbx_batch *b);
 int gbx_batch_destroy(gbx_batch b);
-/* Replays the graph once on the batch's stream and waits for it. */
+/* Replays the graph once on the batch's stream and waits for it.  Every  */
+/* run (and gbx_batch_time) returns GBX_EMISMATCH if an input's spin basis */
+/* is no longer the one the graph was captured for, and stamps that basis  */
+/* on the outputs it writes.                                                */
 int gbx_batch_run(gbx_batch b);
 int gbx_batch_time(gbx_batch b, int nrep, double *usec);
 /* Work items and persistent workgroups of the launch. */

@@ -107,9 +107,14 @@ def test_batch_rechecks_the_spin_basis_on_every_run(gpu, gbx):
     with pytest.raises(gbx.GbxError) as e:
         b.time(1)
     assert e.value.status == 9
-    objs[1][2].set_basis(gbx.BASIS_TDIAG)
+    # back in the captured basis (the round trip re-rounds the input, so the
+    # reference is recomputed from the converted input)
+    g, U, x, y = objs[1][:4]
+    x.set_basis(gbx.BASIS_TDIAG)
     b.run()
-    assert np.array_equal(objs[1][3].set_basis(gbx.BASIS_CHIRAL).download(), want[1])
+    ref = gbx.SpinorField(g, x.Ls, x.nrhs, "f32")
+    gbx.dhop(U, ref, x, cfg=gbx.kcfg(1, 64, 0, 0))
+    assert np.array_equal(y.set_basis(gbx.BASIS_CHIRAL).download(), ref.set_basis(gbx.BASIS_CHIRAL).download())
 
 
 def test_batch_checks_each_lattice_like_the_single_entries(gpu, gbx):
