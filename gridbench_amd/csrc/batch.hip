@@ -12,10 +12,19 @@
 // atomic's round trip overlaps the work.  The counter is cleared by a
 // one-thread kernel node (no memset nodes in the graph).
 #include <algorithm>
+#include <cstring>
 #include <vector>
+
+#include <mutex>
 
 #include "dhop_dev.h"
 #include "gbx_internal.h"
+
+// The no-allocation entry's work-queue counter and lattice table, one copy
+// per GPU, held until the process ends.  In host code the name is not a
+// device address: host code uses the address hipGetSymbolAddress returns for
+// the current device (na_scratch_ptr), never the name.
+static __device__ __align__(256) unsigned char scratch[GBX_NOALLOC_SCRATCH_BYTES];
 
 namespace {
 
@@ -30,6 +39,23 @@ struct LatDesc {
 constexpr int kThreads = 256;
 
 __global__ void k_reset(uint32_t *c) { c[0] = 0u; }
+
+// One work item: sites q0 .. q0+nsl-1 of lattice L, every minor index.
+template <int P, bool DAG, int PF, bool TD>
+__device__ __forceinline__ void batch_item(const LatDesc &L, uint32_t q0, uint32_t nsl, typename dev::Traits<P>::acc *sU,
+                                           uint32_t *sN, int nt)
+{
+  typedef typename dev::Traits<P>::acc A;
+  if (L.c12) dev::stage_links<P, true>(L.gauge, L.nbr, q0, nsl, sU, sN);
+  else dev::stage_links<P, false>(L.gauge, L.nbr, q0, nsl, sU, sN);
+  const uint32_t M = L.M, n = nsl * M;
+  for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
+    const uint32_t sl = w / M, m = w - sl * M;
+    A rr[12], ri[12];
+    dev::dhop_site<P, DAG, PF, false, TD>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
+    dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
+  }
+}
 
 template <int P, bool DAG, int PF, bool TD = false>
 __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
@@ -49,17 +75,65 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
     if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);  // claimed now, read after this item
     const uint4 it = items[idx];
     const LatDesc L = lat[it.x];
-    const uint32_t q0 = it.y, nsl = it.z;
-    if (L.c12) dev::stage_links<P, true>(L.gauge, L.nbr, q0, nsl, sU, sN);
-    else dev::stage_links<P, false>(L.gauge, L.nbr, q0, nsl, sU, sN);
-    const uint32_t M = L.M, n = nsl * M;
-    for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
-      const uint32_t sl = w / M, m = w - sl * M;
-      A rr[12], ri[12];
-      dev::dhop_site<P, DAG, PF, false, TD>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
-      dev::store_site<P, false>(L.out, nullptr, q0 + sl, M, m, rr, ri, (A)0, (A)1, nt != 0);
-    }
+    batch_item<P, DAG, PF, TD>(L, it.y, it.z, sU, sN, nt);
     __syncthreads();  // LDS is reused by the next item; s_next is written
+    idx = s_next;
+  }
+}
+
+// ------------------------------------------------ the no-allocation variant
+// gbx_batch_run_noalloc keeps its work-queue counter and its lattice table
+// in `scratch`, one fixed array per GPU (the library allocates nothing per
+// call).  The table arrives as the set-up kernel's argument (no host-to-
+// device copy); items are not listed: item k belongs to the lattice i with
+// item0[i] <= k < item0[i+1], and covers sites (k - item0[i]) * spw[i] on.
+struct NaLat {
+  LatDesc d;
+  uint32_t item0, spw;
+};
+struct NaTable {
+  uint32_t n, nitems, max_spw, pad;
+  NaLat lat[GBX_NOALLOC_MAX_LATTICES];
+};
+constexpr size_t kNaCounterBytes = 256;
+static_assert(kNaCounterBytes + sizeof(NaTable) <= GBX_NOALLOC_SCRATCH_BYTES, "scratch too small for the table");
+static_assert(sizeof(NaTable) <= 4096 - 64, "the table is passed as a kernel argument");
+
+// One thread: clear the counter, copy the table (a graph-capturable kernel
+// node, no memset or copy node).
+__global__ void k_na_setup(NaTable t, unsigned char *__restrict__ sp)
+{
+  *(uint32_t *)sp = 0u;
+  NaTable *dst = (NaTable *)(sp + kNaCounterBytes);
+  dst->n = t.n;
+  dst->nitems = t.nitems;
+  dst->max_spw = t.max_spw;
+  for (uint32_t i = 0; i < t.n; i++) dst->lat[i] = t.lat[i];
+}
+
+template <int P, bool DAG, int PF, bool TD = false>
+__global__ void __launch_bounds__(kThreads) k_dhop_batch_na(unsigned char *__restrict__ sp, int nt)
+{
+  typedef typename dev::Traits<P>::acc A;
+  extern __shared__ __align__(16) unsigned char smem[];
+  __shared__ uint32_t s_next;
+  uint32_t *counter = (uint32_t *)sp;
+  const NaTable *tab = (const NaTable *)(sp + kNaCounterBytes);
+  const uint32_t n = tab->n, nitems = tab->nitems;
+  A *sU = (A *)smem;
+  uint32_t *sN = (uint32_t *)(sU + (size_t)tab->max_spw * 8 * 18);
+  if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+  __syncthreads();
+  uint32_t idx = s_next;
+  while (idx < nitems) {  // idx is the same in every thread of the workgroup
+    __syncthreads();
+    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
+    uint32_t i = 0;
+    while (i + 1 < n && tab->lat[i + 1].item0 <= idx) i++;
+    const NaLat &e = tab->lat[i];
+    const uint32_t q0 = (idx - e.item0) * e.spw;
+    batch_item<P, DAG, PF, TD>(e.d, q0, min(e.spw, e.d.nslot - q0), sU, sN, nt);
+    __syncthreads();
     idx = s_next;
   }
 }
@@ -215,6 +289,110 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: instantiate"));
   for (int i = 0; i < n; i++) out[i]->basis = in[i]->basis;  // what every run writes
   *bp = b;
+  return GBX_OK;
+}
+
+namespace {
+
+std::mutex g_na_mu;
+unsigned char *g_na_ptr[GBX_MAX_DEVICES] = {};
+
+// The device address of `scratch` on device `dev` (made current), looked up
+// once per device.
+int na_scratch_ptr(int dev, unsigned char **ptr)
+{
+  GBX_CHECK(dev >= 0 && dev < GBX_MAX_DEVICES, GBX_EINVAL, "gbx_batch_run_noalloc: device %d", dev);
+  std::lock_guard<std::mutex> lk(g_na_mu);
+  if (!g_na_ptr[dev]) {
+    GBX_HIP(hipSetDevice(dev));
+    void *p = nullptr;
+    GBX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    GBX_CHECK(p != nullptr, GBX_EHIP, "gbx_batch_run_noalloc: no device address for the scratch array");
+    g_na_ptr[dev] = (unsigned char *)p;
+  }
+  *ptr = g_na_ptr[dev];
+  return GBX_OK;
+}
+
+template <int P, bool DAG, bool TD>
+void launch_na_t(dim3 grid, size_t lds, hipStream_t st, unsigned char *sp)
+{
+  hipLaunchKernelGGL((k_dhop_batch_na<P, DAG, 1, TD>), grid, dim3(kThreads), lds, st, sp, 1);
+}
+
+template <int P>
+void launch_na(bool dag, bool td, dim3 grid, size_t lds, hipStream_t st, unsigned char *sp)
+{
+  if constexpr (P == GBX_F64 || P == GBX_F32) {
+    if (td) {
+      if (dag) launch_na_t<P, true, true>(grid, lds, st, sp);
+      else launch_na_t<P, false, true>(grid, lds, st, sp);
+      return;
+    }
+  }
+  if (dag) launch_na_t<P, true, false>(grid, lds, st, sp);
+  else launch_na_t<P, false, false>(grid, lds, st, sp);
+}
+
+}  // namespace
+
+extern "C" int gbx_batch_run_noalloc(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag)
+{
+  GBX_CHECK(u && out && in, GBX_ENULL, "gbx_batch_run_noalloc: NULL argument");
+  GBX_CHECK(n >= 1, GBX_EINVAL, "gbx_batch_run_noalloc: n=%d", n);
+  GBX_CHECK(n <= GBX_NOALLOC_MAX_LATTICES, GBX_EINVAL,
+            "gbx_batch_run_noalloc: %d lattices; at most %d fit its fixed lattice table (use gbx_batch_create)", n,
+            GBX_NOALLOC_MAX_LATTICES);
+  GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_batch_run_noalloc: dag=%d", dag);
+  for (int i = 0; i < n; i++) GBX_CHECK(u[i] && out[i] && in[i], GBX_ENULL, "gbx_batch_run_noalloc: NULL handle at %d", i);
+  const int prec = u[0]->prec, dev = u[0]->geom->device, basis = in[0]->basis;
+  NaTable t;
+  memset(&t, 0, sizeof t);
+  uint64_t nitems = 0;
+  uint32_t max_spw = 1;
+  for (int i = 0; i < n; i++) {
+    GBX_TRY(dhop_check_operands(u[i], out[i], in[i]));
+    GBX_CHECK(u[i]->prec == prec, GBX_EMISMATCH, "gbx_batch_run_noalloc: lattice %d precision differs from lattice 0", i);
+    GBX_CHECK(in[i]->basis == basis, GBX_EMISMATCH, "gbx_batch_run_noalloc: lattice %d is in another spin basis", i);
+    GBX_CHECK(u[i]->geom->device == dev, GBX_EMISMATCH, "gbx_batch_run_noalloc: lattice %d is on another device", i);
+    for (int j = 0; j < n; j++)
+      GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
+                "gbx_batch_run_noalloc: output %d aliases another lattice's field", i);
+    const uint32_t M = in[i]->M, nslot = (uint32_t)u[i]->geom->vol;
+    uint32_t spw = (kThreads + M - 1) / M;  // as gbx_batch_create cuts items
+    if (spw < 1) spw = 1;
+    if (spw > 64) spw = 64;
+    max_spw = std::max(max_spw, spw);
+    t.lat[i].d = LatDesc{u[i]->d, u[i]->geom->nbr, in[i]->d, out[i]->d, M, nslot, (uint32_t)u[i]->comp12, 0};
+    t.lat[i].item0 = (uint32_t)nitems;
+    t.lat[i].spw = spw;
+    nitems += (nslot + spw - 1) / spw;
+    GBX_CHECK(nitems < (1ull << 31), GBX_EINVAL, "gbx_batch_run_noalloc: %llu work items", (unsigned long long)nitems);
+  }
+  t.n = (uint32_t)n;
+  t.nitems = (uint32_t)nitems;
+  t.max_spw = max_spw;
+  unsigned char *sp = nullptr;
+  GBX_TRY(na_scratch_ptr(dev, &sp));
+  int ncu = 256;
+  GBX_HIP(hipSetDevice(dev));
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const dim3 grid((unsigned)(ncu * (prec == GBX_F64 ? 4 : 6)));
+  const size_t lds = lds_for(prec, max_spw);
+  // inputs may still be in flight on their geometries' streams
+  for (int i = 0; i < n; i++) GBX_HIP(hipStreamSynchronize(u[i]->geom->stream));
+  hipStream_t st = u[0]->geom->stream;
+  hipLaunchKernelGGL(k_na_setup, dim3(1), dim3(1), 0, st, t, sp);
+  const bool td = basis == GBX_BASIS_TDIAG;
+  switch (prec) {
+  case GBX_F64: launch_na<GBX_F64>(dag, td, grid, lds, st, sp); break;
+  case GBX_F32: launch_na<GBX_F32>(dag, td, grid, lds, st, sp); break;
+  case GBX_F16: launch_na<GBX_F16>(dag, td, grid, lds, st, sp); break;
+  default: launch_na<GBX_BF16>(dag, td, grid, lds, st, sp); break;
+  }
+  GBX_HIP(hipGetLastError());
+  for (int i = 0; i < n; i++) out[i]->basis = basis;
+  GBX_HIP(hipStreamSynchronize(st));
   return GBX_OK;
 }
 

@@ -39,6 +39,10 @@ void gbx_clear_error();
 
 int gbx_require_device();  // GBX_ENODEV unless a HIP device is usable
 
+// Devices the library keeps per-device state for (the no-allocation entry's
+// scratch address).
+#define GBX_MAX_DEVICES 64
+
 // ---------------------------------------------------------------------------
 // Storage precisions.  Spinors and links are stored as FP64, FP32, FP16 or
 // BF16; arithmetic is FP64 for FP64 storage and FP32 for everything else.

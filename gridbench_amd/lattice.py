@@ -846,6 +846,19 @@ class Batch:
             pass
 
 
+def batch_run_noalloc(gauges, outs, ins, dag=False):
+    """The batched application of Batch(...).run() without any device
+    allocation (gbx_batch_run_noalloc): at most NOALLOC_MAX_LATTICES lattices,
+    one fixed scratch array per GPU; calls overlapping on one GPU race."""
+    n = len(gauges)
+    assert n == len(outs) == len(ins)
+    arr = lambda objs: (C.c_void_p * n)(*[o.h for o in objs])
+    check(lib.gbx_batch_run_noalloc(n, arr(gauges), arr(outs), arr(ins), int(bool(dag))), "gbx_batch_run_noalloc")
+
+
+NOALLOC_MAX_LATTICES = 32  # GBX_NOALLOC_MAX_LATTICES
+
+
 def stream_copy_bw(nbytes=2 << 30, nrep=20):
     """Achievable HBM bandwidth (GB/s, read + write) of a streaming copy."""
     g = C.c_double()
@@ -879,7 +892,7 @@ __all__ = [
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
     "dhop", "dhop_axpby",
-    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "stream_copy_bw", "stream_bw", "bytes_per_site", "dhop_xpay",
+    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "batch_run_noalloc", "NOALLOC_MAX_LATTICES", "stream_copy_bw", "stream_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
     "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time", "grid_coords", "halo_buffers",

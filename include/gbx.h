@@ -546,6 +546,19 @@ int gbx_batch_time(gbx_batch b, int nrep, double *usec);
 /* Work items and persistent workgroups of the launch. */
 int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups);
 
+/* No counterpart in the reference.  For callers that cannot allocate     */
+/* device memory -- code under HIP graph capture, tight serving loops,     */
+/* harnesses that forbid hipMalloc: the same batched application as       */
+/* gbx_batch_create + gbx_batch_run (same checks, the same bits), enqueued  */
+/* on the stream of lattice 0's geometry and waited for, with no graph and */
+/* no allocation.  Its work-queue counter and lattice table live in one    */
+/* fixed device array of GBX_NOALLOC_SCRATCH_BYTES per GPU, kept until the */
+/* process ends; n is at most GBX_NOALLOC_MAX_LATTICES (GBX_EINVAL above).  */
+/* Calls that overlap on one GPU share that array and race.                */
+#define GBX_NOALLOC_MAX_LATTICES 32
+#define GBX_NOALLOC_SCRATCH_BYTES 4096
+int gbx_batch_run_noalloc(int n, const gbx_gauge *u, const gbx_spinor *out, const gbx_spinor *in, int dag);
+
 /* ---------------------------------------------------------------------- */
 /* HBM bandwidth probe: device-to-device copy of `bytes` with the          */
 /* library's own streaming kernel; returns GB/s (1e9 B/s) counting read + */
