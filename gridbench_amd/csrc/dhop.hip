@@ -392,6 +392,10 @@ int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, in
     GBX_CHECK(whole, GBX_EUNSUPPORTED, "dhop: the tiled kernel (kind 2) runs on the whole lattice only");
     return tile_launch(u, out, in, dag, k, st);
   }
+  if (k.kind == 3) {
+    GBX_CHECK(whole, GBX_EUNSUPPORTED, "dhop: the slab-marching kernel (kind 3) runs on the whole lattice only");
+    return march_launch(u, out, in, dag, k, st);
+  }
   GBX_CHECK(k.kind == 0, GBX_EINVAL, "dhop: kernel kind %d", k.kind);
   return dhop_launch(u, out, in, dag, from_kcfg(k), 0.0, 1.0, nullptr, st, false, range);
 }
@@ -422,7 +426,7 @@ gbx_kcfg dhop_range_cfg(const gbx_gauge_s *u, const gbx_spinor_s *in)
 {
   gbx_kcfg k;
   gbx_dhop_get_cfg(const_cast<gbx_gauge_s *>(u), const_cast<gbx_spinor_s *>(in), &k);
-  if (k.kind == 2) k = to_kcfg(dhop_default_cfg(u, in));  // the tiled kernel covers whole lattices only
+  if (k.kind >= 2) k = to_kcfg(dhop_default_cfg(u, in));  // the tiled and marching kernels cover whole lattices only
   return k;
 }
 

@@ -211,6 +211,11 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
 int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
 void tile_tables_free(gbx_geom_s *g);
 int pack_tile(const int e[4]);
+// Slab-marching kernel (kind 3): out = Dhop(in) on a whole lattice; tile
+// extents e (e[md] = slab thickness) and march dimension md packed by pack_march.
+int march_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *in);
+int march_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st);
+int pack_march(const int e[4], int md);
 // Any kernel kind (VALU or MFMA) with an explicit configuration.
 int dhop_run(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k, hipStream_t st,
              SlotRange range = SlotRange());

@@ -95,6 +95,33 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
       }
     gbx_clear_error();
   }
+  if (kinds & GBX_KIND_MARCH) {
+    // Slab-marching kernel: march along t or z; slabs of 1, 2 or 4 sites;
+    // tiles of 1..8 sites over the other three dimensions; whole or half
+    // segments; minor chunks that make whole wavefronts per tile site.
+    const int *L = u->geom->dims;
+    static const int t3[][3] = {{1, 1, 1}, {2, 1, 1}, {1, 2, 1}, {2, 2, 1}, {1, 2, 2}, {2, 1, 2}, {4, 1, 1},
+                                {1, 4, 1}, {2, 2, 2}, {4, 2, 1}, {2, 4, 1}};
+    std::vector<int> mcs{0};
+    for (int mc : minor_chunks(M))
+      if (mc >= 16) mcs.push_back(mc);
+    for (int md : {3, 2})
+      for (int ts : {1, 2, 4})
+        for (const auto &t : t3)
+          for (int mc : mcs)
+            for (int half = 0; half <= 1; half++) {
+              int e[4], j = 0;
+              for (int d = 0; d < 4; d++) e[d] = d == md ? ts : t[j++];
+              const int seg = half ? L[md] / 2 : 0;
+              if (half && (L[md] % 2 || (L[md] / 2) % ts)) continue;
+              const uint32_t mw = mc ? (uint32_t)mc : M;
+              const int thr = t[0] * t[1] * t[2] * ts * (int)mw;
+              gbx_kcfg k{0, thr, 1, 1, 3, mc, 0, 0, seg, pack_march(e, md)};
+              if (march_cfg_check(k, u, in) != GBX_OK) continue;
+              v.push_back(k);
+            }
+    gbx_clear_error();
+  }
   if ((kinds & GBX_KIND_MFMA) && mfma_supported(u, in)) {
     // MFMA form: minor columns per wavefront unit (multiples of 16 dividing
     // M), sites per workgroup, wavefronts per workgroup, next-tile gather.
@@ -127,6 +154,17 @@ std::vector<gbx_kcfg> refine_candidates(const gbx_kcfg &b)
       }
   if (b.kind == 2) {  // tile kernel: XCD order, store kind, residency cap
     for (gbx_kcfg &k : v) k.wgpc = k.wgpc > 4 ? 0 : k.wgpc;
+    return v;
+  }
+  if (b.kind == 3) {  // marching kernel: its LDS sets the residency; XCD order and store kind
+    v.clear();
+    for (int xcd = 0; xcd <= 1; xcd++)
+      for (int nt = 0; nt <= 1; nt++) {
+        gbx_kcfg k = b;
+        k.xcd = xcd;
+        k.nt = nt;
+        v.push_back(k);
+      }
     return v;
   }
   if (b.mc > 0 && b.kind == 0)  // chunk-inner visiting order: site groups per chunk cycle
@@ -162,14 +200,15 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
 {
   GBX_CHECK(u && out && in && best && best_usec, GBX_ENULL, "gbx_tune: NULL argument");
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_tune: dag=%d", dag);
-  GBX_CHECK(kinds >= 1 && kinds <= 7, GBX_EINVAL, "gbx_tune: kinds=%d", kinds);
-  const bool all = kinds == (GBX_KIND_VALU | GBX_KIND_MFMA | GBX_KIND_TILE);
+  GBX_CHECK(kinds >= 1 && kinds <= GBX_KIND_ALL, GBX_EINVAL, "gbx_tune: kinds=%d", kinds);
+  const bool all = kinds == GBX_KIND_ALL;
   std::string sig = dhop_signature(u, in);
   if (!all) {
     sig += "#";
     if (kinds & GBX_KIND_VALU) sig += "valu";
     if (kinds & GBX_KIND_MFMA) sig += "mfma";
     if (kinds & GBX_KIND_TILE) sig += "tile";
+    if (kinds & GBX_KIND_MARCH) sig += "march";
   }
   *best = gbx_kcfg{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
   *best_usec = 0.0;
@@ -183,7 +222,7 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
         *best = k;
         *best_usec = us;
       }
-    if (best->threads > 0 && (best->spw > 0 || best->kind == 2)) {
+    if (best->threads > 0 && (best->spw > 0 || best->kind >= 2)) {
       // validate against this build before trusting it
       double us2;
       if (time_cfg(u, out, in, dag, *best, 2, &us2) == GBX_OK) {
@@ -238,5 +277,5 @@ extern "C" int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int da
 extern "C" int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
                         double *best_usec)
 {
-  return gbx_tune_kinds(u, out, in, dag, GBX_KIND_VALU | GBX_KIND_MFMA | GBX_KIND_TILE, cache_path, best, best_usec);
+  return gbx_tune_kinds(u, out, in, dag, GBX_KIND_ALL, cache_path, best, best_usec);
 }

@@ -452,6 +452,20 @@ def pack_tile(ext):
     return int(ext[0]) | (int(ext[1]) << 4) | (int(ext[2]) << 8) | (int(ext[3]) << 12)
 
 
+def pack_march(ext, md):
+    """Kind 3: extents of the tile (three dimensions) and of the slab along the
+    march dimension md (each 1..15), in gbx_kcfg.tile's packed form."""
+    return pack_tile(ext) | (int(md) << 16)
+
+
+def march_cfg(ext, md, mc, seg=0, xcd=1, nt=1):
+    """Kind 3 configuration: threads = tile sites x slab x minor chunk."""
+    n = 1
+    for d in range(4):
+        n *= int(ext[d])
+    return KCfg(0, n * int(mc), xcd, nt, 3, int(mc), 0, 0, int(seg), pack_march(ext, md))
+
+
 def kcfg(spw, threads, xcd=1, nt=1, kind=0, mc=0, pf=0, wgpc=0, cblk=0, tile=0):
     """Kernel configuration; kind 2 (LDS-tiled) takes tile = 4D box extents."""
     if not isinstance(tile, int):
@@ -626,7 +640,7 @@ def tune(U, out, inp, dag=False, cache_path=None, kinds=None):
         check(lib.gbx_tune(U.h, out.h, inp.h, int(bool(dag)), cp, C.byref(best), C.byref(us)), "gbx_tune")
     else:
         names = [kinds] if isinstance(kinds, str) else list(kinds)
-        k = sum({"valu": 1, "mfma": 2, "tile": 4}[n] for n in names)
+        k = sum({"valu": 1, "mfma": 2, "tile": 4, "march": 8}[n] for n in names)
         check(lib.gbx_tune_kinds(U.h, out.h, inp.h, int(bool(dag)), k, cp, C.byref(best), C.byref(us)),
               "gbx_tune_kinds")
     return best, us.value
@@ -891,6 +905,7 @@ __all__ = [
     "BASIS_CHIRAL", "BASIS_TDIAG", "FLOPS_PER_SITE", "F64", "F32", "F16", "BF16", "PREC_NAMES", "GbxError", "device_count", "gen_gaussian",
     "gen_su3_links", "double_links", "gen_doubled_gauge", "gen_spinor", "table_generate", "table_recognise",
     "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
+    "pack_march", "march_cfg",
     "dhop", "dhop_axpby",
     "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "batch_run_noalloc", "NOALLOC_MAX_LATTICES", "stream_copy_bw", "stream_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",

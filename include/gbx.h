@@ -307,7 +307,9 @@ typedef struct {
   int xcd;     /* 1: give each XCD a contiguous range of sites           */
   int nt;      /* 1: non-temporal output stores                          */
   int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS),   */
-               /* 2: LDS-tiled VALU kernel (whole lattices)              */
+               /* 2: LDS-tiled VALU kernel (whole lattices), 3: slab-    */
+               /* marching VALU kernel (whole lattices, FP64 / FP32,     */
+               /* 18-real links)                                         */
   int mc;      /* minor indices (s, rhs) per workgroup; 0 = all Ls*nrhs  */
   int pf;      /* 1 (2): gather of leg d+1 (d+2) issued before leg d is  */
                /* computed; VALU kernel, workgroups of <= 256 threads    */
@@ -316,10 +318,16 @@ typedef struct {
                /* set of sites in flight per XCD to what its L2 holds    */
   int cblk;    /* with mc < Ls*nrhs: workgroup site-groups visited for   */
                /* every minor chunk before moving on (0 = one sweep of   */
-               /* the lattice per chunk); keeps a block's links in L2    */
+               /* the lattice per chunk); keeps a block's links in L2.   */
+               /* Kind 3: sites of the march dimension one workgroup     */
+               /* covers (0 = its whole extent)                          */
   int tile;    /* kind 2: 4D box a workgroup owns, extents packed as     */
                /* x | y << 4 | z << 8 | t << 12 (each 1..15, dividing    */
-               /* the lattice); legs inside the box read LDS             */
+               /* the lattice); legs inside the box read LDS.  Kind 3:   */
+               /* the same packing of a tile over three dimensions and  */
+               /* the slab thickness along the march dimension md, with  */
+               /* md << 16; threads = tile sites x slab x minor chunk,   */
+               /* slab x minor chunk a multiple of 64                    */
 } gbx_kcfg;
 int gbx_dhop_cfg(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg);
 /* gbx_dhop_xpay with an explicit VALU-kernel configuration (kind 0).     */
@@ -352,12 +360,15 @@ int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx
 int gbx_tune(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const char *cache_path, gbx_kcfg *best,
              double *best_usec);
 /* The same search restricted to kernel kinds: bit 0 = VALU (kind 0), bit 1 */
-/* = MFMA (kind 1), bit 2 = LDS-tiled (kind 2); kinds = 7 is gbx_tune.  A restricted result is cached  */
+/* = MFMA (kind 1), bit 2 = LDS-tiled (kind 2), bit 3 = slab-marching     */
+/* (kind 3); GBX_KIND_ALL is gbx_tune.  A restricted result is cached      */
 /* under its own key and does not change what gbx_dhop uses.  For          */
-/* comparing the two forms on one problem.                                 */
+/* comparing the forms on one problem.                                     */
 #define GBX_KIND_VALU 1
 #define GBX_KIND_MFMA 2
 #define GBX_KIND_TILE 4
+#define GBX_KIND_MARCH 8
+#define GBX_KIND_ALL 15
 int gbx_tune_kinds(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, int kinds, const char *cache_path,
                    gbx_kcfg *best, double *best_usec);
 
