@@ -492,6 +492,8 @@ def batch_section(gbx, nrep):
         y = gbx.SpinorField(g, Ls, nrhs, "f32")
         lats.append((g, dbl, U, x, y))
     b = gbx.Batch([l[2] for l in lats], [l[4] for l in lats], [l[3] for l in lats])
+    us_default = b.time(3)
+    bcfg, _ = b.tune(nrep=3)  # threads, gather depth, workgroups per CU, points per item
     items, wgs = b.info()
     b.run()
     us = b.time(nrep)
@@ -508,7 +510,10 @@ def batch_section(gbx, nrep):
             f.close()
     b.close()
     r = {"lattices": [{"L": L, "Ls": Ls, "nrhs": n} for L, Ls, n in specs], "spin_basis": "tdiag", "work_items": items,
-         "persistent_workgroups": wgs, "us_per_launch": round(us, 2), "gflops": round(flop / (us * 1e-6) / 1e9, 1),
+         "persistent_workgroups": wgs, "launch_shape": {k: v for k, v in bcfg.as_dict().items()
+                                                         if k in ("threads", "pf", "wgpc", "nt", "spw")},
+         "us_per_launch_untuned_shape": round(us_default, 2),
+         "us_per_launch": round(us, 2), "gflops": round(flop / (us * 1e-6) / 1e9, 1),
          "hbm_gbps_alg": round(byts / (us * 1e-6) / 1e9, 1), "rel_l2_vs_f64_per_lattice": errs,
          "max_rel_l2_vs_f64": max(errs)}
     log(f"batch: {us:.1f} us, {r['gflops']} GFLOP/s, max err {max(errs):.2e}")

@@ -172,6 +172,9 @@ PROTOTYPES = {
     "gbx_batch_run": (i32, [vp]),
     "gbx_batch_time": (i32, [vp, i32, pdbl]),
     "gbx_batch_info": (i32, [vp, pu64, pi32]),
+    "gbx_batch_get_cfg": (i32, [vp, C.POINTER(KCfg)]),
+    "gbx_batch_set_cfg": (i32, [vp, C.POINTER(KCfg)]),
+    "gbx_batch_tune": (i32, [vp, i32, C.POINTER(KCfg), pdbl]),
     "gbx_batch_run_noalloc": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32]),
     "gbx_stream_copy_bw": (i32, [u64, i32, pdbl]),
     "gbx_stream_bw": (i32, [u64, i32, i32, pdbl]),

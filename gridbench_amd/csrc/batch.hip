@@ -41,7 +41,7 @@ constexpr int kThreads = 256;
 __global__ void k_reset(uint32_t *c) { c[0] = 0u; }
 
 // One work item: sites q0 .. q0+nsl-1 of lattice L, every minor index.
-template <int P, bool DAG, int PF, bool TD>
+template <int P, bool DAG, int PF, bool TD, int T = kThreads>
 __device__ __forceinline__ void batch_item(const LatDesc &L, uint32_t q0, uint32_t nsl, typename dev::Traits<P>::acc *sU,
                                            uint32_t *sN, int nt)
 {
@@ -49,7 +49,7 @@ __device__ __forceinline__ void batch_item(const LatDesc &L, uint32_t q0, uint32
   if (L.c12) dev::stage_links<P, true>(L.gauge, L.nbr, q0, nsl, sU, sN);
   else dev::stage_links<P, false>(L.gauge, L.nbr, q0, nsl, sU, sN);
   const uint32_t M = L.M, n = nsl * M;
-  for (uint32_t w = threadIdx.x; w < n; w += kThreads) {
+  for (uint32_t w = threadIdx.x; w < n; w += T) {
     const uint32_t sl = w / M, m = w - sl * M;
     A rr[12], ri[12];
     dev::dhop_site<P, DAG, PF, false, TD>(L.in, sN + sl * 8, M, m, sU + sl * 8 * 18, rr, ri);
@@ -57,10 +57,9 @@ __device__ __forceinline__ void batch_item(const LatDesc &L, uint32_t q0, uint32
   }
 }
 
-template <int P, bool DAG, int PF, bool TD = false>
-__global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
-                                                         uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw,
-                                                         int nt)
+template <int P, bool DAG, int PF, bool TD, int T>
+__global__ void __launch_bounds__(T) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
+                                                  uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw, int nt)
 {
   typedef typename dev::Traits<P>::acc A;
   extern __shared__ __align__(16) unsigned char smem[];
@@ -75,7 +74,7 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch(const LatDesc *__restri
     if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);  // claimed now, read after this item
     const uint4 it = items[idx];
     const LatDesc L = lat[it.x];
-    batch_item<P, DAG, PF, TD>(L, it.y, it.z, sU, sN, nt);
+    batch_item<P, DAG, PF, TD, T>(L, it.y, it.z, sU, sN, nt);
     __syncthreads();  // LDS is reused by the next item; s_next is written
     idx = s_next;
   }
@@ -140,11 +139,19 @@ __global__ void __launch_bounds__(kThreads) k_dhop_batch_na(unsigned char *__res
 
 }  // namespace
 
+// Launch shape of the persistent kernel (gbx_batch_tune picks it): threads
+// per workgroup, gather pipeline depth, persistent workgroups per CU,
+// streaming stores, and the (site, minor) points one work item aims at.
+struct BatchShape {
+  int threads = kThreads, pf = 1, wgpc = 0, nt = 1, points = kThreads;
+};
+
 struct gbx_batch_s {
   int prec = 0, dag = 0, n = 0;
   bool td = false;  // every field in the GBX_BASIS_TDIAG spin basis
-  uint32_t nitems = 0, max_spw = 1;
-  int grid = 0;
+  uint32_t nitems = 0, max_spw = 1, cap_items = 0;
+  int grid = 0, ncu = 256;
+  BatchShape shape;
   hipStream_t stream = nullptr;
   LatDesc *d_lat = nullptr;
   uint4 *d_items = nullptr;
@@ -152,16 +159,23 @@ struct gbx_batch_s {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   std::vector<gbx_geom_s *> geoms;
+  std::vector<LatDesc> lat;
   // The fields the graph reads and writes (they must outlive the batch).
   // Every run checks that the inputs are still in the spin basis the graph
   // was captured for and stamps that basis on the outputs it writes.
   std::vector<gbx_spinor_s *> in, out;
   int basis = GBX_BASIS_CHIRAL;
+  void drop_graph()
+  {
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    exec = nullptr;
+    graph = nullptr;
+  }
   ~gbx_batch_s()
   {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (exec) (void)hipGraphExecDestroy(exec);
-    if (graph) (void)hipGraphDestroy(graph);
+    drop_graph();
     (void)hipFree(d_lat);
     (void)hipFree(d_items);
     (void)hipFree(d_counter);
@@ -174,38 +188,120 @@ namespace {
 
 size_t lds_for(int prec, uint32_t spw) { return (size_t)spw * 8 * 18 * (prec == GBX_F64 ? 8 : 4) + (size_t)spw * 8 * 4; }
 
+// Sites per work item for a lattice of minor extent M: about `points`
+// (site, minor) points, at most 64 sites.
+uint32_t item_sites(uint32_t M, int points)
+{
+  uint32_t spw = ((uint32_t)points + M - 1) / M;
+  return spw < 1 ? 1 : spw > 64 ? 64 : spw;
+}
+
+template <int P, bool DAG, bool TD, int T>
+void launch_t(gbx_batch_s *b, hipStream_t st, size_t lds)
+{
+  if (b->shape.pf == 2)
+    hipLaunchKernelGGL((k_dhop_batch<P, DAG, 2, TD, T>), dim3(b->grid), dim3(T), lds, st, (const LatDesc *)b->d_lat,
+                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, b->shape.nt);
+  else
+    hipLaunchKernelGGL((k_dhop_batch<P, DAG, 1, TD, T>), dim3(b->grid), dim3(T), lds, st, (const LatDesc *)b->d_lat,
+                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, b->shape.nt);
+}
+
+template <int P, bool TD>
+void launch_d(gbx_batch_s *b, hipStream_t st, size_t lds)
+{
+  if (b->shape.threads == 512) {
+    if (b->dag) launch_t<P, true, TD, 512>(b, st, lds);
+    else launch_t<P, false, TD, 512>(b, st, lds);
+  } else {
+    if (b->dag) launch_t<P, true, TD, 256>(b, st, lds);
+    else launch_t<P, false, TD, 256>(b, st, lds);
+  }
+}
+
 template <int P>
-void launch(gbx_batch_s *b, hipStream_t st, int nt)
+void launch(gbx_batch_s *b, hipStream_t st)
 {
   const size_t lds = lds_for(P, b->max_spw);
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, b->d_counter);
   if constexpr (P == GBX_F64 || P == GBX_F32) {  // the precisions a TDIAG field can have
     if (b->td) {
-      if (b->dag)
-        hipLaunchKernelGGL((k_dhop_batch<P, true, 1, true>), dim3(b->grid), dim3(kThreads), lds, st,
-                           (const LatDesc *)b->d_lat, (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
-      else
-        hipLaunchKernelGGL((k_dhop_batch<P, false, 1, true>), dim3(b->grid), dim3(kThreads), lds, st,
-                           (const LatDesc *)b->d_lat, (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+      launch_d<P, true>(b, st, lds);
       return;
     }
   }
-  if (b->dag)
-    hipLaunchKernelGGL((k_dhop_batch<P, true, 1>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
-                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
-  else
-    hipLaunchKernelGGL((k_dhop_batch<P, false, 1>), dim3(b->grid), dim3(kThreads), lds, st, (const LatDesc *)b->d_lat,
-                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, nt);
+  launch_d<P, false>(b, st, lds);
 }
 
 void launch_any(gbx_batch_s *b, hipStream_t st)
 {
   switch (b->prec) {
-  case GBX_F64: launch<GBX_F64>(b, st, 1); break;
-  case GBX_F32: launch<GBX_F32>(b, st, 1); break;
-  case GBX_F16: launch<GBX_F16>(b, st, 1); break;
-  default: launch<GBX_BF16>(b, st, 1); break;
+  case GBX_F64: launch<GBX_F64>(b, st); break;
+  case GBX_F32: launch<GBX_F32>(b, st); break;
+  case GBX_F16: launch<GBX_F16>(b, st); break;
+  default: launch<GBX_BF16>(b, st); break;
   }
+}
+
+// Work list for the batch's shape: every lattice cut into items of
+// item_sites(M, points) sites, ordered by decreasing cost (the tail of the
+// queue holds the small ones); uploaded into d_items (grown as needed).
+int build_items(gbx_batch_s *b)
+{
+  struct Item {
+    uint32_t lat, q0, nsl;
+    uint64_t cost;
+  };
+  std::vector<Item> items;
+  uint32_t max_spw = 1;
+  for (int i = 0; i < b->n; i++) {
+    const LatDesc &L = b->lat[i];
+    const uint32_t spw = item_sites(L.M, b->shape.points);
+    max_spw = std::max(max_spw, spw);
+    for (uint32_t q0 = 0; q0 < L.nslot; q0 += spw) {
+      const uint32_t nsl = std::min(spw, L.nslot - q0);
+      items.push_back(Item{(uint32_t)i, q0, nsl, (uint64_t)nsl * L.M});
+    }
+  }
+  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
+  std::vector<uint4> h(items.size());
+  for (size_t k = 0; k < items.size(); k++) h[k] = make_uint4(items[k].lat, items[k].q0, items[k].nsl, 0);
+  if (h.size() > b->cap_items) {
+    (void)hipFree(b->d_items);
+    b->d_items = nullptr;
+    b->cap_items = 0;
+    GBX_HIP(hipMalloc(&b->d_items, sizeof(uint4) * h.size()));
+    b->cap_items = (uint32_t)h.size();
+  }
+  GBX_HIP(hipMemcpy(b->d_items, h.data(), sizeof(uint4) * h.size(), hipMemcpyHostToDevice));
+  b->nitems = (uint32_t)h.size();
+  b->max_spw = max_spw;
+  const int wgpc = b->shape.wgpc > 0 ? b->shape.wgpc : (b->prec == GBX_F64 ? 4 : 6);
+  b->grid = b->ncu * wgpc;
+  return GBX_OK;
+}
+
+// (Re)capture the graph for the current shape: {counter reset, persistent kernel}.
+int capture(gbx_batch_s *b)
+{
+  b->drop_graph();
+  GBX_HIP(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+  launch_any(b, b->stream);
+  hipGraph_t g = nullptr;
+  GBX_HIP(hipStreamEndCapture(b->stream, &g));
+  GBX_CHECK(g != nullptr, GBX_EHIP, "batch: empty graph capture");
+  b->graph = g;
+  GBX_HIP(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+  return GBX_OK;
+}
+
+int shape_check(const BatchShape &sh)
+{
+  GBX_CHECK(sh.threads == 256 || sh.threads == 512, GBX_EINVAL, "batch: threads=%d (256 or 512)", sh.threads);
+  GBX_CHECK(sh.pf == 1 || sh.pf == 2, GBX_EINVAL, "batch: pf=%d (1 or 2)", sh.pf);
+  GBX_CHECK(sh.wgpc >= 0 && sh.wgpc <= 16, GBX_EINVAL, "batch: wgpc=%d", sh.wgpc);
+  GBX_CHECK(sh.points >= 64 && sh.points <= 4096, GBX_EINVAL, "batch: points per item %d", sh.points);
+  return GBX_OK;
 }
 
 }  // namespace
@@ -218,12 +314,6 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   GBX_CHECK(dag == 0 || dag == 1, GBX_EINVAL, "gbx_batch_create: dag=%d", dag);
   const int prec = u[0] ? u[0]->prec : -1;
   std::vector<LatDesc> lat(n);
-  struct Item {
-    uint32_t lat, q0, nsl;
-    uint64_t cost;
-  };
-  std::vector<Item> items;
-  uint32_t max_spw = 1;
   for (int i = 0; i < n; i++) {
     GBX_CHECK(u[i] && out[i] && in[i], GBX_ENULL, "gbx_batch_create: NULL handle at %d", i);
     // the single-lattice entries' checks: allocated fields, geometry, equal
@@ -234,18 +324,9 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     for (int j = 0; j < n; j++)
       GBX_CHECK(out[i]->d != in[j]->d && (j == i || out[i]->d != out[j]->d), GBX_EALIAS,
                 "gbx_batch_create: output %d aliases another lattice's field", i);
-    const uint32_t M = in[i]->M, nslot = (uint32_t)u[i]->geom->vol;
-    uint32_t spw = (kThreads + M - 1) / M;
-    if (spw < 1) spw = 1;
-    if (spw > 64) spw = 64;
-    max_spw = std::max(max_spw, spw);
-    lat[i] = LatDesc{u[i]->d, u[i]->geom->nbr, in[i]->d, out[i]->d, M, nslot, (uint32_t)u[i]->comp12, 0};
-    for (uint32_t q0 = 0; q0 < nslot; q0 += spw) {
-      const uint32_t nsl = std::min(spw, nslot - q0);
-      items.push_back(Item{(uint32_t)i, q0, nsl, (uint64_t)nsl * M});
-    }
+    lat[i] = LatDesc{u[i]->d, u[i]->geom->nbr, in[i]->d, out[i]->d, in[i]->M, (uint32_t)u[i]->geom->vol,
+                     (uint32_t)u[i]->comp12, 0};
   }
-  std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b) { return a.cost > b.cost; });
   GBX_TRY(gbx_require_device());
   gbx_batch_s *b = new gbx_batch_s();
   b->prec = prec;
@@ -255,38 +336,29 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
   b->in.assign(in, in + n);
   b->out.assign(out, out + n);
   b->n = n;
-  b->nitems = (uint32_t)items.size();
-  b->max_spw = max_spw;
+  b->lat = lat;
   for (int i = 0; i < n; i++) {
     geom_retain(u[i]->geom);
     b->geoms.push_back(u[i]->geom);
   }
-  int dev = 0, ncu = 256;
+  int dev = 0;
   (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-  b->grid = ncu * (prec == GBX_F64 ? 4 : 6);
-  std::vector<uint4> hitems(items.size());
-  for (size_t k = 0; k < items.size(); k++) hitems[k] = make_uint4(items[k].lat, items[k].q0, items[k].nsl, 0);
+  (void)hipDeviceGetAttribute(&b->ncu, hipDeviceAttributeMultiprocessorCount, dev);
   auto fail = [&](int s) {
     delete b;
     return s;
   };
   if (hipStreamCreate(&b->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: stream"));
-  if (hipMalloc(&b->d_lat, sizeof(LatDesc) * n) != hipSuccess || hipMalloc(&b->d_items, sizeof(uint4) * hitems.size()) != hipSuccess ||
-      hipMalloc(&b->d_counter, 256) != hipSuccess)
+  if (hipMalloc(&b->d_lat, sizeof(LatDesc) * n) != hipSuccess || hipMalloc(&b->d_counter, 256) != hipSuccess)
     return fail(gbx_fail(GBX_ENOMEM, "gbx_batch_create: device allocation"));
-  if (hipMemcpy(b->d_lat, lat.data(), sizeof(LatDesc) * n, hipMemcpyHostToDevice) != hipSuccess ||
-      hipMemcpy(b->d_items, hitems.data(), sizeof(uint4) * hitems.size(), hipMemcpyHostToDevice) != hipSuccess)
+  if (hipMemcpy(b->d_lat, lat.data(), sizeof(LatDesc) * n, hipMemcpyHostToDevice) != hipSuccess)
     return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: upload"));
+  int rc = build_items(b);
+  if (rc != GBX_OK) return fail(rc);
   // Inputs may still be in flight on their geometries' streams.
   for (gbx_geom_s *g : b->geoms) (void)hipStreamSynchronize(g->stream);
-  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
-    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: begin capture"));
-  launch_any(b, b->stream);
-  if (hipStreamEndCapture(b->stream, &b->graph) != hipSuccess || !b->graph)
-    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: end capture"));
-  if (hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0) != hipSuccess)
-    return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: instantiate"));
+  rc = capture(b);
+  if (rc != GBX_OK) return fail(rc);
   for (int i = 0; i < n; i++) out[i]->basis = in[i]->basis;  // what every run writes
   *bp = b;
   return GBX_OK;
@@ -449,6 +521,91 @@ extern "C" int gbx_batch_time(gbx_batch b, int nrep, double *usec)
   if (e1) (void)hipEventDestroy(e1);
   if (rc == GBX_OK) *usec = 1000.0 * ms / nrep;
   return rc;
+}
+
+namespace {
+
+gbx_kcfg shape_to_kcfg(const BatchShape &sh)
+{
+  gbx_kcfg k{};
+  k.spw = sh.points;
+  k.threads = sh.threads;
+  k.pf = sh.pf;
+  k.wgpc = sh.wgpc;
+  k.nt = sh.nt;
+  return k;
+}
+
+int apply_shape(gbx_batch_s *b, const BatchShape &sh)
+{
+  GBX_TRY(shape_check(sh));
+  GBX_HIP(hipStreamSynchronize(b->stream));  // no replay may still read the old work list
+  b->shape = sh;
+  GBX_TRY(build_items(b));
+  return capture(b);
+}
+
+}  // namespace
+
+extern "C" int gbx_batch_set_cfg(gbx_batch b, const gbx_kcfg *cfg)
+{
+  GBX_CHECK(b && cfg, GBX_ENULL, "gbx_batch_set_cfg: NULL argument");
+  BatchShape sh;
+  sh.points = cfg->spw;
+  sh.threads = cfg->threads;
+  sh.pf = cfg->pf;
+  sh.wgpc = cfg->wgpc;
+  sh.nt = cfg->nt;
+  return apply_shape(b, sh);
+}
+
+extern "C" int gbx_batch_get_cfg(gbx_batch b, gbx_kcfg *cfg)
+{
+  GBX_CHECK(b && cfg, GBX_ENULL, "gbx_batch_get_cfg: NULL argument");
+  *cfg = shape_to_kcfg(b->shape);
+  return GBX_OK;
+}
+
+extern "C" int gbx_batch_tune(gbx_batch b, int nrep, gbx_kcfg *best, double *best_usec)
+{
+  GBX_CHECK(b && best && best_usec, GBX_ENULL, "gbx_batch_tune: NULL argument");
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_batch_tune: nrep=%d", nrep);
+  GBX_TRY(batch_check_basis(b, "gbx_batch_tune"));
+  std::vector<BatchShape> cands;
+  for (int threads : {256, 512})
+    for (int pf : {1, 2})
+      for (int wgpc : {2, 3, 4, 6, 8})
+        for (int mult : {1, 2}) {
+          BatchShape sh;
+          sh.threads = threads;
+          sh.pf = pf;
+          sh.wgpc = wgpc;
+          sh.points = threads * mult;
+          sh.nt = 1;
+          cands.push_back(sh);
+        }
+  double bus = 1e300;
+  BatchShape bsh = b->shape;
+  auto trial = [&](const BatchShape &sh) {
+    double us = 0.0;
+    if (apply_shape(b, sh) != GBX_OK || gbx_batch_time(b, nrep, &us) != GBX_OK) {
+      gbx_clear_error();
+      return;
+    }
+    if (us < bus) {
+      bus = us;
+      bsh = sh;
+    }
+  };
+  for (const BatchShape &sh : cands) trial(sh);
+  GBX_CHECK(bus < 1e300, GBX_EHIP, "gbx_batch_tune: every candidate failed");
+  BatchShape alt = bsh;
+  alt.nt = 0;
+  trial(alt);
+  GBX_TRY(apply_shape(b, bsh));
+  *best = shape_to_kcfg(bsh);
+  *best_usec = bus;
+  return GBX_OK;
 }
 
 extern "C" int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups)

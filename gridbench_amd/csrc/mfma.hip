@@ -63,13 +63,16 @@ __device__ __forceinline__ void gval(int mu, int a, int &re, int &im)
 }
 
 // Work unit of a wavefront: one site of the workgroup x `mw` minor columns
-// (mw a multiple of 16).  Workgroup = `spw` sites; its waves take units
-// round-robin, site-major, and rebuild A only when the site changes.
+// (mw a multiple of 16).  Workgroup = `spw` sites x one sweep chunk of `mcs`
+// columns; its waves take units round-robin, site-major, and rebuild A only
+// when the site changes.  With mcs < M the lattice is swept once per chunk
+// (chunk-outer order): each sweep's working set is an mcs-column field, so
+// more sites' neighbours fit an XCD's L2 (the links are staged once per chunk).
 template <int P, bool DAG, bool C12, bool PF>
 __global__ void __launch_bounds__(256)
     k_dhop_mfma(const void *__restrict__ gauge, const uint32_t *__restrict__ nbr, const void *__restrict__ in,
-                void *__restrict__ out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t nwg,
-                uint32_t per_xcd, int xcd, int nt)
+                void *__restrict__ out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t mcs,
+                uint32_t nwg_sites, uint32_t nwg, uint32_t per_xcd, int xcd, int nt)
 {
   typedef MTraits<P> T;
   typedef typename T::V8 V8;
@@ -81,7 +84,7 @@ __global__ void __launch_bounds__(256)
 
   const uint32_t L = xcd ? (blockIdx.x & 7u) * per_xcd + (blockIdx.x >> 3) : blockIdx.x;
   if (L >= nwg) return;  // uniform over the workgroup, before the barrier
-  const uint32_t q0 = L * spw;
+  const uint32_t chunk = L / nwg_sites, q0 = (L - chunk * nwg_sites) * spw;
   const uint32_t nsl = min(spw, nslot - q0);
   dev::stage_links<P, C12>(gauge, nbr, q0, nsl, sU, sN);
 
@@ -108,7 +111,7 @@ __global__ void __launch_bounds__(256)
     }
   }
 
-  const uint32_t cpw = M / mw;  // wave units per site
+  const uint32_t cpw = mcs / mw;  // wave units per site
   const uint32_t units = nsl * cpw;
   const V8 *vin = (const V8 *)in;
   V4 *vout = (V4 *)out;
@@ -116,7 +119,7 @@ __global__ void __launch_bounds__(256)
   V8 Af[2][6];
   uint32_t nb0 = 0, nb1 = 0;
   for (uint32_t w = wave; w < units; w += nwave) {
-    const uint32_t sl = w / cpw, m0 = (w - sl * cpw) * mw;
+    const uint32_t sl = w / cpw, m0 = chunk * mcs + (w - sl * cpw) * mw;
     if (sl != cur) {
       cur = sl;
       nb0 = sN[sl * 8 + 2 * g];
@@ -199,26 +202,29 @@ __global__ void __launch_bounds__(256)
 
 template <int P, bool DAG, bool C12>
 void launch_pf(bool pf, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge, const uint32_t *nbr,
-               const void *in, void *out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t nwg,
-               uint32_t per_xcd, int xcd, int nt)
+               const void *in, void *out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw, uint32_t mcs,
+               uint32_t nwg_sites, uint32_t nwg, uint32_t per_xcd, int xcd, int nt)
 {
   if (pf)
-    hipLaunchKernelGGL((k_dhop_mfma<P, DAG, C12, true>), grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg,
-                       per_xcd, xcd, nt);
+    hipLaunchKernelGGL((k_dhop_mfma<P, DAG, C12, true>), grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, mcs,
+                       nwg_sites, nwg, per_xcd, xcd, nt);
   else
     hipLaunchKernelGGL((k_dhop_mfma<P, DAG, C12, false>), grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw,
-                       nwg, per_xcd, xcd, nt);
+                       mcs, nwg_sites, nwg, per_xcd, xcd, nt);
 }
 
 template <int P>
 void launch_prec(bool dag, bool c12, bool pf, dim3 grid, dim3 block, size_t lds, hipStream_t st, const void *gauge,
                  const uint32_t *nbr, const void *in, void *out, uint32_t nslot, uint32_t M, uint32_t spw, uint32_t mw,
-                 uint32_t nwg, uint32_t per_xcd, int xcd, int nt)
+                 uint32_t mcs, uint32_t nwg_sites, uint32_t nwg, uint32_t per_xcd, int xcd, int nt)
 {
-  if (!dag && !c12) launch_pf<P, false, false>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
-  else if (!dag) launch_pf<P, false, true>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
-  else if (!c12) launch_pf<P, true, false>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
-  else launch_pf<P, true, true>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, nwg, per_xcd, xcd, nt);
+#define GBX_MF(D_, C_) \
+  launch_pf<P, D_, C_>(pf, grid, block, lds, st, gauge, nbr, in, out, nslot, M, spw, mw, mcs, nwg_sites, nwg, per_xcd, xcd, nt)
+  if (!dag && !c12) GBX_MF(false, false);
+  else if (!dag) GBX_MF(false, true);
+  else if (!c12) GBX_MF(true, false);
+  else GBX_MF(true, true);
+#undef GBX_MF
 }
 
 }  // namespace
@@ -230,8 +236,10 @@ int mfma_supported(const gbx_gauge_s *u, const gbx_spinor_s *in)
 
 // kind = 1 configuration: spw sites per workgroup, threads = 64 .. 256
 // (one wavefront per 64), mc = minor columns per wavefront unit (a multiple
-// of 16 dividing M; 0 = all of M), pf = gather the next 16 columns before
-// the current MFMAs, xcd / nt / wgpc as for the VALU kernel (cblk unused).
+// of 16 dividing M; 0 = all of M), cblk = columns per lattice sweep (a
+// multiple of the unit dividing M; 0 = all of M, one sweep), pf = gather the
+// next 16 columns before the current MFMAs, xcd / nt / wgpc as for the VALU
+// kernel.
 int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &c, hipStream_t st,
                 SlotRange range)
 {
@@ -245,6 +253,9 @@ int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   const uint32_t mw = c.mc <= 0 ? in->M : (uint32_t)c.mc;
   GBX_CHECK(mw % 16 == 0 && in->M % mw == 0, GBX_EINVAL, "dhop (MFMA): mc=%d must be a multiple of 16 dividing %u", c.mc,
             in->M);
+  const uint32_t mcs = c.cblk <= 0 ? in->M : (uint32_t)c.cblk;
+  GBX_CHECK(mcs % mw == 0 && in->M % mcs == 0, GBX_EINVAL,
+            "dhop (MFMA): cblk=%d must be a multiple of the unit's %u columns dividing %u", c.cblk, mw, in->M);
   GBX_CHECK(c.wgpc >= 0 && c.wgpc <= 64, GBX_EINVAL, "dhop (MFMA): wgpc=%d", c.wgpc);
   size_t lds = (size_t)c.spw * 8 * 18 * 4 + (size_t)c.spw * 8 * 4;
   GBX_CHECK(lds <= 160 * 1024, GBX_EINVAL, "dhop (MFMA): spw=%d needs %zu B of LDS", c.spw, lds);
@@ -256,7 +267,10 @@ int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   GBX_CHECK(q0 <= q1, GBX_EINVAL, "dhop (MFMA): slot range %llu .. %llu", (unsigned long long)q0, (unsigned long long)q1);
   if (q0 == q1) return GBX_OK;
   const uint32_t nslot = (uint32_t)(q1 - q0);
-  const uint32_t nwg = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
+  const uint32_t nwg_sites = (nslot + (uint32_t)c.spw - 1) / (uint32_t)c.spw;
+  const uint64_t nwg64 = (uint64_t)nwg_sites * (in->M / mcs);
+  GBX_CHECK(nwg64 < (1ull << 31), GBX_EINVAL, "dhop (MFMA): %llu workgroups", (unsigned long long)nwg64);
+  const uint32_t nwg = (uint32_t)nwg64;
   const uint32_t per_xcd = (nwg + 7) / 8;
   const dim3 grid(c.xcd ? per_xcd * 8 : nwg), block((unsigned)c.threads);
   const bool c12 = u->comp12 != 0, pf = c.pf != 0;
@@ -267,10 +281,10 @@ int mfma_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
   void *dout = (char *)out->d + q0 * (uint64_t)in->M * 24 * pb;
   if (in->prec == GBX_F16)
     launch_prec<GBX_F16>(dag != 0, c12, pf, grid, block, lds, st, gauge, nbr, in->d, dout, nslot, in->M, (uint32_t)c.spw,
-                         mw, nwg, per_xcd, c.xcd, c.nt);
+                         mw, mcs, nwg_sites, nwg, per_xcd, c.xcd, c.nt);
   else
     launch_prec<GBX_BF16>(dag != 0, c12, pf, grid, block, lds, st, gauge, nbr, in->d, dout, nslot, in->M, (uint32_t)c.spw,
-                          mw, nwg, per_xcd, c.xcd, c.nt);
+                          mw, mcs, nwg_sites, nwg, per_xcd, c.xcd, c.nt);
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }

@@ -130,7 +130,11 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
       for (int spw : {1, 2, 4, 8})
         for (int t : {128, 256}) {
           if ((uint64_t)spw * (M / mw) * 64 < (uint64_t)t) continue;  // fewer units than wavefronts
-          for (int pf = 0; pf <= 1; pf++) v.push_back(gbx_kcfg{spw, t, 1, 1, 1, mw == M ? 0 : (int)mw, pf, 0, 0});
+          for (int pf = 0; pf <= 1; pf++)
+            for (uint32_t sw = M; sw >= mw && sw * 4 >= M; sw /= 2) {  // columns per lattice sweep
+              if (M % sw || sw % mw) continue;
+              v.push_back(gbx_kcfg{spw, t, 1, 1, 1, mw == M ? 0 : (int)mw, pf, 0, sw == M ? 0 : (int)sw, 0});
+            }
         }
     }
   }

@@ -848,6 +848,22 @@ class Batch:
         check(lib.gbx_batch_info(self.h, C.byref(items), C.byref(wgs)), "gbx_batch_info")
         return items.value, wgs.value
 
+    def get_cfg(self):
+        """Launch shape: threads, pf, wgpc, nt, spw = points per work item."""
+        k = KCfg()
+        check(lib.gbx_batch_get_cfg(self.h, C.byref(k)), "gbx_batch_get_cfg")
+        return k
+
+    def set_cfg(self, cfg):
+        check(lib.gbx_batch_set_cfg(self.h, C.byref(cfg)), "gbx_batch_set_cfg")
+        return self
+
+    def tune(self, nrep=3):
+        """Time launch shapes on these fields and keep the fastest; returns (cfg, us)."""
+        k, us = KCfg(), C.c_double()
+        check(lib.gbx_batch_tune(self.h, int(nrep), C.byref(k), C.byref(us)), "gbx_batch_tune")
+        return k, us.value
+
     def close(self):
         if self.h:
             lib.gbx_batch_destroy(self.h)

@@ -556,6 +556,18 @@ int gbx_batch_run(gbx_batch b);
 int gbx_batch_time(gbx_batch b, int nrep, double *usec);
 /* Work items and persistent workgroups of the launch. */
 int gbx_batch_info(gbx_batch b, uint64_t *items, int *workgroups);
+/* Launch shape of the persistent kernel, in gbx_kcfg fields: threads (256 */
+/* or 512 per workgroup), pf (gather pipeline depth 1 or 2), wgpc          */
+/* (persistent workgroups per CU; 0 = 4 for FP64, 6 otherwise), nt         */
+/* (streaming stores) and spw (the (site, minor) points a work item aims   */
+/* at: a lattice of minor extent M is cut into items of ceil(spw / M)      */
+/* sites, at most 64).  gbx_batch_create starts at threads 256, pf 1,     */
+/* wgpc 0, nt 1, spw 256.  Setting a shape recaptures the graph.           */
+int gbx_batch_get_cfg(gbx_batch b, gbx_kcfg *cfg);
+int gbx_batch_set_cfg(gbx_batch b, const gbx_kcfg *cfg);
+/* Times shapes (threads, pf, wgpc, points per item, then the store kind)  */
+/* on the batch's own fields, nrep replays each, and keeps the fastest.    */
+int gbx_batch_tune(gbx_batch b, int nrep, gbx_kcfg *best, double *best_usec);
 
 /* No counterpart in the reference.  For callers that cannot allocate     */
 /* device memory -- code under HIP graph capture, tight serving loops,     */

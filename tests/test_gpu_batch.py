@@ -127,3 +127,38 @@ def test_batch_checks_each_lattice_like_the_single_entries(gpu, gbx):
     with pytest.raises(gbx.GbxError) as e:
         gbx.Batch([U], [bad], [x])
     assert e.value.status == 9
+
+
+@pytest.mark.parametrize("prec", ["f32", "f64"])
+def test_batch_launch_shapes_and_tuner(gpu, gbx, prec):
+    """Every launch shape of the persistent kernel (threads, gather depth,
+    persistent workgroups per CU, points per work item, store kind) computes
+    the same bits as the single-lattice kernel; the tuner keeps one of them."""
+    objs = build(gbx, prec, LATTICES)
+    b = gbx.Batch([o[1] for o in objs], [o[3] for o in objs], [o[2] for o in objs])
+    want = []
+    for g, U, x, y, dbl, psi in objs:
+        ref = gbx.SpinorField(g, x.Ls, x.nrhs, prec)
+        gbx.dhop(U, ref, x, cfg=gbx.kcfg(1, 64, 0, 0))
+        want.append(ref.download())
+    assert b.get_cfg().threads == 256
+    for threads, pf, wgpc, points, nt in ((512, 2, 3, 1024, 0), (256, 2, 8, 256, 1), (512, 1, 2, 512, 1),
+                                          (256, 1, 0, 64, 0)):
+        b.set_cfg(gbx.kcfg(points, threads, 0, nt, pf=pf, wgpc=wgpc))
+        for o in objs:
+            o[3].zero()
+        b.run()
+        for o, w in zip(objs, want):
+            assert np.array_equal(o[3].download(), w), (threads, pf, wgpc, points, nt)
+    cfg, us = b.tune(nrep=2)
+    assert us > 0 and cfg.threads in (256, 512) and cfg.pf in (1, 2)
+    assert b.get_cfg().as_dict() == cfg.as_dict()
+    for o in objs:
+        o[3].zero()
+    b.run()
+    for o, w in zip(objs, want):
+        assert np.array_equal(o[3].download(), w)
+    for bad in (gbx.kcfg(256, 128, pf=1), gbx.kcfg(256, 256, pf=0), gbx.kcfg(32, 256, pf=1), gbx.kcfg(256, 256, pf=1, wgpc=17)):
+        with pytest.raises(gbx.GbxError) as e:
+            b.set_cfg(bad)
+        assert e.value.status == 1

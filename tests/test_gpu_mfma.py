@@ -62,18 +62,20 @@ def test_mfma_compressed_links(gpu, gbx, problem, prec, dag):
         assert rel_l2(mf[r], p.oracle_lex(rhs=r, dag=dag)) <= storage_tol(prec, p.psi)
 
 
-@pytest.mark.parametrize("spw,threads,xcd,nt,mc,pf,wgpc", [
-    (1, 64, 0, 0, 0, 0, 0), (1, 256, 1, 1, 16, 1, 0), (3, 192, 1, 1, 32, 1, 0), (7, 128, 0, 1, 64, 0, 4),
-    (16, 256, 1, 0, 16, 1, 2), (5, 64, 1, 1, 32, 0, 0)])
-def test_mfma_launch_shapes_identical(gpu, gbx, problem, spw, threads, xcd, nt, mc, pf, wgpc):
-    """Every launch shape computes each output element by the same sequence of
-    MFMAs, so the results are bit-identical across shapes."""
+@pytest.mark.parametrize("spw,threads,xcd,nt,mc,pf,wgpc,cblk", [
+    (1, 64, 0, 0, 0, 0, 0, 0), (1, 256, 1, 1, 16, 1, 0, 0), (3, 192, 1, 1, 32, 1, 0, 0), (7, 128, 0, 1, 64, 0, 4, 0),
+    (16, 256, 1, 0, 16, 1, 2, 0), (5, 64, 1, 1, 32, 0, 0, 0), (4, 256, 1, 1, 16, 1, 0, 32), (3, 128, 1, 0, 16, 0, 0, 16),
+    (2, 64, 0, 1, 32, 1, 0, 32)])
+def test_mfma_launch_shapes_identical(gpu, gbx, problem, spw, threads, xcd, nt, mc, pf, wgpc, cblk):
+    """Every launch shape -- including lattice sweeps over column chunks
+    (cblk) -- computes each output element by the same sequence of MFMAs, so
+    the results are bit-identical across shapes."""
     p = problem((6, 4, 4, 4), 16, nrhs=4)
     geom, U, x, y = fields(gbx, p, "f16")
     gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, 0, 0, kind=1))
     base = y.download()
     y.zero()
-    gbx.dhop(U, y, x, cfg=gbx.kcfg(spw, threads, xcd, nt, kind=1, mc=mc, pf=pf, wgpc=wgpc))
+    gbx.dhop(U, y, x, cfg=gbx.kcfg(spw, threads, xcd, nt, kind=1, mc=mc, pf=pf, wgpc=wgpc, cblk=cblk))
     assert np.array_equal(y.download(), base)
 
 
@@ -89,7 +91,8 @@ def test_mfma_rejects_unsupported(gpu, gbx, problem):
         gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, kind=1))
     assert e.value.status == 8
     geom, U, x, y = fields(gbx, p, "f16")
-    for bad in (gbx.kcfg(1, 64, kind=1, mc=8), gbx.kcfg(1, 320, kind=1), gbx.kcfg(1, 64, kind=1, mc=48)):
+    for bad in (gbx.kcfg(1, 64, kind=1, mc=8), gbx.kcfg(1, 320, kind=1), gbx.kcfg(1, 64, kind=1, mc=48),
+                gbx.kcfg(1, 64, kind=1, mc=16, cblk=24)):
         with pytest.raises(gbx.GbxError) as e:
             gbx.dhop(U, y, x, cfg=bad)
         assert e.value.status == 1
