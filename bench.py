@@ -476,9 +476,11 @@ def dwf_section(gbx, tune):
             "b_seed": 501, "system": "D x = b, D = Mooee - Dhop / 2 (Shamir)", "even_odd": rows}
 
 
-def batch_section(gbx, nrep):
+def batch_section(gbx, nrep, shape=None):
     """40 uneven lattices in one persistent graph-captured launch, FP32, on
-    fields in the TDIAG spin basis (errors are basis-independent norms)."""
+    fields in the TDIAG spin basis (errors are basis-independent norms).
+    shape = (threads, pf, wgpc, points) fixes the launch shape instead of
+    tuning it (for profiling)."""
     rng = np.random.default_rng(505)
     specs = [(int(rng.choice([8, 12, 16, 20, 24])), int(rng.choice([8, 12, 16])), int(rng.integers(1, 9)))
              for _ in range(40)]
@@ -493,7 +495,11 @@ def batch_section(gbx, nrep):
         lats.append((g, dbl, U, x, y))
     b = gbx.Batch([l[2] for l in lats], [l[4] for l in lats], [l[3] for l in lats])
     us_default = b.time(3)
-    bcfg, _ = b.tune(nrep=3)  # threads, gather depth, workgroups per CU, points per item
+    if shape:
+        b.set_cfg(gbx.kcfg(shape[3], shape[0], 0, 1, pf=shape[1], wgpc=shape[2]))
+        bcfg = b.get_cfg()
+    else:
+        bcfg, _ = b.tune(nrep=3)  # threads, gather depth, workgroups per CU, points per item
     items, wgs = b.info()
     b.run()
     us = b.time(nrep)
@@ -677,6 +683,8 @@ def main():
     ap.add_argument("--only", default="", help="comma-separated variant names to run (profiling)")
     ap.add_argument("--no-compare", action="store_true", help="skip the MFMA-vs-VALU form comparison (profiling)")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--batch-shape", default="",
+                    help="threads,pf,wgpc,points: the batch's launch shape, not tuned (profiling)")
     ap.add_argument("--tune-cache", default=TUNE_CACHE, help="autotuner cache (JSON lines); '' = tune afresh")
     ap.add_argument("--retune", action="store_true", help="delete the cache file first")
     args = ap.parse_args()
@@ -768,7 +776,8 @@ def main():
         extra["cg_even_odd"] = cg_eo_section(gbx, tune)
         extra["dwf"] = dwf_section(gbx, tune)
     if not args.no_batch:
-        extra["batch"] = batch_section(gbx, 20)
+        shape = tuple(int(v) for v in args.batch_shape.split(",")) if args.batch_shape else None
+        extra["batch"] = batch_section(gbx, 20, shape)
     if not args.no_halo:
         extra["halo"] = halo_section(gbx, tune, dbl, psi, ref64, 100)
     if not args.no_dropin:

@@ -1,16 +1,21 @@
 // Batched hopping term over many independent lattices in one persistent,
 // HIP-graph-captured launch.
 //
-// Work list: every lattice's slot range is cut into items of `spw` sites
-// (spw*M close to one workgroup of work); items are ordered by decreasing
-// cost so the tail of the queue holds the small ones.  A persistent grid of
-// workgroups (a few per CU) pulls items from an atomic counter until the list
-// is exhausted -- every workgroup reaches that exit.  Per item the workgroup
-// stages the item's links into LDS and computes all (site, m) points, exactly
-// like the single-lattice kernel (with its gather pipeline PF).  The next
-// item's index is claimed while the current item is computed, so the
-// atomic's round trip overlaps the work.  The counter is cleared by a
-// one-thread kernel node (no memset nodes in the graph).
+// Work list: one queue per XCD.  Queue x holds, for every lattice in turn,
+// the x-th eighth of its slot range cut into items of `spw` sites (spw*M
+// close to one workgroup of work), in slot order: each XCD's L2 then sees the
+// spatially compact neighbourhoods of the slot order, as the single-lattice
+// kernel's XCD mapping gives it (one queue over all eight XCDs spread
+// consecutive items over different L2s, and most gathers missed).  A
+// persistent grid of workgroups (a few per CU; the hardware deals workgroup b
+// to XCD b % 8) pulls items from its own XCD's queue through an atomic
+// counter, then from the other queues in turn until every queue is exhausted
+// -- every workgroup reaches that exit, and the stealing evens out the tail.
+// Per item the workgroup stages the item's links into LDS and computes all
+// (site, m) points, exactly like the single-lattice kernel (with its gather
+// pipeline PF).  The next item's index is claimed while the current item is
+// computed, so the atomic's round trip overlaps the work.  The counters are
+// cleared by a kernel node (no memset nodes in the graph).
 #include <algorithm>
 #include <cstring>
 #include <vector>
@@ -38,7 +43,24 @@ struct LatDesc {
 
 constexpr int kThreads = 256;
 
-__global__ void k_reset(uint32_t *c) { c[0] = 0u; }
+// The per-XCD queues' counters, 64 bytes apart (their own cache lines).
+constexpr int kQueues = 8, kCtrStride = 16;
+__global__ void k_reset_queues(uint32_t *c)
+{
+  if (threadIdx.x < kQueues) c[threadIdx.x * kCtrStride] = 0u;
+}
+
+// Items [start[q], start[q+1]) form queue q.
+struct QueueBounds {
+  uint32_t start[kQueues + 1];
+};
+
+// Next item of queue q, or UINT32_MAX once it is exhausted.
+__device__ __forceinline__ uint32_t claim(uint32_t *ctr, const QueueBounds &qb, uint32_t q)
+{
+  const uint32_t i = qb.start[q] + atomicAdd(ctr + q * kCtrStride, 1u);
+  return i < qb.start[q + 1] ? i : 0xffffffffu;
+}
 
 // One work item: sites q0 .. q0+nsl-1 of lattice L, every minor index.
 template <int P, bool DAG, int PF, bool TD, int T = kThreads>
@@ -59,24 +81,29 @@ __device__ __forceinline__ void batch_item(const LatDesc &L, uint32_t q0, uint32
 
 template <int P, bool DAG, int PF, bool TD, int T>
 __global__ void __launch_bounds__(T) k_dhop_batch(const LatDesc *__restrict__ lat, const uint4 *__restrict__ items,
-                                                  uint32_t nitems, uint32_t *__restrict__ counter, uint32_t max_spw, int nt)
+                                                  QueueBounds qb, uint32_t *__restrict__ counter, uint32_t max_spw, int nt)
 {
   typedef typename dev::Traits<P>::acc A;
   extern __shared__ __align__(16) unsigned char smem[];
   __shared__ uint32_t s_next;
   A *sU = (A *)smem;
   uint32_t *sN = (uint32_t *)(sU + (size_t)max_spw * 8 * 18);
-  if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);
-  __syncthreads();
-  uint32_t idx = s_next;
-  while (idx < nitems) {  // idx is the same in every thread of the workgroup
-    __syncthreads();      // every thread has read s_next
-    if (threadIdx.x == 0) s_next = atomicAdd(counter, 1u);  // claimed now, read after this item
-    const uint4 it = items[idx];
-    const LatDesc L = lat[it.x];
-    batch_item<P, DAG, PF, TD, T>(L, it.y, it.z, sU, sN, nt);
-    __syncthreads();  // LDS is reused by the next item; s_next is written
-    idx = s_next;
+  const uint32_t x = blockIdx.x % kQueues;  // the XCD this workgroup runs on
+  for (uint32_t k = 0; k < kQueues; k++) {  // its own queue, then the others
+    const uint32_t q = (x + k) % kQueues;
+    if (threadIdx.x == 0) s_next = claim(counter, qb, q);
+    __syncthreads();
+    uint32_t idx = s_next;
+    while (idx != 0xffffffffu) {  // the same in every thread of the workgroup
+      __syncthreads();            // every thread has read s_next
+      if (threadIdx.x == 0) s_next = claim(counter, qb, q);  // claimed now, read after this item
+      const uint4 it = items[idx];
+      const LatDesc L = lat[it.x];
+      batch_item<P, DAG, PF, TD, T>(L, it.y, it.z, sU, sN, nt);
+      __syncthreads();  // LDS is reused by the next item; s_next is written
+      idx = s_next;
+    }
+    __syncthreads();  // every thread has read the exhausted claim before the next queue's
   }
 }
 
@@ -150,6 +177,7 @@ struct gbx_batch_s {
   int prec = 0, dag = 0, n = 0;
   bool td = false;  // every field in the GBX_BASIS_TDIAG spin basis
   uint32_t nitems = 0, max_spw = 1, cap_items = 0;
+  QueueBounds qb{};
   int grid = 0, ncu = 256;
   BatchShape shape;
   hipStream_t stream = nullptr;
@@ -201,10 +229,10 @@ void launch_t(gbx_batch_s *b, hipStream_t st, size_t lds)
 {
   if (b->shape.pf == 2)
     hipLaunchKernelGGL((k_dhop_batch<P, DAG, 2, TD, T>), dim3(b->grid), dim3(T), lds, st, (const LatDesc *)b->d_lat,
-                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, b->shape.nt);
+                       (const uint4 *)b->d_items, b->qb, b->d_counter, b->max_spw, b->shape.nt);
   else
     hipLaunchKernelGGL((k_dhop_batch<P, DAG, 1, TD, T>), dim3(b->grid), dim3(T), lds, st, (const LatDesc *)b->d_lat,
-                       (const uint4 *)b->d_items, b->nitems, b->d_counter, b->max_spw, b->shape.nt);
+                       (const uint4 *)b->d_items, b->qb, b->d_counter, b->max_spw, b->shape.nt);
 }
 
 template <int P, bool TD>
@@ -223,7 +251,7 @@ template <int P>
 void launch(gbx_batch_s *b, hipStream_t st)
 {
   const size_t lds = lds_for(P, b->max_spw);
-  hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, st, b->d_counter);
+  hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, st, b->d_counter);
   if constexpr (P == GBX_F64 || P == GBX_F32) {  // the precisions a TDIAG field can have
     if (b->td) {
       launch_d<P, true>(b, st, lds);
@@ -243,29 +271,25 @@ void launch_any(gbx_batch_s *b, hipStream_t st)
   }
 }
 
-// Work list for the batch's shape: every lattice cut into items of
-// item_sites(M, points) sites, ordered by decreasing cost (the tail of the
-// queue holds the small ones); uploaded into d_items (grown as needed).
+// Work list for the batch's shape: queue x holds, lattice after lattice,
+// the slots [x * nslot / 8, (x + 1) * nslot / 8) of each lattice cut into
+// items of item_sites(M, points) sites, in slot order; uploaded into d_items
+// (grown as needed), queue bounds in qb.
 int build_items(gbx_batch_s *b)
 {
-  struct Item {
-    uint32_t lat, q0, nsl;
-    uint64_t cost;
-  };
-  std::vector<Item> items;
+  std::vector<uint4> h;
   uint32_t max_spw = 1;
-  for (int i = 0; i < b->n; i++) {
-    const LatDesc &L = b->lat[i];
-    const uint32_t spw = item_sites(L.M, b->shape.points);
-    max_spw = std::max(max_spw, spw);
-    for (uint32_t q0 = 0; q0 < L.nslot; q0 += spw) {
-      const uint32_t nsl = std::min(spw, L.nslot - q0);
-      items.push_back(Item{(uint32_t)i, q0, nsl, (uint64_t)nsl * L.M});
+  for (int x = 0; x < kQueues; x++) {
+    b->qb.start[x] = (uint32_t)h.size();
+    for (int i = 0; i < b->n; i++) {
+      const LatDesc &L = b->lat[i];
+      const uint32_t spw = item_sites(L.M, b->shape.points);
+      max_spw = std::max(max_spw, spw);
+      const uint32_t s0 = (uint32_t)((uint64_t)L.nslot * x / kQueues), s1 = (uint32_t)((uint64_t)L.nslot * (x + 1) / kQueues);
+      for (uint32_t q0 = s0; q0 < s1; q0 += spw) h.push_back(make_uint4((uint32_t)i, q0, std::min(spw, s1 - q0), 0));
     }
   }
-  std::stable_sort(items.begin(), items.end(), [](const Item &x, const Item &y) { return x.cost > y.cost; });
-  std::vector<uint4> h(items.size());
-  for (size_t k = 0; k < items.size(); k++) h[k] = make_uint4(items[k].lat, items[k].q0, items[k].nsl, 0);
+  b->qb.start[kQueues] = (uint32_t)h.size();
   if (h.size() > b->cap_items) {
     (void)hipFree(b->d_items);
     b->d_items = nullptr;
@@ -349,7 +373,8 @@ extern "C" int gbx_batch_create(int n, const gbx_gauge *u, const gbx_spinor *out
     return s;
   };
   if (hipStreamCreate(&b->stream) != hipSuccess) return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: stream"));
-  if (hipMalloc(&b->d_lat, sizeof(LatDesc) * n) != hipSuccess || hipMalloc(&b->d_counter, 256) != hipSuccess)
+  if (hipMalloc(&b->d_lat, sizeof(LatDesc) * n) != hipSuccess ||
+      hipMalloc(&b->d_counter, sizeof(uint32_t) * kQueues * kCtrStride) != hipSuccess)
     return fail(gbx_fail(GBX_ENOMEM, "gbx_batch_create: device allocation"));
   if (hipMemcpy(b->d_lat, lat.data(), sizeof(LatDesc) * n, hipMemcpyHostToDevice) != hipSuccess)
     return fail(gbx_fail(GBX_EHIP, "gbx_batch_create: upload"));
