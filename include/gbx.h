@@ -306,7 +306,9 @@ typedef struct {
   int threads; /* workgroup size, multiple of 64, <= 1024               */
   int xcd;     /* 1: give each XCD a contiguous range of sites           */
   int nt;      /* 1: non-temporal output stores                          */
-  int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS),   */
+  int kind;    /* 0: VALU register kernel, 1: MFMA kernel (multi-RHS;    */
+               /* FP16 / BF16 full spinors, FP32 projected half spinors; */
+               /* Ls*nrhs a multiple of 16),                             */
                /* 2: LDS-tiled VALU kernel (whole lattices), 3: slab-    */
                /* marching VALU kernel (whole lattices, FP64 / FP32,     */
                /* 18-real links)                                         */

@@ -218,7 +218,8 @@ def test_f64_kernel_24x24x24x24_ls12_n16(gpu, c4):
         assert rel_l2(c4["y64"].download_rhs(r), c4["want"][r]) <= 1e-14
 
 
-@pytest.mark.parametrize("prec,form", [("f32", "valu"), ("f32", "tile"), ("bf16", "valu"), ("bf16", "mfma")])
+@pytest.mark.parametrize("prec,form", [("f32", "valu"), ("f32", "tile"), ("f32", "mfma"), ("bf16", "valu"),
+                                       ("bf16", "mfma")])
 def test_multi_rhs_24x24x24x24_ls12_n16(gpu, gbx, c4, tune_cache, prec, form):
     g, Ls, nrhs = c4["geom"], c4["Ls"], c4["nrhs"]
     U = gbx.GaugeField(g, prec).upload(c4["dbl"])

@@ -1,5 +1,6 @@
 """Matrix-core (MFMA) form of the multi-right-hand-side hopping term (kernel
-kind 1, FP16/BF16 storage, FP32 accumulation): every right-hand side against
+kind 1, FP16/BF16 storage, FP32 accumulation; the FP32-storage form is
+tested in test_gpu_mfma32.py): every right-hand side against
 the FP64 oracle (same bounds as the VALU kernel on 16-bit storage), and the
 whole output against the VALU kernel on the same 16-bit inputs.
 
@@ -86,8 +87,8 @@ def test_mfma_rejects_unsupported(gpu, gbx, problem):
         gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, kind=1))
     assert e.value.status == 8
     p = problem((4, 4, 4, 4), 16, nrhs=1)
-    geom, U, x, y = fields(gbx, p, "f32")
-    with pytest.raises(gbx.GbxError) as e:  # FP32 storage has no MFMA form here
+    geom, U, x, y = fields(gbx, p, "f64")
+    with pytest.raises(gbx.GbxError) as e:  # FP64 storage has no MFMA form
         gbx.dhop(U, y, x, cfg=gbx.kcfg(1, 64, kind=1))
     assert e.value.status == 8
     geom, U, x, y = fields(gbx, p, "f16")
@@ -114,7 +115,7 @@ def test_tuner_considers_mfma(gpu, gbx, problem, tmp_path):
 def test_tune_restricted_to_one_kind(gpu, gbx, problem, tmp_path):
     """gbx_tune_kinds: each restricted search returns a configuration of its
     own kind, both agree with the oracle, neither changes what gbx_dhop uses;
-    FP32 storage has no MFMA candidates (GBX_EUNSUPPORTED)."""
+    FP64 storage has no MFMA candidates (GBX_EUNSUPPORTED)."""
     p = problem((8, 4, 4, 8), 8, nrhs=2)
     geom, U, x, y = fields(gbx, p, "bf16")
     before = gbx.dhop_get_cfg(U, x).as_dict()
@@ -126,7 +127,7 @@ def test_tune_restricted_to_one_kind(gpu, gbx, problem, tmp_path):
         for r in range(2):
             assert rel_l2(got[r], p.oracle_lex(rhs=r)) <= storage_tol("bf16", p.psi)
     assert gbx.dhop_get_cfg(U, x).as_dict() == before
-    geom, U, x, y = fields(gbx, p, "f32")
+    geom, U, x, y = fields(gbx, p, "f64")
     with pytest.raises(gbx.GbxError) as e:
         gbx.tune(U, y, x, kinds="mfma")
     assert e.value.status == 8
