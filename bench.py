@@ -766,8 +766,11 @@ def main():
     stream_gbps = gbx.stream_copy_bw(4 << 30, 20)
     read3_gbps = gbx.stream_bw(6 << 30, 3, 20)
     read0_gbps = gbx.stream_bw(4 << 30, 0, 20)
-    achievable = max(stream_gbps, read3_gbps, read0_gbps)
-    log(f"HBM probes: copy {stream_gbps:.0f}, 3:1 read:write {read3_gbps:.0f}, read-only {read0_gbps:.0f} GB/s")
+    gather_gbps = gbx.gather_bw(2 << 30, 10)  # 256-byte runs at random places: the gathers' shape, from HBM
+    gather_l2_gbps = gbx.gather_bw(2 << 20, 10)  # the same from a buffer every XCD's L2 holds (not HBM)
+    achievable = max(stream_gbps, read3_gbps, read0_gbps, gather_gbps)
+    log(f"HBM probes: copy {stream_gbps:.0f}, 3:1 read:write {read3_gbps:.0f}, read-only {read0_gbps:.0f}, "
+        f"gathers of 256 B runs {gather_gbps:.0f} GB/s (from L2: {gather_l2_gbps:.0f} GB/s)")
 
     extra = {}
     if not args.no_variants:
@@ -847,7 +850,9 @@ def main():
                          "headline_rel_l2_vs_chiral_kernel": float(f"{err_h:.3e}")},
         "hbm_gbps_alg": round(achieved, 1),
         "achievable_hbm_gbps": {"copy": round(stream_gbps, 1), "read3_write1": round(read3_gbps, 1),
-                                "read_only": round(read0_gbps, 1), "max": round(achievable, 1)},
+                                "read_only": round(read0_gbps, 1), "gather_256B_runs": round(gather_gbps, 1),
+                                "max": round(achievable, 1)},
+        "l2_gather_256B_runs_gbps": round(gather_l2_gbps, 1),
         "frac_of_achievable_hbm": round(achieved / achievable, 3),
         "headline_check": {"rel_l2_vs_chiral_kernel": float(f"{err_h:.3e}"), "tol": HEADLINE_CHECK_TOL,
                            "passed": err_h <= HEADLINE_CHECK_TOL},

@@ -178,6 +178,7 @@ PROTOTYPES = {
     "gbx_batch_run_noalloc": (i32, [i32, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32]),
     "gbx_stream_copy_bw": (i32, [u64, i32, pdbl]),
     "gbx_stream_bw": (i32, [u64, i32, i32, pdbl]),
+    "gbx_gather_bw": (i32, [u64, i32, pdbl]),
 }
 
 for _name, (_res, _args) in PROTOTYPES.items():

@@ -1,7 +1,8 @@
 // Achievable-HBM-bandwidth probes: a grid-stride 16-byte copy kernel (the
 // "STREAM copy", also a plain copy of known size for calibrating byte
 // counters) and read-weighted variants (R reads per write, or reads only),
-// since the hopping kernels read several times what they write.
+// since the hopping kernels read several times what they write; and a
+// gather probe of the hopping kernels' access shape.
 #include "gbx_internal.h"
 
 namespace {
@@ -61,6 +62,34 @@ __global__ void __launch_bounds__(256) k_readmix(const u4 *__restrict__ src, u4 
   }
 }
 
+// Gather probe: each wavefront reads 4 runs of 256 contiguous bytes (16
+// lanes x 16 B) at pseudo-random 256-byte-aligned places of a buffer of
+// power-of-two size -- the access shape of a hopping kernel's neighbour
+// gather (one chunk of 4 sites x 16 minor indices) without its arithmetic.
+// U loads in flight per thread; each thread folds its loads into one value,
+// written once.  Every index is masked into the buffer.
+template <int U>
+__global__ void __launch_bounds__(256) k_gather_runs(const u4 *__restrict__ src, u4 *__restrict__ dst, uint64_t mask,
+                                                     int iters)
+{
+  const uint32_t lane = threadIdx.x & 63;
+  const uint64_t gid = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+  uint32_t h = (uint32_t)((gid >> 6) * 2654435761u) ^ 0x9e3779b9u;  // the same in every lane of a wavefront
+  u4 acc = {0u, 0u, 0u, 0u};
+  for (int it = 0; it < iters; it++) {
+    u4 v[U];
+#pragma unroll
+    for (int k = 0; k < U; k++) {
+      h = h * 1664525u + 1013904223u;
+      const uint64_t run = (uint64_t)(h >> 4) + (uint64_t)(lane >> 4) * 0x9E37u;
+      v[k] = src[(run * 16 + (lane & 15)) & mask];
+    }
+#pragma unroll
+    for (int k = 0; k < U; k++) acc ^= v[k];
+  }
+  dst[gid] = acc;
+}
+
 void launch_readmix(int mix, dim3 grid, hipStream_t st, const u4 *src, u4 *dst, uint64_t nout)
 {
   switch (mix) {
@@ -115,6 +144,55 @@ extern "C" int gbx_stream_bw(uint64_t read_bytes, int mix, int nrep, double *gbp
     float ms = 0.f;
     if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
     if (rc == GBX_OK) *gbps = (double)(read_bytes + out_bytes) * nrep / (ms * 1e-3) / 1e9;
+  }
+  if (st) (void)hipStreamSynchronize(st);
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (st) (void)hipStreamDestroy(st);
+  (void)hipFree(a);
+  (void)hipFree(b);
+  return rc;
+}
+
+extern "C" int gbx_gather_bw(uint64_t buffer_bytes, int nrep, double *gbps)
+{
+  GBX_CHECK(gbps, GBX_ENULL, "gbx_gather_bw: NULL result");
+  GBX_CHECK(buffer_bytes >= (1ull << 20) && (buffer_bytes & (buffer_bytes - 1)) == 0, GBX_EINVAL,
+            "gbx_gather_bw: buffer_bytes=%llu (a power of two, at least 1 MiB)", (unsigned long long)buffer_bytes);
+  GBX_CHECK(nrep >= 1, GBX_EINVAL, "gbx_gather_bw: nrep=%d", nrep);
+  GBX_TRY(gbx_require_device());
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  constexpr int U = 8, iters = 64;
+  const dim3 grid((unsigned)(ncu * 8));
+  const uint64_t nthreads = (uint64_t)grid.x * 256, per_launch = nthreads * iters * U * 16;
+  void *a = nullptr, *b = nullptr;
+  GBX_HIP(hipMalloc(&a, buffer_bytes));
+  if (hipMalloc(&b, nthreads * 16) != hipSuccess) {
+    (void)hipFree(a);
+    return gbx_fail(GBX_ENOMEM, "gbx_gather_bw: cannot allocate the result array");
+  }
+  int rc = GBX_OK;
+  hipStream_t st = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (hipStreamCreate(&st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipStreamCreate failed");
+  if (rc == GBX_OK && (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+    rc = gbx_fail(GBX_EHIP, "hipEventCreate failed");
+  if (rc == GBX_OK) {
+    const uint64_t mask = buffer_bytes / 16 - 1;
+    auto launch = [&]() {
+      hipLaunchKernelGGL((k_gather_runs<U>), grid, dim3(256), 0, st, (const u4 *)a, (u4 *)b, mask, iters);
+    };
+    if (hipMemsetAsync(a, 0x3c, buffer_bytes, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipMemsetAsync failed");
+    if (rc == GBX_OK) launch();  // warm-up
+    if (rc == GBX_OK && hipEventRecord(e0, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+    for (int r = 0; r < nrep && rc == GBX_OK; r++) launch();
+    if (rc == GBX_OK && hipEventRecord(e1, st) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventRecord failed");
+    if (rc == GBX_OK && hipEventSynchronize(e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "gather kernel failed");
+    float ms = 0.f;
+    if (rc == GBX_OK && hipEventElapsedTime(&ms, e0, e1) != hipSuccess) rc = gbx_fail(GBX_EHIP, "hipEventElapsedTime failed");
+    if (rc == GBX_OK) *gbps = (double)per_launch * nrep / (ms * 1e-3) / 1e9;
   }
   if (st) (void)hipStreamSynchronize(st);
   if (e0) (void)hipEventDestroy(e0);

@@ -904,6 +904,15 @@ def stream_bw(read_bytes=4 << 30, mix=3, nrep=20):
     return g.value
 
 
+def gather_bw(buffer_bytes=2 << 30, nrep=10):
+    """Read bandwidth (GB/s) of wavefront gathers of 4 random 256-byte runs
+    from a buffer of buffer_bytes (a power of two): 2 GiB measures HBM, a
+    few MiB the L2s."""
+    g = C.c_double()
+    check(lib.gbx_gather_bw(int(buffer_bytes), int(nrep), C.byref(g)), "gbx_gather_bw")
+    return g.value
+
+
 # ------------------------------------------------------------ work models
 def bytes_per_site(prec_spinor, prec_gauge, Ls, nrhs, comp12=False):
     """Algorithmic HBM bytes per (5D site, right-hand side): in + out spinor
@@ -923,7 +932,7 @@ __all__ = [
     "dslash_kernel", "dslash_last_path", "DslashContext", "Geometry", "GaugeField", "SpinorField", "kcfg", "pack_tile",
     "pack_march", "march_cfg",
     "dhop", "dhop_axpby",
-    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "batch_run_noalloc", "NOALLOC_MAX_LATTICES", "stream_copy_bw", "stream_bw", "bytes_per_site", "dhop_xpay",
+    "dhop_get_cfg", "dhop_time", "tune", "cg_solve", "Batch", "batch_run_noalloc", "NOALLOC_MAX_LATTICES", "stream_copy_bw", "stream_bw", "gather_bw", "bytes_per_site", "dhop_xpay",
     "cg_solve_eo", "rb_split", "rb_merge", "DWF_M5", "DWF_MASS", "dwf_mooee", "dwf_apply", "dwf_solve_eo",
     "static_data_write", "static_data_read_header", "static_data_read", "dhop_range", "halo_exchange", "part_split",
     "part_merge", "DeviceBuffer", "part_tables", "halo_overlap_time", "grid_coords", "halo_buffers",

@@ -596,6 +596,11 @@ int gbx_stream_copy_bw(uint64_t bytes, int nrep, double *gbps);
 /* read + write bytes.  A kernel that reads several times what it writes  */
 /* (Dhop reads about 2.7 B of HBM per byte written) is bounded by these.  */
 int gbx_stream_bw(uint64_t read_bytes, int mix, int nrep, double *gbps);
+/* Gather variant, the hopping kernels' access shape: every wavefront load */
+/* reads 4 runs of 256 contiguous bytes at pseudo-random places of a      */
+/* buffer of `buffer_bytes` (a power of two, >= 1 MiB).  A 2 GiB buffer    */
+/* measures HBM, a few MiB the L2s.  GB/s counts the bytes read.          */
+int gbx_gather_bw(uint64_t buffer_bytes, int nrep, double *gbps);
 
 #ifdef __cplusplus
 }

@@ -150,6 +150,18 @@ def test_halo_argument_errors(gbx):
     assert lib.gbx_dhop_range(None, None, None, 0, 0, 1) == 2
 
 
+def test_probe_argument_errors(gbx):
+    """The bandwidth probes check their arguments before touching a device."""
+    from gridbench_amd._lib import lib
+    g = C.c_double()
+    assert lib.gbx_gather_bw(1 << 20, 1, None) == 2
+    assert lib.gbx_gather_bw(3 << 20, 1, C.byref(g)) == 1      # not a power of two
+    assert lib.gbx_gather_bw(1 << 19, 1, C.byref(g)) == 1      # below 1 MiB
+    assert lib.gbx_gather_bw(1 << 20, 0, C.byref(g)) == 1
+    assert lib.gbx_stream_bw(1 << 20, 5, 1, C.byref(g)) == 1   # 5 reads per write is not offered
+    assert lib.gbx_stream_bw(1 << 20, 0, 1, None) == 2
+
+
 @pytest.mark.skipif(__import__("gridbench_amd").device_count() > 0, reason="host has a GPU")
 def test_no_device_is_reported_not_fatal(gbx):
     from gridbench_amd._lib import lib
@@ -157,6 +169,7 @@ def test_no_device_is_reported_not_fatal(gbx):
     assert lib.gbx_geom_create((C.c_int * 4)(4, 4, 4, 4), None, C.byref(h)) == 7
     g = C.c_double()
     assert lib.gbx_stream_copy_bw(1 << 20, 1, C.byref(g)) == 7
+    assert lib.gbx_gather_bw(1 << 20, 1, C.byref(g)) == 7
     U, out, inp, nbr, nsite, Ls, prm, nsimd = drop_in_args(gbx)
     with pytest.raises(gbx.GbxError) as e:
         gbx.dslash_kernel(1, U, out, inp, nbr, nsite, Ls, prm, nsimd)
