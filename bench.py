@@ -827,6 +827,9 @@ def main():
         roof_extra[f"{name}_frac"] = r["frac"]
         roof_extra[f"{name}_traffic_ratio"] = r["traffic_ratio"]
     t64 = traffic_from_profiles()
+    if t64:  # the memory-side rate the counters imply for this run's launch time, against the largest probe
+        roof_extra["traffic_gbps"] = round(t64 / (us_ev * 1e-6) / 1e9, 1)
+        roof_extra["traffic_frac_of_achievable"] = round(t64 / (us_ev * 1e-6) / 1e9 / achievable, 3)
 
     result = {
         "metric": "DWF Dhop GFLOP/s + HBM GB/s @24^4xLs16: FP64 1-RHS vs FP32/FP16 16-RHS",
