@@ -339,12 +339,19 @@ __device__ __forceinline__ void leg_load(const void *__restrict__ in, uint32_t n
   else load_nb<P>(in, nb, M, m, v);
 }
 
+// The leg arithmetic runs at raised wave priority (s_setprio 1), the gathers
+// at the normal one: a wavefront holding a gathered leg finishes it and
+// issues its next gathers sooner, which evens out the stream of gathers
+// (FP64: about 1 % on TDIAG fields, 1.7 % on chiral fields; the arithmetic
+// and its results are unchanged).
 template <bool DAG, int D, bool TD, bool FIRST, int MUL, typename A>
 __device__ __forceinline__ void leg_apply(const A *v, const A *U, A *rr, A *ri)
 {
   static_assert(!(FIRST && LegHalf<DAG, D, TD>::value), "a half leg never comes first");
+  __builtin_amdgcn_s_setprio(1);
   if constexpr (LegHalf<DAG, D, TD>::value) leg_half<LegHalf<DAG, D, TD>::upper, MUL>(v, U, rr, ri);
   else leg_compute<LegKind<DAG, D, TD>::value, FIRST, MUL>(v, U, rr, ri);
+  __builtin_amdgcn_s_setprio(0);
 }
 
 // Full 8-leg hopping term for (site with links U, neighbours nb[8], minor m).
