@@ -326,16 +326,18 @@ __global__ void __launch_bounds__(256)
   };
   // 12 projected reals h[2 (3a + c) + re/im] of one leg from its chunks.
   auto project = [&](const Leg32 &p, const f4 *c, float *h) {
+    if (p.half) {  // only its 3 chunks were gathered
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) h[4 * k + j] = c[k][j];
+      return;
+    }
     float v[24];
 #pragma unroll
     for (int k = 0; k < 6; k++)
 #pragma unroll
       for (int j = 0; j < 4; j++) v[4 * k + j] = c[k][j];
-    if (p.half) {
-#pragma unroll
-      for (int j = 0; j < 12; j++) h[j] = v[j];
-      return;
-    }
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
