@@ -259,6 +259,206 @@ void launch_p(bool dag, bool c12, dim3 grid, dim3 block, size_t lds, hipStream_t
   else hipLaunchKernelGGL((k_dhop_tile<P, true, true, TD>), grid, block, lds, st, a);
 }
 
+// ------------------------------------------------------------ async form
+// Persistent variant (kind 2 with pf = 1; FP64 / FP32, 18-real links): each
+// workgroup walks a strided sequence of (tile, minor chunk) units with two
+// LDS buffers.  While it computes unit k from one buffer, the spinors and
+// links of unit k+1 travel into the other by direct global -> LDS loads
+// (global_load_lds_dwordx4), which hold no registers.  Those loads are L1
+// requests like any other, though, and the L1's in-flight capacity, not
+// VGPRs, bounds these kernels: with half the LDS per box (two buffers) this
+// form issues more L2 requests than the one-shot kernel and measures slower
+// on the 16-RHS FP32 problem (7.3 against 5.3 ms at 24^4 x Ls16).  It is a
+// tuner option.  Units are chunk-major and each XCD takes a contiguous eighth
+// of them, its workgroups taking consecutive units (so the tiles in flight on
+// an XCD are neighbours); with 8 minor chunks an XCD works on its own chunk of
+// the whole lattice.  LDS per buffer: spinors [T][NCH][mc] (minor fastest:
+// a wavefront's 64 lanes fill 64 consecutive 16-byte items, as the
+// instruction requires) and links [T][8][18] (the per-slot global layout,
+// copied as is); each area is padded to whole wavefronts of items.  Same
+// loads, same arithmetic, same leg order as k_dhop: bit-identical results.
+struct AsyncArgs {
+  const void *gauge;
+  const uint32_t *nbr, *tslots;
+  const void *in;
+  void *out;
+  uint32_t ntiles, M, mc, nunits, units_per_xcd, wg_per_xcd;
+  int ex, ey, ez, et, wrap, nt;
+};
+
+// 16-byte items of one buffer: spinor and link areas, each a multiple of 64.
+struct AsyncLayout {
+  uint32_t nspin, nlink, spin_pad, link_pad;
+};
+inline __host__ __device__ AsyncLayout async_layout(int T, int nch, int link_items, uint32_t mc)
+{
+  AsyncLayout l;
+  l.nspin = (uint32_t)T * (uint32_t)nch * mc;
+  l.nlink = (uint32_t)T * (uint32_t)link_items;
+  l.spin_pad = (l.nspin + 63) / 64 * 64;
+  l.link_pad = (l.nlink + 63) / 64 * 64;
+  return l;
+}
+
+// Launch bound: at 512 threads FP64 spills (24 accumulation doubles per item).
+constexpr int async_max_threads(int prec) { return prec == GBX_F64 ? 256 : 512; }
+
+// A leg's chunks from an LDS area whose chunks lie `stride` items apart.
+template <int P, bool DAG, int D, bool TD, typename A, typename V>
+__device__ __forceinline__ void lds_leg_load_s(const V *p, uint32_t stride, A *v)
+{
+  constexpr int NCH = dev::Traits<P>::NCH, R = dev::Traits<P>::R;
+  constexpr bool half = dev::LegHalf<DAG, D, TD>::value;
+  constexpr int K0 = (half && !dev::LegHalf<DAG, D, TD>::upper) ? NCH / 2 : 0, N = half ? NCH / 2 : NCH;
+#pragma unroll
+  for (int k = 0; k < N; k++) dev::load_chunk<P, A>(p, (uint64_t)(K0 + k) * stride, v + k * R);
+}
+
+// Start unit u's loads into one buffer (spinor area ds, link area du, slot
+// and neighbour tables dq, dn): spinors and links asynchronously (lanes past
+// an area's end load its last item into the padding), the tables
+// synchronously.
+// One wavefront-wide direct load: lane l's 16 bytes at src go to LDS at
+// dst + 16 l (dst uniform over the wavefront).  Kept out of templates: the
+// host pass rejects the builtin in a dependent context.
+__device__ __forceinline__ void lds_dma16(const void *src, void *dst)
+{
+  typedef __attribute__((address_space(3))) void lds_void;
+  __builtin_amdgcn_global_load_lds(src, (lds_void *)dst, 16, 0, 0);
+}
+
+template <int P, typename V>
+__device__ __forceinline__ void async_issue(const AsyncArgs &a, const AsyncLayout &lay, int T, uint32_t u, V *ds, V *du,
+                                            uint32_t *dq, uint32_t *dn)
+{
+  constexpr int NCH = dev::Traits<P>::NCH;
+  constexpr int LV = 8 * 18 * (int)sizeof(typename dev::Traits<P>::acc) / 16;
+  const uint32_t nthr = blockDim.x, lane = threadIdx.x & 63u, wave = threadIdx.x >> 6, nwave = nthr >> 6;
+  const uint32_t mc = a.mc, tile = u % a.ntiles, m0 = (u / a.ntiles) * mc;
+  const uint32_t *ts = a.tslots + (uint64_t)tile * T;
+  for (uint32_t it0 = wave * 64; it0 < lay.spin_pad; it0 += nwave * 64) {
+    const uint32_t it = min(it0 + lane, lay.nspin - 1);
+    const uint32_t m = it % mc, ik = it / mc, k = ik % NCH, i = ik / NCH;
+    const V *src = (const V *)a.in + ((uint64_t)ts[i] * NCH + k) * a.M + m0 + m;
+    lds_dma16(src, ds + it0);
+  }
+  for (uint32_t it0 = wave * 64; it0 < lay.link_pad; it0 += nwave * 64) {
+    const uint32_t it = min(it0 + lane, lay.nlink - 1);
+    const uint32_t i = it / LV, r = it - i * LV;
+    const V *src = (const V *)a.gauge + (uint64_t)ts[i] * LV + r;
+    lds_dma16(src, du + it0);
+  }
+  for (uint32_t w = threadIdx.x; w < (uint32_t)T * 9; w += nthr) {
+    if (w < (uint32_t)T) {
+      dq[w] = ts[w];
+    } else {
+      const uint32_t t = w - T;
+      dn[t] = a.nbr[(uint64_t)ts[t >> 3] * 8 + (t & 7)];
+    }
+  }
+}
+
+template <int P, bool DAG, bool TD>
+__global__ void __launch_bounds__(async_max_threads(P)) k_dhop_tile_async(AsyncArgs a)
+{
+  typedef typename dev::Traits<P>::acc A;
+  typedef typename dev::Traits<P>::vec V;
+  constexpr int NCH = dev::Traits<P>::NCH;
+  constexpr int LV = 8 * 18 * (int)sizeof(A) / 16;  // 16-byte link items per site
+  constexpr int CH = dev::MUL_CHAIN;
+  const int e[4] = {a.ex, a.ey, a.ez, a.et};
+  const int T = e[0] * e[1] * e[2] * e[3];
+  const uint32_t mc = a.mc;
+  const AsyncLayout lay = async_layout(T, NCH, LV, mc);
+  extern __shared__ __align__(16) unsigned char smem[];
+  V *sIn = (V *)smem;                                  // [2][spin_pad]
+  V *sUv = sIn + 2 * (size_t)lay.spin_pad;             // [2][link_pad]
+  uint32_t *sQ = (uint32_t *)(sUv + 2 * (size_t)lay.link_pad);  // [2][T]
+  uint32_t *sN = sQ + 2 * T;                           // [2][T][8]
+  int *sL = (int *)(sN + 2 * 8 * T);                   // [T][8]
+
+  const uint32_t nthr = blockDim.x;
+  const uint32_t x = blockIdx.x & 7u, j = blockIdx.x >> 3;
+  const uint32_t base = x * a.units_per_xcd;
+  const uint32_t end = min(base + a.units_per_xcd, a.nunits);
+  if (base + j >= end) return;  // uniform over the workgroup, before any barrier
+
+  // Leg d's neighbour inside the box (wrapping only along an extent that
+  // spans the lattice), or -1: the same for every tile.
+  for (uint32_t w = threadIdx.x; w < (uint32_t)T * 8; w += nthr) {
+    const int i = (int)(w >> 3), d = (int)(w & 7), mu = d & 3;
+    int stride = 1;
+    for (int k = 0; k < mu; k++) stride *= e[k];
+    const int c = (i / stride) % e[mu];
+    int c2 = c + (d < 4 ? 1 : -1);
+    bool inside = c2 >= 0 && c2 < e[mu];
+    if (!inside && ((a.wrap >> mu) & 1)) {
+      c2 = (c2 + e[mu]) % e[mu];
+      inside = true;
+    }
+    sL[w] = inside ? i + (c2 - c) * stride : -1;
+  }
+
+  async_issue<P>(a, lay, T, base + j, sIn, sUv, sQ, sN);
+  __builtin_amdgcn_s_waitcnt(0);  // this thread's loads, the direct-to-LDS ones included, have landed
+  __syncthreads();
+  const uint32_t items = (uint32_t)T * mc;
+  for (uint32_t k = 0;; k++) {
+    const uint32_t b = k & 1u, u = base + j + k * a.wg_per_xcd, un = u + a.wg_per_xcd;
+    const bool next = un < end;
+    if (next) async_issue<P>(a, lay, T, un, sIn + (b ^ 1u) * lay.spin_pad, sUv + (b ^ 1u) * lay.link_pad,
+                             sQ + (b ^ 1u) * T, sN + (b ^ 1u) * 8 * T);
+    const uint32_t m0 = (u / a.ntiles) * mc;
+    const V *cin = sIn + b * lay.spin_pad;
+    const A *cU = (const A *)(sUv + b * lay.link_pad);
+    const uint32_t *cQ = sQ + b * T, *cN = sN + b * 8 * T;
+    for (uint32_t w = threadIdx.x; w < items; w += nthr) {
+      const int i = (int)(w / mc);
+      const uint32_t ml = w - (uint32_t)i * mc, m = m0 + ml;
+      A rr[12], ri[12];
+      A v[24];
+#define GBX_ALEG(D, FIRST)                                                                                             \
+      {                                                                                                                \
+        const int ln = sL[i * 8 + (D)];                                                                                \
+        if (ln >= 0) {                                                                                                 \
+          lds_leg_load_s<P, DAG, D, TD>(cin + ((uint32_t)ln * NCH) * mc + ml, mc, v);                                  \
+          dev::leg_apply<DAG, D, TD, FIRST, CH>(v, cU + (i * 8 + D) * 18, rr, ri);                                     \
+        } else {                                                                                                       \
+          dev::leg_load<P, DAG, D, TD>(a.in, cN[i * 8 + D], a.M, m, v);                                                \
+          dev::leg_apply<DAG, D, TD, FIRST, CH>(v, cU + (i * 8 + D) * 18, rr, ri);                                     \
+        }                                                                                                              \
+      }
+      GBX_ALEG(0, true)
+      GBX_ALEG(1, false)
+      GBX_ALEG(2, false)
+      GBX_ALEG(3, false)
+      GBX_ALEG(4, false)
+      GBX_ALEG(5, false)
+      GBX_ALEG(6, false)
+      GBX_ALEG(7, false)
+#undef GBX_ALEG
+      dev::store_site<P, false>(a.out, nullptr, cQ[i], a.M, m, rr, ri, (A)0, (A)1, a.nt != 0);
+    }
+    __builtin_amdgcn_s_waitcnt(0);  // the next unit's loads have landed (this thread's)
+    __syncthreads();                // ... and every thread's; this buffer is free again
+    if (!next) break;
+  }
+}
+
+template <int P, bool TD>
+void launch_async_p(bool dag, dim3 grid, dim3 block, size_t lds, hipStream_t st, const AsyncArgs &a)
+{
+  if (dag) hipLaunchKernelGGL((k_dhop_tile_async<P, true, TD>), grid, block, lds, st, a);
+  else hipLaunchKernelGGL((k_dhop_tile_async<P, false, TD>), grid, block, lds, st, a);
+}
+
+size_t tile_async_lds(int prec, int T, uint32_t mc)
+{
+  const int nch = prec == GBX_F64 ? 12 : 6, lv = prec == GBX_F64 ? 72 : 36;
+  const AsyncLayout l = async_layout(T, nch, lv, mc);
+  return 2 * (size_t)(l.spin_pad + l.link_pad) * 16 + 2 * (size_t)T * 9 * 4 + (size_t)T * 8 * 4;
+}
+
 size_t tile_lds(int prec, int T, int mc)
 {
   const size_t acc = prec == GBX_F64 ? 8 : 4;
@@ -287,12 +487,72 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
     GBX_CHECK(s.e[d] >= 1 && g->dims[d] % s.e[d] == 0, GBX_EINVAL, "tile kernel: tile extent %d = %d does not divide %d", d,
               s.e[d], g->dims[d]);
   const uint32_t mc = (k.mc <= 0 || (uint32_t)k.mc > in->M) ? in->M : (uint32_t)k.mc;
+  GBX_CHECK(k.pf == 0 || k.pf == 1, GBX_EINVAL, "tile kernel: pf=%d (0: one-shot, 1: persistent with async staging)",
+            k.pf);
+  GBX_CHECK(k.wgpc >= 0 && k.wgpc <= 64, GBX_EINVAL, "tile kernel: wgpc=%d", k.wgpc);
+  if (k.pf == 1) {
+    GBX_CHECK((u->prec == GBX_F64 || u->prec == GBX_F32) && !u->comp12, GBX_EUNSUPPORTED,
+              "tile kernel (async staging): FP64 / FP32 fields with 18-real links");
+    GBX_CHECK(in->M % mc == 0, GBX_EINVAL, "tile kernel (async staging): mc=%u must divide Ls*nrhs=%u", mc, in->M);
+    GBX_CHECK(k.threads >= 64 && k.threads <= async_max_threads(u->prec) && k.threads % 64 == 0, GBX_EINVAL,
+              "tile kernel (async staging): threads=%d (max %d)", k.threads, async_max_threads(u->prec));
+    const size_t lds = tile_async_lds(u->prec, s.T(), mc);
+    GBX_CHECK(lds <= 160 * 1024, GBX_EINVAL, "tile kernel (async staging): tile of %d sites x %u minor needs %zu B of LDS",
+              s.T(), mc, lds);
+    return GBX_OK;
+  }
   const int tmax = tile_max_threads(u->prec, in->basis == GBX_BASIS_TDIAG);  // the kernel's launch bound
   GBX_CHECK(k.threads >= 64 && k.threads <= tmax && k.threads % 64 == 0, GBX_EINVAL, "tile kernel: threads=%d (max %d)",
             k.threads, tmax);
-  GBX_CHECK(k.wgpc >= 0 && k.wgpc <= 64, GBX_EINVAL, "tile kernel: wgpc=%d", k.wgpc);
   const size_t lds = tile_lds(u->prec, s.T(), (int)mc);
   GBX_CHECK(lds <= 160 * 1024, GBX_EINVAL, "tile kernel: tile of %d sites x %u minor needs %zu B of LDS", s.T(), mc, lds);
+  return GBX_OK;
+}
+
+// The persistent form: one workgroup per (CU, wgpc) slot, all resident at
+// once; XCD x's workgroups share its eighth of the units.
+static int tile_async_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k,
+                             hipStream_t st)
+{
+  gbx_geom_s *g = u->geom;
+  const TileShape s = unpack_tile(k.tile);
+  AsyncArgs a;
+  GBX_TRY(tile_table(g, k.tile, &a.tslots, &a.ntiles));
+  a.gauge = u->d;
+  a.nbr = g->nbr;
+  a.in = in->d;
+  a.out = out->d;
+  a.M = in->M;
+  a.mc = (k.mc <= 0 || (uint32_t)k.mc > in->M) ? in->M : (uint32_t)k.mc;
+  const uint64_t nunits = (uint64_t)a.ntiles * (a.M / a.mc);
+  GBX_CHECK(nunits < (1ull << 31), GBX_EINVAL, "tile kernel: %llu units", (unsigned long long)nunits);
+  a.nunits = (uint32_t)nunits;
+  a.units_per_xcd = (a.nunits + 7) / 8;
+  int dev = 0, ncu = 256;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const size_t lds = tile_async_lds(u->prec, s.T(), a.mc);
+  const int fit = (int)((size_t)160 * 1024 / lds);
+  const int per_cu = k.wgpc > 0 ? std::min(k.wgpc, fit) : fit;
+  a.wg_per_xcd = (uint32_t)std::max(1, ncu / 8 * per_cu);
+  a.ex = s.e[0];
+  a.ey = s.e[1];
+  a.ez = s.e[2];
+  a.et = s.e[3];
+  a.wrap = 0;
+  for (int d = 0; d < 4; d++)
+    if (s.e[d] == g->dims[d]) a.wrap |= 1 << d;
+  a.nt = k.nt;
+  const dim3 grid(8 * a.wg_per_xcd), block((unsigned)k.threads);
+  const bool td = in->basis == GBX_BASIS_TDIAG;
+  if (u->prec == GBX_F64) {
+    if (td) launch_async_p<GBX_F64, true>(dag != 0, grid, block, lds, st, a);
+    else launch_async_p<GBX_F64, false>(dag != 0, grid, block, lds, st, a);
+  } else {
+    if (td) launch_async_p<GBX_F32, true>(dag != 0, grid, block, lds, st, a);
+    else launch_async_p<GBX_F32, false>(dag != 0, grid, block, lds, st, a);
+  }
+  GBX_HIP(hipGetLastError());
   return GBX_OK;
 }
 
@@ -300,6 +560,7 @@ int tile_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in,
 {
   GBX_TRY(tile_cfg_check(k, u, in));
   out->basis = in->basis;
+  if (k.pf == 1) return tile_async_launch(u, out, in, dag, k, st);
   gbx_geom_s *g = u->geom;
   const TileShape s = unpack_tile(k.tile);
   TileArgs a;

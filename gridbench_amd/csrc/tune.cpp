@@ -93,6 +93,19 @@ std::vector<gbx_kcfg> shape_candidates(const gbx_gauge_s *u, const gbx_spinor_s 
           v.push_back(k);
         }
       }
+    // Persistent form with async staging: minor chunks dividing M,
+    // workgroups of up to one item per thread.
+    for (const auto &e : shapes)
+      for (int mc : minor_chunks(M)) {
+        const uint32_t mw = mc ? (uint32_t)mc : M;
+        const uint64_t items = (uint64_t)e[0] * e[1] * e[2] * e[3] * mw;
+        for (int t : {128, 256, 512}) {
+          if ((uint64_t)t > (items + 63) / 64 * 64 && t != 128) continue;
+          gbx_kcfg k{0, t, 1, 1, 2, mc, 1, 0, 0, pack_tile(e)};
+          if (tile_cfg_check(k, u, in) != GBX_OK) continue;
+          v.push_back(k);
+        }
+      }
     gbx_clear_error();
   }
   if (kinds & GBX_KIND_MARCH) {

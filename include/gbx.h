@@ -314,7 +314,12 @@ typedef struct {
                /* 18-real links)                                         */
   int mc;      /* minor indices (s, rhs) per workgroup; 0 = all Ls*nrhs  */
   int pf;      /* 1 (2): gather of leg d+1 (d+2) issued before leg d is  */
-               /* computed; VALU kernel, workgroups of <= 256 threads    */
+               /* computed; VALU kernel, workgroups of <= 256 threads.   */
+               /* Kind 2: 1 = persistent form, the next (box, minor      */
+               /* chunk)'s spinors and links staged by direct global-to- */
+               /* LDS loads while the current one computes (FP64 / FP32, */
+               /* 18-real links, mc dividing Ls*nrhs, <= 256 / 512       */
+               /* threads)                                               */
   int wgpc;    /* cap on resident workgroups per CU (0 = none): the LDS  */
                /* request is padded to 160 KiB / wgpc, which shrinks the */
                /* set of sites in flight per XCD to what its L2 holds    */
