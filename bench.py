@@ -90,16 +90,25 @@ def log(msg):
 # chiral kernel (itself pinned to the oracle at <= 1e-14 by the tests).
 HEADLINE_CHECK_TOL = 1e-13
 # rocprofv3 PMC summaries (TCC read/write requests, copy-calibrated) of the
-# kernels behind the headline and the 16-RHS rows.
-PROFILE_OF = {"f64_1rhs": "dhop_f64_24x24x24x24_ls16_pmc.md",
-              "f32_16rhs": "dhop_tile_f32_24x24x24x24_ls16_n16_stats_pmc.md",
-              "f16_16rhs": "dhop_mfma_f16_bf16_24x24x24x24_n16_stats_pmc.md"}
+# kernels behind the headline and the 16-RHS rows, with the launch
+# configuration each one measured: a row's traffic figure is used only when
+# the row ran that configuration.
+PROFILE_OF = {"f64_1rhs": ("dhop_f64_24x24x24x24_ls16_pmc.md",
+                           {"kind": 0, "spw": 4, "threads": 64, "mc": 0, "pf": 2, "tile": 0}),
+              "f32_16rhs": ("dhop_tile_f32_24x24x24x24_ls16_n16_stats_pmc.md",
+                            {"kind": 2, "threads": 768, "mc": 64, "pf": 0, "tile": 0x2223}),
+              "f16_16rhs": ("dhop_mfma_f16_bf16_24x24x24x24_n16_stats_pmc.md",
+                            {"kind": 1, "spw": 4, "threads": 256, "mc": 0, "pf": 0, "tile": 0})}
 
 
-def traffic_from_profiles(row="f64_1rhs"):
+def traffic_from_profiles(row, cfg):
     """HBM bytes per launch of a row's kernel, from its committed rocprofv3
-    PMC summary (the `traffic_bytes_per_launch:` line), or None."""
-    path = os.path.join(ROOT, "profiles", PROFILE_OF[row])
+    PMC summary (the `traffic_bytes_per_launch:` line); None when the row ran
+    another configuration than the profiled one, or the summary is missing."""
+    name, profiled = PROFILE_OF[row]
+    if any(cfg.get(k) != v for k, v in profiled.items()):
+        return None
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
@@ -816,7 +825,7 @@ def main():
     rows16 = {}
     for v in extra.get("variants", []):
         if v["name"] in ("f32_16rhs", "f16_16rhs"):
-            t = traffic_from_profiles(v["name"])
+            t = traffic_from_profiles(v["name"], v["cfg"])
             alg = gbx.bytes_per_site(v["prec"], v["prec"], v["Ls"], v["nrhs"], v["comp12"]) * sites5 * v["nrhs"]
             rows16[v["name"]] = {"us": v["us_per_launch"], "gflops": v["gflops"], "hbm_gbps_alg": v["hbm_gbps_alg"],
                                  "frac": round(v["hbm_gbps_alg"] / HBM_PEAK_GBPS, 4),
@@ -837,7 +846,7 @@ def main():
             cfg_extra[f"{name}_{k}"] = r[k]
         roof_extra[f"{name}_frac"] = r["frac"]
         roof_extra[f"{name}_traffic_ratio"] = r["traffic_ratio"]
-    t64 = traffic_from_profiles()
+    t64 = traffic_from_profiles("f64_1rhs", best.as_dict())
     if t64:  # the memory-side rate the counters imply for this run's launch time, against the largest probe
         roof_extra["traffic_gbps"] = round(t64 / (us_ev * 1e-6) / 1e9, 1)
         roof_extra["traffic_frac_of_achievable"] = round(t64 / (us_ev * 1e-6) / 1e9 / achievable, 3)
