@@ -445,11 +445,16 @@ __global__ void __launch_bounds__(async_max_threads(P)) k_dhop_tile_async(AsyncA
   }
 }
 
-template <int P, bool TD>
-void launch_async_p(bool dag, dim3 grid, dim3 block, size_t lds, hipStream_t st, const AsyncArgs &a)
+typedef void (*AsyncKernel)(AsyncArgs);
+
+AsyncKernel async_kernel(int prec, bool dag, bool td)
 {
-  if (dag) hipLaunchKernelGGL((k_dhop_tile_async<P, true, TD>), grid, block, lds, st, a);
-  else hipLaunchKernelGGL((k_dhop_tile_async<P, false, TD>), grid, block, lds, st, a);
+  if (prec == GBX_F64) {
+    if (td) return dag ? k_dhop_tile_async<GBX_F64, true, true> : k_dhop_tile_async<GBX_F64, false, true>;
+    return dag ? k_dhop_tile_async<GBX_F64, true, false> : k_dhop_tile_async<GBX_F64, false, false>;
+  }
+  if (td) return dag ? k_dhop_tile_async<GBX_F32, true, true> : k_dhop_tile_async<GBX_F32, false, true>;
+  return dag ? k_dhop_tile_async<GBX_F32, true, false> : k_dhop_tile_async<GBX_F32, false, false>;
 }
 
 size_t tile_async_lds(int prec, int T, uint32_t mc)
@@ -509,8 +514,10 @@ int tile_cfg_check(const gbx_kcfg &k, const gbx_gauge_s *u, const gbx_spinor_s *
   return GBX_OK;
 }
 
-// The persistent form: one workgroup per (CU, wgpc) slot, all resident at
-// once; XCD x's workgroups share its eighth of the units.
+// The persistent form: as many workgroups per CU as can be resident at once
+// (registers, LDS; at most wgpc), so that the static split of the units
+// leaves no workgroup waiting for a slot; XCD x's workgroups share its eighth
+// of the units.
 static int tile_async_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_spinor_s *in, int dag, const gbx_kcfg &k,
                              hipStream_t st)
 {
@@ -528,13 +535,17 @@ static int tile_async_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_
   GBX_CHECK(nunits < (1ull << 31), GBX_EINVAL, "tile kernel: %llu units", (unsigned long long)nunits);
   a.nunits = (uint32_t)nunits;
   a.units_per_xcd = (a.nunits + 7) / 8;
-  int dev = 0, ncu = 256;
-  (void)hipGetDevice(&dev);
-  (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+  const bool td = in->basis == GBX_BASIS_TDIAG;
+  const AsyncKernel kern = async_kernel(u->prec, dag != 0, td);
   const size_t lds = tile_async_lds(u->prec, s.T(), a.mc);
-  const int fit = (int)((size_t)160 * 1024 / lds);
+  int dev = 0, ncu = 0, fit = 0;
+  GBX_HIP(hipGetDevice(&dev));
+  GBX_HIP(hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev));
+  GBX_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&fit, kern, k.threads, lds));
+  GBX_CHECK(fit >= 1 && ncu >= 8, GBX_EINVAL, "tile kernel (async staging): %d resident workgroups per CU, %d CUs", fit,
+            ncu);
   const int per_cu = k.wgpc > 0 ? std::min(k.wgpc, fit) : fit;
-  a.wg_per_xcd = (uint32_t)std::max(1, ncu / 8 * per_cu);
+  a.wg_per_xcd = (uint32_t)(ncu / 8 * per_cu);
   a.ex = s.e[0];
   a.ey = s.e[1];
   a.ez = s.e[2];
@@ -544,14 +555,7 @@ static int tile_async_launch(const gbx_gauge_s *u, gbx_spinor_s *out, const gbx_
     if (s.e[d] == g->dims[d]) a.wrap |= 1 << d;
   a.nt = k.nt;
   const dim3 grid(8 * a.wg_per_xcd), block((unsigned)k.threads);
-  const bool td = in->basis == GBX_BASIS_TDIAG;
-  if (u->prec == GBX_F64) {
-    if (td) launch_async_p<GBX_F64, true>(dag != 0, grid, block, lds, st, a);
-    else launch_async_p<GBX_F64, false>(dag != 0, grid, block, lds, st, a);
-  } else {
-    if (td) launch_async_p<GBX_F32, true>(dag != 0, grid, block, lds, st, a);
-    else launch_async_p<GBX_F32, false>(dag != 0, grid, block, lds, st, a);
-  }
+  hipLaunchKernelGGL(kern, grid, block, lds, st, a);
   GBX_HIP(hipGetLastError());
   return GBX_OK;
 }
