@@ -267,8 +267,9 @@ int gbx_spinor_fill_gaussian(gbx_spinor x, uint64_t seed);
 /* Dhop is basis-covariant (out takes the input's basis); the result, set  */
 /* back to the chiral basis, agrees with the chiral kernel's to rounding.   */
 /* No counterpart in the reference (its kernel stores chiral spinors).      */
-/* TDIAG fields: FP64 / FP32; accepted by gbx_dhop* and gbx_tune* (VALU   */
-/* and LDS-tiled forms), gbx_batch_*, gbx_cg_solve (FP32 inner),           */
+/* TDIAG fields: FP64 / FP32; accepted by gbx_dhop* and gbx_tune* (VALU,  */
+/* LDS-tiled, slab-marching and FP32 MFMA forms), gbx_batch_*,             */
+/* gbx_cg_solve (FP32 inner),                                              */
 /* gbx_cg_solve_f64 and gbx_cg_solve_eo (FP32 inner or none),              */
 /* the norm / dot / axpy / scale / zero / copy routines and the whole-face  */
 /* halo routines (a face is copied as stored: sending and receiving parts  */
@@ -359,8 +360,9 @@ int gbx_dhop_get_cfg(gbx_gauge u, gbx_spinor in, gbx_kcfg *cfg);
 int gbx_dhop_time(gbx_gauge u, gbx_spinor out, gbx_spinor in, int dag, const gbx_kcfg *cfg, int nrep,
                   double *usec);
 
-/* Autotuner: times candidate configurations (sites per workgroup,         */
-/* workgroup size, XCD ordering, store kind, VALU vs MFMA) on these        */
+/* Autotuner: times candidate configurations (kernel kind: VALU, MFMA,    */
+/* LDS-tiled, slab-marching; sites per workgroup, workgroup size, minor    */
+/* chunk, box, XCD ordering, store kind, residency cap) on these           */
 /* fields and records the fastest for this (precision, comp12, Ls, nrhs,   */
 /* geometry) signature; gbx_dhop uses it from then on.  Optional cache     */
 /* file (JSON lines) is read first and appended to (may be NULL).          */
